@@ -1,0 +1,165 @@
+"""Pins the CPU oracle: golden vectors from the reference + closed forms.
+
+These tests run on the CPU only.  They check the checker: a C restatement
+that drifts from the reference (a dropped term, a swapped direction, a wrong
+boundary order) fails one of the composite-step cases below, each of which
+exercises a different collision/boundary combination.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests import _golden as G
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    orc.build_library()
+
+
+# ----------------------------------------------------------------- lattices
+@pytest.mark.parametrize("E,W,opp", [(orc.E9, orc.W9, orc.OPP9), (orc.E19, orc.W19, orc.OPP19)])
+def test_lattice_moments(E, W, opp):
+    """Isotropy: sum w = 1, sum w e = 0, sum w e_a e_b = delta/3, opp reverses e."""
+    assert abs(W.sum() - 1.0) < 1e-15
+    np.testing.assert_allclose(W @ E, 0.0, atol=1e-15)
+    second = np.einsum("i,ia,ib->ab", W, E, E)
+    np.testing.assert_allclose(second, np.eye(E.shape[1]) / 3.0, atol=1e-15)
+    np.testing.assert_array_equal(E[opp], -E)
+
+
+def test_feq_moments():
+    rng = np.random.default_rng(0)
+    rho = 1.0 + 0.01 * rng.standard_normal((5, 6))
+    ux, uy = 0.05 * rng.standard_normal((2, 5, 6))
+    f = orc.feq(orc.E9, orc.W9, rho, ux, uy)
+    np.testing.assert_allclose(f.sum(0), rho, rtol=1e-14)
+    np.testing.assert_allclose(np.einsum("i,iyx->yx", orc.E9[:, 0], f), rho * ux, atol=1e-15)
+    np.testing.assert_allclose(np.einsum("i,iyx->yx", orc.E9[:, 1], f), rho * uy, atol=1e-15)
+
+
+def test_mrt_inverses_match_reference():
+    with np.load(G.GOLDEN / "mrt_inverses.npz") as d:
+        np.testing.assert_allclose(orc.M9_INV, d["M9_inv"], rtol=0, atol=1e-15)
+        np.testing.assert_allclose(orc.M19_INV, d["M19_inv"], rtol=0, atol=1e-15)
+    # the reference's D3Q19 basis is rank 16 (its ghost rows are dependent)
+    assert np.linalg.matrix_rank(orc.M19) == 16
+    assert np.linalg.matrix_rank(orc.M9) == 9
+
+
+# -------------------------------------------------------------------- links
+def test_links_match_reference_c1_c2():
+    with np.load(G.GOLDEN / "links_c1_c2.npz") as d:
+        solid = d["c1_solid"]
+        yy, xx = np.mgrid[0:200, 0:400].astype(np.float64)
+        phi = np.sqrt((xx + 0.5 - 100.0) ** 2 + (yy + 0.5 - 100.0) ** 2) - 20.0
+        links, q = orc.build_links(solid, phi)
+        np.testing.assert_array_equal(links, d["c1_links"])
+        np.testing.assert_array_equal(q, d["c1_q"])
+        zz, yy, xx = np.mgrid[0:64, 0:64, 0:128].astype(np.float64)
+        solid3 = (xx - 32.0) ** 2 + (yy - 32.0) ** 2 + (zz - 32.0) ** 2 <= 64.0
+        assert int(solid3.sum()) == int(d["c2_solid_count"])
+        phi3 = np.sqrt((xx + 0.5 - 32.0) ** 2 + (yy + 0.5 - 32.0) ** 2 + (zz + 0.5 - 32.0) ** 2) - 8.0
+        links3, q3 = orc.build_links(solid3, phi3)
+        np.testing.assert_array_equal(links3, d["c2_links"])
+        np.testing.assert_array_equal(q3, d["c2_q"])
+
+
+def test_links_brute_force_small():
+    rng = np.random.default_rng(3)
+    solid = rng.random((5, 6, 7)) < 0.3
+    links, _ = orc.build_links(solid)
+    expect = []
+    for i in range(1, 19):
+        e = orc.E19[i]
+        for z in range(5):
+            for y in range(6):
+                for x in range(7):
+                    zn = min(max(z + e[2], 0), 4)
+                    yn = min(max(y + e[1], 0), 5)
+                    xn = min(max(x + e[0], 0), 6)
+                    if not solid[z, y, x] and solid[zn, yn, xn]:
+                        expect.append((i, z, y, x))
+    np.testing.assert_array_equal(links, np.array(expect).reshape(-1, 4))
+
+
+# --------------------------------------------------------------- composites
+def make_oracle(dim, case):
+    kw = dict(case["kw"])
+    solid = case["solid"]
+    use_phi = kw.get("bouzidi", False) or kw.get("van_driest", False)
+    phi = case.get("phi") if use_phi else None
+    args = dict(omega=float(case["omega"]), u0=float(case["u0"]), D=float(case["D"]), phi=phi)
+    inlet_mode = "zouhe"
+    if kw.pop("synthetic_inflow", False):
+        inlet_mode = "field"
+    kw.pop("synthetic_inflow_intensity", None)
+    if kw.pop("sem_inlet", False):
+        inlet_mode = "sem"
+    for k in ("sem_Tu", "sem_L_int", "sem_N"):
+        kw.pop(k, None)
+    bf = (kw.pop("body_force_x", 0.0), kw.pop("body_force_y", 0.0), kw.pop("body_force_z", 0.0))
+    s = orc.OracleSolver(solid, inlet_mode=inlet_mode, body_force=bf, **args, **kw)
+    s.set_state(G.initial_f(case))
+    return s
+
+
+def run_and_compare(dim, name, f_tol=1e-10, c_tol=1e-9):
+    case = G.load(dim, name)
+    s = make_oracle(dim, case)
+    assert abs(s.f.sum() - float(case["f0_sum"])) < 1e-10
+    np.testing.assert_array_equal(s.links_ref, case["links"])
+    np.testing.assert_array_equal(s.q_vals, case["q_vals"])
+    syn = None
+    if s.inlet_mode == "field":
+        shape = (s.ny,) if dim == 2 else (s.nz, s.ny)
+        syn = G.SyntheticInflow(shape, s.u0, float(case["kw"]["synthetic_inflow_intensity"]))
+    snaps = G.SNAP2 if dim == 2 else G.SNAP3
+    coeffs = []
+    for step in range(1, max(snaps) + 1):
+        plane = None
+        if syn is not None:
+            plane = syn.next()
+        elif s.inlet_mode == "sem":
+            plane = case["sem_planes"][step - 1]
+        raw = s.step(plane)
+        coeffs.append(s.coefficients(raw))
+        if step in snaps:
+            assert G.rel_err(s.f_ref(), case[f"f_{step}"]) < f_tol, (name, step)
+            assert G.rel_err(s.prev_ref(), case[f"fprev_{step}"]) < f_tol, (name, step)
+            if s.g is not None:
+                assert G.rel_err(s.g_ref(), case[f"g_{step}"]) < f_tol, (name, step)
+    c = np.array(coeffs)
+    if dim == 2:
+        cols = dict(Cd=0, Cl=1, Cm=5, Cd_p=6, Cd_v=7)
+    else:
+        cols = dict(Cd=0, Cly=1, Clz=2, Cmx=3, Cmy=4, Cmz=5, Cd_p=6, Cd_v=7)
+    for key, col in cols.items():
+        ref = case[f"{key}_hist"]
+        scale = max(np.max(np.abs(ref)), 1e-3)
+        assert np.max(np.abs(c[:, col] - ref)) / scale < c_tol, (name, key)
+    prof = s.last["profile"]
+    for k, ax in (("fx", 0), ("fy", 1)) + ((("fz", 2),) if dim == 3 else ()):
+        np.testing.assert_allclose(prof[:, ax], case[f"profile_{k}"], rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("name", G.CASES_2D)
+def test_oracle_2d_matches_reference(name):
+    run_and_compare(2, name)
+
+
+@pytest.mark.parametrize("name", G.CASES_3D)
+def test_oracle_3d_matches_reference(name):
+    run_and_compare(3, name)
+
+
+def test_oracle_detects_planted_mistake():
+    """A plausible slip (slip-wall pairs swapped for no-slip) must fail parity."""
+    case = G.load(2, "bgk_bb_conv_slip")
+    s = make_oracle(2, case)
+    s.wall_bc = "noslip"
+    s._wall_tables()
+    for _ in range(G.SNAP2[-1]):
+        s.step()
+    assert G.rel_err(s.f_ref(), case[f"f_{G.SNAP2[-1]}"]) > 1e-8
