@@ -1,0 +1,192 @@
+"""Shared machinery of Solver (D2Q9) and Solver3D (D3Q19).
+
+The device context holds every population array; the Python object keeps the
+reference's public attributes (histories, links, parameters) and turns the
+reference's per-step Python loop into chunked ``aero_step`` calls, syncing
+with the host only at check points (aero/lbm/solver.py:470-632).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+import warnings
+from typing import Optional
+
+import numpy as np
+
+from . import _abi
+from .diagnostics import check_stability_stats
+
+VALID_BACKENDS = ("auto", "hip", "numpy", "numba", "cupy")
+
+
+def resolve_backend(backend: str) -> str:
+    """The reference selects numpy/numba/cupy kernels (solver.py:192-203,
+    solver3d.py:197-221).  Here every name runs the same HIP kernels; unknown
+    names raise ValueError exactly as the reference does."""
+    if backend not in VALID_BACKENDS:
+        raise ValueError(f"Unknown backend '{backend}'. Choose 'auto', 'numpy', 'numba', 'cupy' or 'hip'.")
+    return "hip"
+
+
+class DeviceSolver:
+    """Holds the aero_ctx and the reference-layout views of its state."""
+
+    _Q: int
+    _is3d: bool
+
+    # ------------------------------------------------------------- setup
+    def _init_device(self, desc: _abi.AeroDesc, solid3: np.ndarray, links4: np.ndarray,
+                     q_vals: np.ndarray, damp: Optional[np.ndarray]):
+        self._ctx = _abi.Context(desc)
+        sol = np.ascontiguousarray(solid3, dtype=np.uint8)
+        l4 = np.ascontiguousarray(links4, dtype=np.int32)
+        qv = np.ascontiguousarray(q_vals, dtype=np.float32)
+        self._ctx.call("aero_set_geometry", _abi.ptr(sol), ctypes.c_int64(len(l4)), _abi.ptr(l4), _abi.ptr(qv))
+        if damp is not None:
+            self._ctx.call("aero_set_damping", _abi.ptr(np.ascontiguousarray(damp, dtype=np.float64)))
+        self._nrow = desc.nz * desc.ny
+        self._N = desc.nz * desc.ny * desc.nx
+
+    # ------------------------------------------------------------- state
+    def _ref_shape(self):
+        raise NotImplementedError
+
+    @property
+    def f(self) -> np.ndarray:
+        """The reference's self.f: post-boundary, pre-collision populations."""
+        out = np.empty(self._ref_shape(), dtype=np.float64)
+        self._ctx.call("aero_get_f", _abi.ptr(out))
+        return out
+
+    @f.setter
+    def f(self, value) -> None:
+        arr = np.ascontiguousarray(value, dtype=np.float64)
+        if arr.shape != self._ref_shape():
+            raise ValueError(f"f must have shape {self._ref_shape()}, got {arr.shape}")
+        self._ctx.call("aero_set_f", _abi.ptr(arr))
+
+    @property
+    def g(self) -> Optional[np.ndarray]:
+        if not self.thermal:
+            return None
+        out = np.empty(self._ref_shape(), dtype=np.float64)
+        self._ctx.call("aero_get_g", _abi.ptr(out))
+        return out
+
+    @g.setter
+    def g(self, value) -> None:
+        if value is None:
+            return
+        if not self.thermal:
+            raise ValueError("g can only be set when thermal=True")
+        arr = np.ascontiguousarray(value, dtype=np.float64)
+        if arr.shape != self._ref_shape():
+            raise ValueError(f"g must have shape {self._ref_shape()}, got {arr.shape}")
+        self._ctx.call("aero_set_g", _abi.ptr(arr))
+
+    def _prev_shape(self):
+        s = self._ref_shape()
+        return s[:-1]
+
+    @property
+    def _f_outlet_prev(self) -> np.ndarray:
+        out = np.empty(self._prev_shape(), dtype=np.float64)
+        self._ctx.call("aero_get_outlet_prev", _abi.ptr(out))
+        return out
+
+    @_f_outlet_prev.setter
+    def _f_outlet_prev(self, value) -> None:
+        arr = np.ascontiguousarray(value, dtype=np.float64)
+        if arr.shape != self._prev_shape():
+            raise ValueError(f"f_outlet_prev must have shape {self._prev_shape()}, got {arr.shape}")
+        self._ctx.call("aero_set_outlet_prev", _abi.ptr(arr))
+
+    def _macroscopic(self):
+        n = self._N
+        rho, ux, uy, uz = (np.empty(n) for _ in range(4))
+        self._ctx.call("aero_macroscopic", _abi.ptr(rho), _abi.ptr(ux), _abi.ptr(uy), _abi.ptr(uz))
+        sh = self._ref_shape()[1:]
+        return rho.reshape(sh), ux.reshape(sh), uy.reshape(sh), uz.reshape(sh)
+
+    def _device_check(self, step: int) -> None:
+        stats = np.empty(_abi.NCHECK)
+        self._ctx.call("aero_check", _abi.ptr(stats))
+        check_stability_stats(stats, step)
+
+    def _link_forces(self):
+        n = ctypes.c_int32(0)
+        self._ctx.call("aero_link_forces", None, None, ctypes.byref(n))
+        cells = np.empty(n.value, dtype=np.int32)
+        forces = np.empty((n.value, 12), dtype=np.float64)
+        self._ctx.call("aero_link_forces", _abi.ptr(cells), _abi.ptr(forces), ctypes.byref(n))
+        return cells, forces
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_int64(0)
+        self._ctx.call("aero_device_bytes", ctypes.byref(b))
+        return int(b.value)
+
+    # ------------------------------------------------------------- stepping
+    def _planes(self, n: int) -> Optional[np.ndarray]:
+        """Per-step inlet planes (n, 3, nrow) for field/SEM inlets, or None."""
+        return None
+
+    def _advance(self, n: int) -> np.ndarray:
+        """Run n steps on the device; returns the (n, 20) coefficient rows."""
+        if getattr(self, "_raise_on_step", None):
+            raise self._raise_on_step
+        planes = self._planes(n)
+        hist = np.empty((n, _abi.NCOEF), dtype=np.float64)
+        self._ctx.call("aero_step", ctypes.c_int32(n), ctypes.c_int64(self.step_count),
+                       _abi.ptr(planes), _abi.ptr(hist))
+        self.step_count += n
+        return hist
+
+    def close(self) -> None:
+        ctx = getattr(self, "_ctx", None)
+        if ctx is not None:
+            ctx.close()
+
+
+def run_loop(solver, steps, check_every, verbose, checkpoint_every, checkpoint_dir, ckpt_name,
+             on_chunk, on_check, auto_stop_fn):
+    """Chunked version of the reference run loop (solver.py:517-570).
+
+    Chunks end at every multiple of check_every and checkpoint_every so the
+    host only synchronises where the reference would look at the state.
+    Returns (stop_reason, last_convergence, last_strouhal).
+    """
+    import pathlib
+    t0 = time.perf_counter()
+    step = 0
+    stop_reason = "max_steps"
+    last_conv = last_st = None
+    while step < steps:
+        nxt = steps
+        for every in (check_every, checkpoint_every if (checkpoint_every and checkpoint_dir) else None):
+            if every:
+                nxt = min(nxt, (step // every + 1) * every)
+        n = nxt - step
+        rows = solver._advance(n)
+        on_chunk(rows)
+        step = nxt
+        if checkpoint_every and checkpoint_dir and step % checkpoint_every == 0:
+            path = pathlib.Path(checkpoint_dir) / ckpt_name.format(step=solver.step_count)
+            solver.save_checkpoint(str(path))
+        if step % check_every == 0:
+            elapsed = time.perf_counter() - t0
+            on_check(step, steps, elapsed)
+            if auto_stop_fn is not None:
+                last_conv, last_st, stop = auto_stop_fn()
+                if stop:
+                    stop_reason = "auto_converged"
+                    break
+    return stop_reason, last_conv, last_st
+
+
+def warn_hdf5(hdf5_path):
+    if hdf5_path:
+        warnings.warn("HDF5 snapshots are not written by this build (the reference writer is a no-op "
+                      "without h5py as well)", RuntimeWarning, stacklevel=3)

@@ -1,0 +1,437 @@
+// Support kernels around the fused step: state materialisation, field
+// collision, recycling prepass, conversions and the fixed-order FP64
+// reductions (forces, stability check, recycling mean).
+#include "launch.h"
+#include "step_impl.h"
+
+namespace aero {
+
+hipError_t launch_step_d3q19_bgk(int st, bool thermal, int kmode, const StepParams& p, hipStream_t s);
+hipError_t launch_step_d3q19_trt(int st, bool thermal, int kmode, const StepParams& p, hipStream_t s);
+hipError_t launch_step_d3q19_mrt(int st, bool thermal, int kmode, const StepParams& p, hipStream_t s);
+
+hipError_t launch_step_d3q19(int coll, int st, bool thermal, int kmode, const StepParams& p, hipStream_t s) {
+  switch (coll) {
+    case COLL_BGK: return launch_step_d3q19_bgk(st, thermal, kmode, p, s);
+    case COLL_TRT: return launch_step_d3q19_trt(st, thermal, kmode, p, s);
+    case COLL_MRT: return launch_step_d3q19_mrt(st, thermal, kmode, p, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------ materialize
+template <class L>
+static hipError_t mat_l(int st, bool thermal, const StepParams& p, hipStream_t s) {
+  const dim3 g((unsigned)p.nblocks_grid);
+#define AERO_CASE(ST)                                                                  \
+  case ST:                                                                             \
+    if (thermal) materialize_kernel<L, ST, true><<<g, BLOCK, 0, s>>>(p);               \
+    else materialize_kernel<L, ST, false><<<g, BLOCK, 0, s>>>(p);                      \
+    return hipGetLastError();
+  switch (st) {
+    AERO_CASE(ST_F64)
+    AERO_CASE(ST_F32)
+    AERO_CASE(ST_F16S)
+    AERO_CASE(ST_BF16S)
+  }
+#undef AERO_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_materialize(int q, int st, bool thermal, const StepParams& p, hipStream_t s) {
+  return q == 9 ? mat_l<D2Q9>(st, thermal, p, s) : mat_l<D3Q19>(st, thermal, p, s);
+}
+
+// ---------------------------------------------------------------- collide
+template <class L, int COLL>
+static hipError_t col_lc(int st, const CollideParams& p, hipStream_t s) {
+  const dim3 g((unsigned)p.nblocks_grid);
+  switch (st) {
+    case ST_F64: collide_kernel<L, COLL, ST_F64><<<g, BLOCK, 0, s>>>(p); break;
+    case ST_F32: collide_kernel<L, COLL, ST_F32><<<g, BLOCK, 0, s>>>(p); break;
+    case ST_F16S: collide_kernel<L, COLL, ST_F16S><<<g, BLOCK, 0, s>>>(p); break;
+    case ST_BF16S: collide_kernel<L, COLL, ST_BF16S><<<g, BLOCK, 0, s>>>(p); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <class L>
+static hipError_t col_l(int coll, int st, const CollideParams& p, hipStream_t s) {
+  switch (coll) {
+    case COLL_BGK: return col_lc<L, COLL_BGK>(st, p, s);
+    case COLL_TRT: return col_lc<L, COLL_TRT>(st, p, s);
+    case COLL_MRT: return col_lc<L, COLL_MRT>(st, p, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_collide(int q, int coll, int st, const CollideParams& p, hipStream_t s) {
+  return q == 9 ? col_l<D2Q9>(coll, st, p, s) : col_l<D3Q19>(coll, st, p, s);
+}
+
+// ------------------------------------------------------ recycling prepass
+// Macroscopic state of plane x = nx-2 after streaming, body force and links
+// (the plane apply_recycling_rescaling_inlet_3d reads, boundary3d.py:185-190).
+template <int ST>
+__global__ __launch_bounds__(BLOCK) void recycling_plane_kernel(StepParams p, double* out) {
+  using S = Storage<ST>;
+  using C = typename S::C;
+  const int row = blockIdx.x * BLOCK + threadIdx.x;
+  if (row >= p.nrow) return;
+  const int y = row % p.ny, z = row / p.ny;
+  const int x = p.nx - 2;
+  const int cell = row * p.nx + x;
+  C fn[19], fpre[19];
+  pre_outlet_state<D3Q19, ST>(p, (const typename S::T*)p.fin, cell, x, y, z, p.flags[cell], fn, fpre);
+  double rho = 0, mx = 0, my = 0, mz = 0;
+#pragma unroll
+  for (int i = 0; i < 19; ++i) {
+    const double v = (double)fn[i];
+    rho += v;
+    mx += D3Q19::EX[i] * v;
+    my += D3Q19::EY[i] * v;
+    mz += D3Q19::EZ[i] * v;
+  }
+  const double inv = rho > 0.0 ? 1.0 / rho : 0.0;
+  out[row] = rho;
+  out[p.nrow + row] = inv * mx;
+  out[2 * p.nrow + row] = inv * my;
+  out[3 * p.nrow + row] = inv * mz;
+}
+
+hipError_t launch_recycling_plane(int st, const StepParams& p, double* out, hipStream_t s) {
+  const dim3 g((unsigned)((p.nrow + BLOCK - 1) / BLOCK));
+  switch (st) {
+    case ST_F64: recycling_plane_kernel<ST_F64><<<g, BLOCK, 0, s>>>(p, out); break;
+    case ST_F32: recycling_plane_kernel<ST_F32><<<g, BLOCK, 0, s>>>(p, out); break;
+    case ST_F16S: recycling_plane_kernel<ST_F16S><<<g, BLOCK, 0, s>>>(p, out); break;
+    case ST_BF16S: recycling_plane_kernel<ST_BF16S><<<g, BLOCK, 0, s>>>(p, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Mean ux over the plane (fixed order) and the rescale factor
+// target/mean (1 when |mean| < 1e-12), stored after the 4 plane arrays.
+__global__ __launch_bounds__(BLOCK) void reduce_recycling_kernel(double* recyc, int nrow, double target) {
+  __shared__ double red[BLOCK];
+  double acc = 0.0;
+  for (int r = threadIdx.x; r < nrow; r += BLOCK) acc += recyc[nrow + r];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double mean = red[0] / (double)nrow;
+    recyc[4 * nrow] = fabs(mean) < 1e-12 ? 1.0 : target / mean;
+  }
+}
+
+hipError_t launch_reduce_recycling(double* recyc, int nrow, double target_u0, hipStream_t s) {
+  reduce_recycling_kernel<<<1, BLOCK, 0, s>>>(recyc, nrow, target_u0);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------- scalar state
+// Pre-collision scalar state (the reference's self.g): pull of the stored
+// post-collision g plus the y-wall Dirichlet values (thermal.py:85-100).
+template <class L, int ST>
+__global__ __launch_bounds__(BLOCK) void materialize_g_kernel(StepParams p, double* out) {
+  using S = Storage<ST>;
+  using C = typename S::C;
+  const long long c64 = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (c64 >= p.ncell) return;
+  const int cell = (int)c64;
+  const int x = cell % p.nx, yz = cell / p.nx, y = yz % p.ny, z = yz / p.ny;
+  C g[L::Q];
+  gather<L, ST>(p, (const typename S::T*)p.gin, x, y, z, (C)p.gshift, g);
+  const size_t N = (size_t)p.ncell;
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    double v = (double)g[i];
+    if (!p.g_bc_skip) {
+      if (y == 0) v = L::W[i] * p.T_hot;
+      if (y == p.ny - 1) v = L::W[i] * p.T_cold;
+    }
+    out[(size_t)i * N + cell] = v;
+  }
+}
+
+hipError_t launch_materialize_g(int q, int st, const StepParams& p, double* out, hipStream_t s) {
+  const dim3 g((unsigned)((p.ncell + BLOCK - 1) / BLOCK));
+#define AERO_CASE(ST)                                                                      \
+  case ST:                                                                                 \
+    if (q == 9) materialize_g_kernel<D2Q9, ST><<<g, BLOCK, 0, s>>>(p, out);                \
+    else materialize_g_kernel<D3Q19, ST><<<g, BLOCK, 0, s>>>(p, out);                      \
+    return hipGetLastError();
+  switch (st) {
+    AERO_CASE(ST_F64)
+    AERO_CASE(ST_F32)
+    AERO_CASE(ST_F16S)
+    AERO_CASE(ST_BF16S)
+  }
+#undef AERO_CASE
+  return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------ conversions
+struct Weights {
+  double w[19];
+};
+
+template <int ST>
+__global__ __launch_bounds__(BLOCK) void store_from_f64_kernel(long long n, int q, const double* src,
+                                                               typename Storage<ST>::T* dst, Weights w,
+                                                               double shift_scale) {
+  using S = Storage<ST>;
+  using C = typename S::C;
+  const long long k = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (k >= n * q) return;
+  const int i = (int)(k / n);
+  S::store(dst, (size_t)k, (C)src[k], (C)(w.w[i] * shift_scale));
+}
+
+hipError_t launch_store_from_f64(int st, int q, long long n, const double* src, void* dst, double shift_scale,
+                                 hipStream_t s) {
+  const dim3 g((unsigned)((n * q + BLOCK - 1) / BLOCK));
+  Weights w{};
+  for (int i = 0; i < q; ++i) w.w[i] = q == 9 ? D2Q9::W[i] : D3Q19::W[i];
+  switch (st) {
+    case ST_F64: store_from_f64_kernel<ST_F64><<<g, BLOCK, 0, s>>>(n, q, src, (double*)dst, w, shift_scale); break;
+    case ST_F32: store_from_f64_kernel<ST_F32><<<g, BLOCK, 0, s>>>(n, q, src, (float*)dst, w, shift_scale); break;
+    case ST_F16S:
+      store_from_f64_kernel<ST_F16S><<<g, BLOCK, 0, s>>>(n, q, src, (__half*)dst, w, shift_scale);
+      break;
+    case ST_BF16S:
+      store_from_f64_kernel<ST_BF16S><<<g, BLOCK, 0, s>>>(n, q, src, (__hip_bfloat16*)dst, w, shift_scale);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(BLOCK) void f64_to_f32_kernel(long long count, const double* src, float* dst) {
+  const long long k = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (k < count) dst[k] = (float)src[k];
+}
+
+hipError_t launch_compute_from_f64(int st, long long count, const double* src, void* dst, hipStream_t s) {
+  if (st == ST_F64) {
+    return hipMemcpyAsync(dst, src, sizeof(double) * count, hipMemcpyDeviceToDevice, s);
+  }
+  f64_to_f32_kernel<<<dim3((unsigned)((count + BLOCK - 1) / BLOCK)), BLOCK, 0, s>>>(count, src, (float*)dst);
+  return hipGetLastError();
+}
+
+// Velocity of an explicit field (rho > 0 guard, solids not zeroed: les.py:510).
+template <class C, int Q>
+__global__ __launch_bounds__(BLOCK) void macro_kernel(long long n, const C* F, C* U) {
+  const long long c = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (c >= n) return;
+  using L = typename std::conditional<Q == 9, D2Q9, D3Q19>::type;
+  C rho = 0, mx = 0, my = 0, mz = 0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const C v = F[(size_t)i * n + c];
+    rho += v;
+    mx += C(L::EX[i]) * v;
+    my += C(L::EY[i]) * v;
+    mz += C(L::EZ[i]) * v;
+  }
+  const C inv = rho > C(0) ? C(1) / rho : C(0);
+  U[c] = inv * mx;
+  U[n + c] = inv * my;
+  U[2 * n + c] = inv * mz;
+}
+
+hipError_t launch_macro_from_compute(int st, int q, long long n, const void* F, void* U, hipStream_t s) {
+  const dim3 g((unsigned)((n + BLOCK - 1) / BLOCK));
+  if (st == ST_F64) {
+    if (q == 9) macro_kernel<double, 9><<<g, BLOCK, 0, s>>>(n, (const double*)F, (double*)U);
+    else macro_kernel<double, 19><<<g, BLOCK, 0, s>>>(n, (const double*)F, (double*)U);
+  } else {
+    if (q == 9) macro_kernel<float, 9><<<g, BLOCK, 0, s>>>(n, (const float*)F, (float*)U);
+    else macro_kernel<float, 19><<<g, BLOCK, 0, s>>>(n, (const float*)F, (float*)U);
+  }
+  return hipGetLastError();
+}
+
+// rho, ux, uy, uz of an f64 field (d2q9.py:44-63: rho > 0 guard).
+template <int Q>
+__global__ __launch_bounds__(BLOCK) void macro4_kernel(long long n, const double* F, double* out) {
+  const long long c = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (c >= n) return;
+  using L = typename std::conditional<Q == 9, D2Q9, D3Q19>::type;
+  double rho = 0, mx = 0, my = 0, mz = 0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const double v = F[(size_t)i * n + c];
+    rho += v;
+    mx += L::EX[i] * v;
+    my += L::EY[i] * v;
+    mz += L::EZ[i] * v;
+  }
+  const double inv = rho > 0.0 ? 1.0 / rho : 0.0;
+  out[c] = rho;
+  out[n + c] = inv * mx;
+  out[2 * n + c] = inv * my;
+  out[3 * n + c] = inv * mz;
+}
+
+hipError_t launch_macro4_f64(int q, long long n, const double* F, double* out, hipStream_t s) {
+  const dim3 g((unsigned)((n + BLOCK - 1) / BLOCK));
+  if (q == 9) macro4_kernel<9><<<g, BLOCK, 0, s>>>(n, F, out);
+  else macro4_kernel<19><<<g, BLOCK, 0, s>>>(n, F, out);
+  return hipGetLastError();
+}
+
+// Per-block partials of check_stability (diagnostics.py:32-71) on an f64
+// field, optionally restricted to one z plane (3D midplane, solver3d.py:608-615).
+template <int Q>
+__global__ __launch_bounds__(BLOCK) void check_f64_kernel(int nx, int ny, int nz, int zc, const double* F,
+                                                          double* part) {
+  using L = typename std::conditional<Q == 9, D2Q9, D3Q19>::type;
+  __shared__ double red[BLOCK / 64][NCHECK];
+  const long long n = (long long)nx * ny * nz;
+  const long long c = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  double v[NCHECK] = {0, 0, 0, 0, 0};
+  if (c < n && (zc < 0 || (int)(c / ((long long)nx * ny)) == zc)) {
+    double rho = 0, mx = 0, my = 0;
+    bool nan = false;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const double f = F[(size_t)i * n + c];
+      nan |= isnan(f);
+      rho += f;
+      mx += L::EX[i] * f;
+      my += L::EY[i] * f;
+    }
+    const double inv = rho > 0.0 ? 1.0 / rho : 0.0;
+    const double ux = inv * mx, uy = inv * my;
+    const double um = sqrt(ux * ux + uy * uy);
+    v[0] = nan ? 1.0 : 0.0;
+    v[1] = rho <= 0.0 ? 1.0 : 0.0;
+    v[2] = isnan(um) ? 0.0 : um;
+    v[3] = rho;
+    v[4] = 1.0;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    v[0] = fmax(v[0], __shfl_xor(v[0], off));
+    v[1] = fmax(v[1], __shfl_xor(v[1], off));
+    v[2] = fmax(v[2], __shfl_xor(v[2], off));
+    v[3] += __shfl_xor(v[3], off);
+    v[4] += __shfl_xor(v[4], off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < NCHECK; ++k) red[threadIdx.x >> 6][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r[NCHECK] = {0, 0, 0, 0, 0};
+    for (int w = 0; w < BLOCK / 64; ++w) {
+      r[0] = fmax(r[0], red[w][0]);
+      r[1] = fmax(r[1], red[w][1]);
+      r[2] = fmax(r[2], red[w][2]);
+      r[3] += red[w][3];
+      r[4] += red[w][4];
+    }
+#pragma unroll
+    for (int k = 0; k < NCHECK; ++k) part[(size_t)blockIdx.x * NCHECK + k] = r[k];
+  }
+}
+
+hipError_t launch_check_f64(int q, int nx, int ny, int nz, int zc, const double* F, int nblocks, double* part,
+                            hipStream_t s) {
+  if (q == 9) check_f64_kernel<9><<<nblocks, BLOCK, 0, s>>>(nx, ny, nz, zc, F, part);
+  else check_f64_kernel<19><<<nblocks, BLOCK, 0, s>>>(nx, ny, nz, zc, F, part);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- reductions
+// Momentum-exchange totals in a fixed order: thread t sums link cells
+// t, t+256, ... sequentially, then a fixed tree combines the 256 partials,
+// so a case gives the same bits whatever else shares the device.
+__global__ __launch_bounds__(BLOCK) void reduce_forces_kernel(const double* fpart, int nrec, double fd, double md,
+                                                              double* row) {
+  __shared__ double red[NFORCE][BLOCK];
+  double acc[NFORCE];
+#pragma unroll
+  for (int k = 0; k < NFORCE; ++k) acc[k] = 0.0;
+  for (int r = threadIdx.x; r < nrec; r += BLOCK) {
+#pragma unroll
+    for (int k = 0; k < NFORCE; ++k) acc[k] += fpart[(size_t)r * NFORCE + k];
+  }
+#pragma unroll
+  for (int k = 0; k < NFORCE; ++k) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int k = 0; k < NFORCE; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double F[NFORCE];
+#pragma unroll
+    for (int k = 0; k < NFORCE; ++k) F[k] = red[k][0];
+    // forces.py:78-103 / forces3d.py:364-395 and the moment coefficients
+    const bool ok = fd != 0.0;
+    row[0] = ok ? F[0] / fd : 0.0;
+    row[1] = ok ? F[1] / fd : 0.0;
+    row[2] = ok ? F[2] / fd : 0.0;
+    row[3] = md != 0.0 ? F[3] / md : 0.0;
+    row[4] = md != 0.0 ? F[4] / md : 0.0;
+    row[5] = md != 0.0 ? F[5] / md : 0.0;
+    row[6] = ok ? F[6] / fd : 0.0;
+    row[7] = ok ? F[9] / fd : 0.0;
+#pragma unroll
+    for (int k = 0; k < NFORCE; ++k) row[8 + k] = F[k];
+  }
+}
+
+hipError_t launch_reduce_forces(const double* fpart, int nrec, double fd, double md, double* row, hipStream_t s) {
+  reduce_forces_kernel<<<1, BLOCK, 0, s>>>(fpart, nrec, fd, md, row);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(BLOCK) void reduce_check_kernel(const double* part, int nblocks, double* out) {
+  __shared__ double red[NCHECK][BLOCK];
+  double a[NCHECK] = {0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nblocks; b += BLOCK) {
+    a[0] = fmax(a[0], part[(size_t)b * NCHECK + 0]);
+    a[1] = fmax(a[1], part[(size_t)b * NCHECK + 1]);
+    a[2] = fmax(a[2], part[(size_t)b * NCHECK + 2]);
+    a[3] += part[(size_t)b * NCHECK + 3];
+    a[4] += part[(size_t)b * NCHECK + 4];
+  }
+#pragma unroll
+  for (int k = 0; k < NCHECK; ++k) red[k][threadIdx.x] = a[k];
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + w]);
+      red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + w]);
+      red[2][threadIdx.x] = fmax(red[2][threadIdx.x], red[2][threadIdx.x + w]);
+      red[3][threadIdx.x] += red[3][threadIdx.x + w];
+      red[4][threadIdx.x] += red[4][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NCHECK; ++k) out[k] = red[k][0];
+  }
+}
+
+hipError_t launch_reduce_check(const double* part, int nblocks, double* out, hipStream_t s) {
+  reduce_check_kernel<<<1, BLOCK, 0, s>>>(part, nblocks, out);
+  return hipGetLastError();
+}
+
+}  // namespace aero

@@ -1,0 +1,658 @@
+// Host side of the C ABI: context state, geometry records, the step loop and
+// state materialisation.  See include/aero_hip.h for the contract.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/aero_hip.h"
+#include "launch.h"
+#include "step_impl.h"
+
+using namespace aero;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void ck(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+template <class F>
+int guarded(F&& fn) {
+  try {
+    fn();
+    return 0;
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return 1;
+  }
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void alloc(size_t b) {
+    if (bytes >= b && p) return;
+    release();
+    if (b == 0) return;
+    ck(hipMalloc(&p, b), "hipMalloc");
+    bytes = b;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const { return (T*)p; }
+};
+
+size_t storage_bytes(int st) {
+  switch (st) {
+    case ST_F64: return 8;
+    case ST_F32: return 4;
+    default: return 2;
+  }
+}
+
+}  // namespace
+
+struct aero_ctx {
+  aero_desc d;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int Q = 0;
+  long long N = 0;
+  int nrow = 0;
+  int nblocks = 0, nblocks_grid = 0;
+  size_t esz = 8;  // storage element size
+  size_t csz = 8;  // compute element size
+  // state: ping-pong sides, `side` is the one the next step reads
+  DevBuf B[2], G[2], prev[2];
+  int side = 0;
+  int prev_side = 0;
+  // geometry
+  DevBuf flags, lrec, row_start, lq, fpart, sigma, minv_d, minv_f, damp;
+  int nrec = 0;
+  std::vector<int> rec_cells;
+  // per-step inputs
+  DevBuf planes, recyc, hist;
+  const double* last_plane = nullptr;
+  long long last_step_no = 0;
+  int last_g_skip = 0;
+  int g_skip_pending = 0;
+  bool has_state = false;  // set_f has defined the state
+  // LES split buffers
+  DevBuf fsplit, usplit;
+  // caches of the reference state in f64
+  DevBuf fcache, gcache, tmp;
+  bool fcache_valid = false, gcache_valid = false;
+  DevBuf checkpart, checkout;
+
+  bool les_split() const { return d.les != AERO_LES_OFF && d.collision != AERO_MRT; }
+  bool convective() const { return d.outlet == AERO_OUT_CONVECTIVE; }
+
+  void set_device() const { ck(hipSetDevice(device), "hipSetDevice"); }
+
+  StepParams base_params() const {
+    StepParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.nx = d.nx;
+    p.ny = d.ny;
+    p.nz = d.nz;
+    p.nrow = nrow;
+    p.ncell = N;
+    p.nblocks = nblocks;
+    p.nblocks_grid = nblocks_grid;
+    p.flags = flags.as<uint8_t>();
+    p.lrec = lrec.as<LinkCell>();
+    p.row_start = row_start.as<int>();
+    p.lq = lq.as<float>();
+    p.fpart = fpart.as<double>();
+    p.sigma = d.sponge_thickness > 0 ? sigma.as<double>() : nullptr;
+    p.minv_d = minv_d.as<double>();
+    p.minv_f = minv_f.as<float>();
+    for (int a = 0; a < 19; ++a) p.mrt_s[a] = d.mrt_s[a];
+    p.omega = d.omega;
+    p.trt_lambda = d.trt_lambda;
+    {
+      // trt2d.py:8-18: s- = 1/tau-, tau+ = lam tau + 1 - lam, tau- = (tau+ tau - 1/2)/(tau+ - 1/2)
+      const double tau = 1.0 / d.omega;
+      const double tp = d.trt_lambda * tau + (1.0 - d.trt_lambda);
+      const double tm = (tp * tau - 0.5) / (tp - 0.5);
+      p.trt_sm = 1.0 / tm;
+    }
+    p.u0 = d.u0;
+    p.u_conv = d.u0;
+    p.inlet_amp = d.inlet_perturbation > 0.0 ? d.inlet_perturbation : 0.0;
+    p.rho_in = d.rho_in;
+    p.rho_out = d.rho_out;
+    p.u_top = d.u_wall_top;
+    p.u_bot = d.u_wall_bottom;
+    p.bf[0] = d.body_force[0];
+    p.bf[1] = d.body_force[1];
+    p.bf[2] = d.body_force[2];
+    p.bf_coeff = 1.0 - 0.5 / (1.0 / d.omega);
+    p.omega_T = 1.0 / (3.0 * d.alpha_T + 0.5);
+    p.gbeta = d.g_gravity * d.beta;
+    p.T_ref = d.T_ref;
+    p.T_hot = d.T_hot;
+    p.T_cold = d.T_cold;
+    p.gshift = (d.storage == ST_F16S || d.storage == ST_BF16S) ? d.T_ref : 0.0;
+    p.center[0] = d.center[0];
+    p.center[1] = d.center[1];
+    p.center[2] = d.center[2];
+    p.inlet_mode = d.inlet;
+    p.outlet_mode = d.outlet;
+    p.wall_mode = d.wall;
+    p.body_force_on = (Q == 19 && d.body_force_on) ? 1 : 0;
+    p.buoyancy = d.buoyancy;
+    p.bouzidi = d.bouzidi;
+    p.check_z = -1;
+    p.recyc = recyc.as<double>();
+    return p;
+  }
+
+  double force_denom() const {
+    return Q == 19 ? 0.5 * d.rho0 * d.u0 * d.u0 * d.D * d.D : 0.5 * d.rho0 * d.u0 * d.u0 * d.D;
+  }
+  double moment_denom() const {
+    return Q == 19 ? 0.5 * d.rho0 * d.u0 * d.u0 * d.D * d.D * d.D : 0.5 * d.rho0 * d.u0 * d.u0 * d.D * d.D;
+  }
+
+  void launch_step_kernel(int kmode, const StepParams& p) {
+    hipError_t e = Q == 9 ? launch_step_d2q9(d.collision, d.storage, d.thermal != 0, kmode, p, stream)
+                          : launch_step_d3q19(d.collision, d.storage, d.thermal != 0, kmode, p, stream);
+    ck(e, "step kernel launch");
+  }
+
+  // B[dst] = collide(F) with the optional LES finite-difference omega.
+  void collide_field(const void* F, const void* U, int dst) {
+    CollideParams cp;
+    std::memset(&cp, 0, sizeof(cp));
+    cp.nx = d.nx;
+    cp.ny = d.ny;
+    cp.nz = d.nz;
+    cp.dim = Q == 19 ? 3 : 2;
+    cp.ncell = N;
+    cp.nblocks = nblocks;
+    cp.nblocks_grid = nblocks_grid;
+    cp.F = F;
+    cp.U = U;
+    cp.fout = B[dst].p;
+    cp.flags = flags.as<uint8_t>();
+    cp.damp = damp.as<double>();
+    cp.minv_d = minv_d.as<double>();
+    cp.minv_f = minv_f.as<float>();
+    for (int a = 0; a < 19; ++a) cp.mrt_s[a] = d.mrt_s[a];
+    StepParams bp = base_params();
+    cp.omega = d.omega;
+    cp.trt_sm = bp.trt_sm;
+    cp.trt_lambda = d.trt_lambda;
+    cp.base_nu = (1.0 / d.omega - 0.5) / 3.0;  // physics.py:27-29
+    cp.les_cs = d.les_cs;
+    cp.les = U ? d.les : AERO_LES_OFF;
+    ck(launch_collide(Q, d.collision, d.storage, cp, stream), "collide launch");
+  }
+
+  void recycling_prepass(int read_side) {
+    StepParams p = base_params();
+    p.fin = B[read_side].p;
+    ck(launch_recycling_plane(d.storage, p, recyc.as<double>(), stream), "recycling plane");
+    ck(launch_reduce_recycling(recyc.as<double>(), nrow, d.u0, stream), "recycling reduce");
+  }
+
+  // f64 post-boundary state of the last completed step into fcache.
+  void materialize_f() {
+    if (fcache_valid) return;
+    if (!has_state) throw std::runtime_error("no state: call aero_set_f first");
+    fcache.alloc(sizeof(double) * Q * N);
+    const int old = side ^ 1;
+    if (d.inlet == AERO_IN_RECYCLING) recycling_prepass(old);
+    StepParams p = base_params();
+    p.fin = B[old].p;
+    p.gin = G[old].p;
+    p.prev_in = prev[convective() ? (prev_side ^ 1) : prev_side].as<double>();
+    p.plane = last_plane;
+    p.phase = 0.17 * (double)last_step_no;
+    p.g_bc_skip = last_g_skip;
+    p.commit = 0;
+    p.out_mode = 0;
+    p.fmat = fcache.as<double>();
+    ck(launch_materialize(Q, d.storage, d.thermal != 0, p, stream), "materialize launch");
+    fcache_valid = true;
+  }
+
+  void materialize_g() {
+    if (gcache_valid) return;
+    gcache.alloc(sizeof(double) * Q * N);
+    StepParams p = base_params();
+    p.gin = G[side].p;
+    p.g_bc_skip = 0;
+    ck(launch_materialize_g(Q, d.storage, p, gcache.as<double>(), stream), "materialize g");
+    gcache_valid = true;
+  }
+
+  void invalidate() {
+    fcache_valid = false;
+    gcache_valid = false;
+  }
+};
+
+namespace {
+
+// Inverse periodic stream on the host: G[i](x) = g[i](x + e_i), so that the
+// pull performed by the next step returns g unchanged.
+void inverse_stream(const aero_ctx* c, const double* g, std::vector<double>& out) {
+  const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
+  const int* EX = c->Q == 9 ? D2Q9::EX : D3Q19::EX;
+  const int* EY = c->Q == 9 ? D2Q9::EY : D3Q19::EY;
+  const int* EZ = c->Q == 9 ? D2Q9::EZ : D3Q19::EZ;
+  out.resize((size_t)c->Q * c->N);
+  for (int i = 0; i < c->Q; ++i) {
+    const size_t base = (size_t)i * c->N;
+    for (int z = 0; z < nz; ++z) {
+      const int zs = ((z + EZ[i]) % nz + nz) % nz;
+      for (int y = 0; y < ny; ++y) {
+        const int ys = ((y + EY[i]) % ny + ny) % ny;
+        for (int x = 0; x < nx; ++x) {
+          const int xs = ((x + EX[i]) % nx + nx) % nx;
+          out[base + ((size_t)z * ny + y) * nx + x] = g[base + ((size_t)zs * ny + ys) * nx + xs];
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* aero_last_error(void) { return g_err.c_str(); }
+
+int aero_version(void) { return AERO_ABI_VERSION; }
+
+int aero_device_count(int32_t* n) {
+  return guarded([&] {
+    int k = 0;
+    ck(hipGetDeviceCount(&k), "hipGetDeviceCount");
+    *n = k;
+  });
+}
+
+int aero_create(const aero_desc* desc, aero_ctx** out) {
+  *out = nullptr;
+  aero_ctx* c = new aero_ctx();
+  int rc = guarded([&] {
+    c->d = *desc;
+    const aero_desc& d = c->d;
+    if (d.q != 9 && d.q != 19) throw std::invalid_argument("q must be 9 or 19");
+    if (d.q == 9 && d.nz != 1) throw std::invalid_argument("D2Q9 requires nz == 1");
+    if (d.nx < 3 || d.ny < 1 || d.nz < 1) throw std::invalid_argument("grid must have nx >= 3, ny >= 1, nz >= 1");
+    if (d.collision < 0 || d.collision > 2) throw std::invalid_argument("unknown collision");
+    if (d.storage < 0 || d.storage > 3) throw std::invalid_argument("unknown storage");
+    if (d.q == 9 && (d.inlet == AERO_IN_SEM || d.inlet == AERO_IN_RECYCLING))
+      throw std::invalid_argument("SEM and recycling inlets are 3D only");
+    if (d.q == 19 && (d.inlet == AERO_IN_PRESSURE || d.outlet == AERO_OUT_PRESSURE))
+      throw std::invalid_argument("pressure inlet/outlet are 2D only");
+    if (!(d.omega > 0.0)) throw std::invalid_argument("omega must be positive");
+    const long long N = (long long)d.nx * d.ny * d.nz;
+    if (N * d.q >= (1ll << 31)) throw std::invalid_argument("grid too large for 32-bit cell indexing");
+    c->Q = d.q;
+    c->N = N;
+    c->nrow = d.nz * d.ny;
+    c->nblocks = (int)((N + BLOCK - 1) / BLOCK);
+    c->nblocks_grid = ((c->nblocks + 7) / 8) * 8;
+    c->esz = storage_bytes(d.storage);
+    c->csz = d.storage == ST_F64 ? 8 : 4;
+    ck(hipGetDevice(&c->device), "hipGetDevice");
+    ck(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    const size_t fbytes = c->esz * d.q * (size_t)N;
+    c->B[0].alloc(fbytes);
+    c->B[1].alloc(fbytes);
+    if (d.thermal) {
+      c->G[0].alloc(fbytes);
+      c->G[1].alloc(fbytes);
+    }
+    const size_t pbytes = sizeof(double) * d.q * (size_t)c->nrow;
+    c->prev[0].alloc(pbytes);
+    c->prev[1].alloc(pbytes);
+    ck(hipMemset(c->prev[0].p, 0, pbytes), "memset");
+    ck(hipMemset(c->prev[1].p, 0, pbytes), "memset");
+    // sponge sigma(x), sponge.py:8-21 (same arithmetic as the reference)
+    std::vector<double> sig(d.nx, 0.0);
+    if (d.sponge_thickness > 0 && d.sponge_strength > 0.0) {
+      const int start = std::max(d.nx - d.sponge_thickness, 0);
+      for (int x = start; x < d.nx; ++x) {
+        const double frac = (double)(x - start + 1) / (double)std::max(d.sponge_thickness, 1);
+        sig[x] = d.sponge_strength * frac * frac;
+      }
+    }
+    c->sigma.alloc(sizeof(double) * d.nx);
+    ck(hipMemcpy(c->sigma.p, sig.data(), sizeof(double) * d.nx, hipMemcpyHostToDevice), "copy sigma");
+    c->minv_d.alloc(sizeof(double) * 361);
+    c->minv_f.alloc(sizeof(float) * 361);
+    std::vector<float> mf(361);
+    for (int k = 0; k < 361; ++k) mf[k] = (float)d.mrt_minv[k];
+    ck(hipMemcpy(c->minv_d.p, d.mrt_minv, sizeof(double) * 361, hipMemcpyHostToDevice), "copy minv");
+    ck(hipMemcpy(c->minv_f.p, mf.data(), sizeof(float) * 361, hipMemcpyHostToDevice), "copy minv");
+    c->recyc.alloc(sizeof(double) * (4 * (size_t)c->nrow + 1));
+    if (c->les_split()) {
+      c->fsplit.alloc(c->csz * d.q * (size_t)N);
+      c->usplit.alloc(c->csz * 3 * (size_t)N);
+    }
+    // default geometry: no solid, no links
+    std::vector<uint8_t> none((size_t)N, 0);
+    c->flags.alloc((size_t)N);
+    ck(hipMemcpy(c->flags.p, none.data(), (size_t)N, hipMemcpyHostToDevice), "copy flags");
+    std::vector<int> rs(c->nrow + 1, 0);
+    c->row_start.alloc(sizeof(int) * (c->nrow + 1));
+    ck(hipMemcpy(c->row_start.p, rs.data(), sizeof(int) * (c->nrow + 1), hipMemcpyHostToDevice), "copy rows");
+    c->lrec.alloc(sizeof(LinkCell));
+    c->lq.alloc(sizeof(float));
+    c->fpart.alloc(sizeof(double) * NFORCE);
+  });
+  if (rc) {
+    aero_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int aero_destroy(aero_ctx* c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags,
+                    &c->lrec, &c->row_start, &c->lq, &c->fpart, &c->sigma, &c->minv_d, &c->minv_f,
+                    &c->damp, &c->planes, &c->recyc, &c->hist, &c->fsplit, &c->usplit, &c->fcache,
+                    &c->gcache, &c->tmp, &c->checkpart, &c->checkout};
+  for (DevBuf* b : bufs) b->release();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int aero_set_geometry(aero_ctx* c, const uint8_t* solid, int64_t nlinks, const int32_t* links, const float* q) {
+  return guarded([&] {
+    c->set_device();
+    const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
+    std::vector<uint8_t> fl((size_t)c->N);
+    for (long long k = 0; k < c->N; ++k) fl[k] = solid[k] ? FLAG_SOLID : 0;
+    // group links by cell: mask + q per direction
+    std::map<int, std::pair<uint32_t, std::vector<std::pair<int, float>>>> cells;
+    for (int64_t k = 0; k < nlinks; ++k) {
+      const int i = links[4 * k], z = links[4 * k + 1], y = links[4 * k + 2], x = links[4 * k + 3];
+      if (i < 1 || i >= c->Q || x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz)
+        throw std::invalid_argument("link out of range");
+      const int cell = (z * ny + y) * nx + x;
+      if (solid[cell]) throw std::invalid_argument("link origin must be a fluid cell");
+      auto& e = cells[cell];
+      if (e.first & (1u << i)) throw std::invalid_argument("duplicate link");
+      e.first |= 1u << i;
+      e.second.push_back({i, q ? q[k] : 0.5f});
+    }
+    std::vector<LinkCell> recs;
+    std::vector<float> qv;
+    std::vector<int> rs(c->nrow + 1, 0);
+    recs.reserve(cells.size());
+    for (auto& kv : cells) {
+      auto& lst = kv.second.second;
+      std::sort(lst.begin(), lst.end());
+      LinkCell r{kv.first, kv.second.first, (int)qv.size(), 0};
+      for (auto& iq : lst) qv.push_back(iq.second);
+      recs.push_back(r);
+      fl[kv.first] |= FLAG_LINKS;
+      rs[kv.first / nx + 1] += 1;
+    }
+    for (int r = 0; r < c->nrow; ++r) rs[r + 1] += rs[r];
+    c->nrec = (int)recs.size();
+    c->rec_cells.clear();
+    for (auto& r : recs) c->rec_cells.push_back(r.cell);
+    ck(hipMemcpy(c->flags.p, fl.data(), (size_t)c->N, hipMemcpyHostToDevice), "copy flags");
+    c->lrec.alloc(sizeof(LinkCell) * std::max<size_t>(1, recs.size()));
+    c->lq.alloc(sizeof(float) * std::max<size_t>(1, qv.size()));
+    c->fpart.alloc(sizeof(double) * NFORCE * std::max<size_t>(1, recs.size()));
+    if (!recs.empty()) {
+      ck(hipMemcpy(c->lrec.p, recs.data(), sizeof(LinkCell) * recs.size(), hipMemcpyHostToDevice), "copy recs");
+      ck(hipMemcpy(c->lq.p, qv.data(), sizeof(float) * qv.size(), hipMemcpyHostToDevice), "copy q");
+    }
+    ck(hipMemset(c->fpart.p, 0, c->fpart.bytes), "memset fpart");
+    ck(hipMemcpy(c->row_start.p, rs.data(), sizeof(int) * (c->nrow + 1), hipMemcpyHostToDevice), "copy rows");
+    c->invalidate();
+  });
+}
+
+int aero_set_damping(aero_ctx* c, const double* damp) {
+  return guarded([&] {
+    c->set_device();
+    if (!damp) {
+      c->damp.release();
+      return;
+    }
+    c->damp.alloc(sizeof(double) * c->N);
+    ck(hipMemcpy(c->damp.p, damp, sizeof(double) * c->N, hipMemcpyHostToDevice), "copy damping");
+  });
+}
+
+int aero_set_f(aero_ctx* c, const double* f) {
+  return guarded([&] {
+    c->set_device();
+    const size_t cnt = (size_t)c->Q * c->N;
+    c->fcache.alloc(sizeof(double) * cnt);
+    ck(hipMemcpyAsync(c->fcache.p, f, sizeof(double) * cnt, hipMemcpyHostToDevice, c->stream), "upload f");
+    // F in compute precision, then B[side] = collide(F) (with LES omega from F)
+    const void* F = c->fcache.p;
+    if (c->d.storage != ST_F64) {
+      c->tmp.alloc(sizeof(float) * cnt);
+      ck(launch_compute_from_f64(c->d.storage, (long long)cnt, c->fcache.as<double>(), c->tmp.p, c->stream),
+         "convert f");
+      F = c->tmp.p;
+    }
+    const void* U = nullptr;
+    if (c->les_split()) {
+      ck(launch_macro_from_compute(c->d.storage, c->Q, c->N, F, c->usplit.p, c->stream), "macro");
+      U = c->usplit.p;
+    }
+    c->collide_field(F, U, c->side);
+    ck(hipStreamSynchronize(c->stream), "sync");
+    if (c->d.storage != ST_F64) c->tmp.release();
+    c->fcache_valid = true;
+    c->has_state = true;  // state defined; fcache holds it until the next step
+  });
+}
+
+int aero_get_f(aero_ctx* c, double* f) {
+  return guarded([&] {
+    c->set_device();
+    c->materialize_f();
+    ck(hipMemcpyAsync(f, c->fcache.p, sizeof(double) * c->Q * c->N, hipMemcpyDeviceToHost, c->stream), "download f");
+    ck(hipStreamSynchronize(c->stream), "sync");
+  });
+}
+
+int aero_set_g(aero_ctx* c, const double* g) {
+  return guarded([&] {
+    if (!c->d.thermal) throw std::runtime_error("scalar transport is off for this context");
+    c->set_device();
+    std::vector<double> inv;
+    inverse_stream(c, g, inv);
+    const size_t cnt = (size_t)c->Q * c->N;
+    c->gcache.alloc(sizeof(double) * cnt);
+    ck(hipMemcpy(c->gcache.p, g, sizeof(double) * cnt, hipMemcpyHostToDevice), "upload g");
+    c->tmp.alloc(sizeof(double) * cnt);
+    ck(hipMemcpy(c->tmp.p, inv.data(), sizeof(double) * cnt, hipMemcpyHostToDevice), "upload g");
+    const double shift = (c->d.storage == ST_F16S || c->d.storage == ST_BF16S) ? c->d.T_ref : 0.0;
+    ck(launch_store_from_f64(c->d.storage, c->Q, c->N, c->tmp.as<double>(), c->G[c->side].p, shift, c->stream),
+       "store g");
+    ck(hipStreamSynchronize(c->stream), "sync");
+    c->tmp.release();
+    c->gcache_valid = true;
+    c->g_skip_pending = 1;  // the given field may not carry the wall values yet
+  });
+}
+
+int aero_get_g(aero_ctx* c, double* g) {
+  return guarded([&] {
+    if (!c->d.thermal) throw std::runtime_error("scalar transport is off for this context");
+    c->set_device();
+    c->materialize_g();
+    ck(hipMemcpyAsync(g, c->gcache.p, sizeof(double) * c->Q * c->N, hipMemcpyDeviceToHost, c->stream), "download g");
+    ck(hipStreamSynchronize(c->stream), "sync");
+  });
+}
+
+int aero_set_outlet_prev(aero_ctx* c, const double* prev) {
+  return guarded([&] {
+    c->set_device();
+    ck(hipMemcpy(c->prev[c->prev_side].p, prev, sizeof(double) * c->Q * c->nrow, hipMemcpyHostToDevice),
+       "upload prev");
+  });
+}
+
+int aero_get_outlet_prev(aero_ctx* c, double* prev) {
+  return guarded([&] {
+    c->set_device();
+    ck(hipStreamSynchronize(c->stream), "sync");
+    ck(hipMemcpy(prev, c->prev[c->prev_side].p, sizeof(double) * c->Q * c->nrow, hipMemcpyDeviceToHost),
+       "download prev");
+  });
+}
+
+int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, double* hist) {
+  return guarded([&] {
+    if (nsteps <= 0) return;
+    if (!c->has_state) throw std::runtime_error("no state: call aero_set_f first");
+    c->set_device();
+    const bool need_planes = c->d.inlet == AERO_IN_FIELD || c->d.inlet == AERO_IN_SEM;
+    if (need_planes && !planes) throw std::invalid_argument("this inlet needs per-step planes");
+    const size_t plane_cnt = 3 * (size_t)c->nrow;
+    if (need_planes) {
+      c->planes.alloc(sizeof(double) * plane_cnt * nsteps);
+      ck(hipMemcpyAsync(c->planes.p, planes, sizeof(double) * plane_cnt * nsteps, hipMemcpyHostToDevice,
+                        c->stream),
+         "upload planes");
+    }
+    c->hist.alloc(sizeof(double) * NCOEF * (size_t)nsteps);
+    const double fd = c->force_denom(), md = c->moment_denom();
+    for (int k = 0; k < nsteps; ++k) {
+      const long long step_no = step0 + k + 1;
+      const int rd = c->side, wr = c->side ^ 1;
+      if (c->d.inlet == AERO_IN_RECYCLING) c->recycling_prepass(rd);
+      StepParams p = c->base_params();
+      p.fin = c->B[rd].p;
+      p.fout = c->B[wr].p;
+      p.gin = c->G[rd].p;
+      p.gout = c->G[wr].p;
+      p.prev_in = c->prev[c->prev_side].as<double>();
+      p.prev_out = c->prev[c->prev_side ^ 1].as<double>();
+      p.plane = need_planes ? c->planes.as<double>() + plane_cnt * k : nullptr;
+      p.phase = 0.17 * (double)step_no;
+      p.g_bc_skip = c->g_skip_pending;
+      p.commit = 1;
+      if (c->les_split()) {
+        p.fsplit = c->fsplit.p;
+        p.usplit = c->usplit.p;
+        c->launch_step_kernel(1, p);
+        c->collide_field(c->fsplit.p, c->usplit.p, wr);
+      } else {
+        c->launch_step_kernel(0, p);
+      }
+      ck(launch_reduce_forces(c->fpart.as<double>(), c->nrec, fd, md, c->hist.as<double>() + (size_t)NCOEF * k,
+                              c->stream),
+         "force reduction");
+      c->last_plane = p.plane;
+      c->last_step_no = step_no;
+      c->last_g_skip = c->g_skip_pending;
+      c->g_skip_pending = 0;
+      c->side = wr;
+      if (c->convective()) c->prev_side ^= 1;
+    }
+    c->invalidate();
+    if (hist) {
+      ck(hipMemcpyAsync(hist, c->hist.p, sizeof(double) * NCOEF * (size_t)nsteps, hipMemcpyDeviceToHost,
+                        c->stream),
+         "download history");
+    }
+    ck(hipStreamSynchronize(c->stream), "step sync");
+  });
+}
+
+int aero_macroscopic(aero_ctx* c, double* rho, double* ux, double* uy, double* uz) {
+  return guarded([&] {
+    c->set_device();
+    c->materialize_f();
+    c->tmp.alloc(sizeof(double) * 4 * c->N);
+    ck(launch_macro4_f64(c->Q, c->N, c->fcache.as<double>(), c->tmp.as<double>(), c->stream), "macro");
+    double* outs[4] = {rho, ux, uy, uz};
+    for (int k = 0; k < 4; ++k)
+      if (outs[k])
+        ck(hipMemcpyAsync(outs[k], c->tmp.as<double>() + (size_t)k * c->N, sizeof(double) * c->N,
+                          hipMemcpyDeviceToHost, c->stream),
+           "download macro");
+    ck(hipStreamSynchronize(c->stream), "sync");
+  });
+}
+
+int aero_check(aero_ctx* c, double* stats) {
+  return guarded([&] {
+    c->set_device();
+    c->materialize_f();
+    const int zc = c->Q == 19 ? c->d.nz / 2 : -1;
+    c->checkpart.alloc(sizeof(double) * NCHECK * c->nblocks);
+    c->checkout.alloc(sizeof(double) * NCHECK);
+    ck(launch_check_f64(c->Q, c->d.nx, c->d.ny, c->d.nz, zc, c->fcache.as<double>(), c->nblocks,
+                        c->checkpart.as<double>(), c->stream),
+       "check");
+    ck(launch_reduce_check(c->checkpart.as<double>(), c->nblocks, c->checkout.as<double>(), c->stream),
+       "check reduce");
+    ck(hipMemcpyAsync(stats, c->checkout.p, sizeof(double) * NCHECK, hipMemcpyDeviceToHost, c->stream), "download");
+    ck(hipStreamSynchronize(c->stream), "sync");
+  });
+}
+
+int aero_link_forces(aero_ctx* c, int32_t* cells, double* forces, int32_t* nrec) {
+  return guarded([&] {
+    c->set_device();
+    if (nrec) *nrec = c->nrec;
+    if (cells) std::memcpy(cells, c->rec_cells.data(), sizeof(int) * c->nrec);
+    if (forces && c->nrec) {
+      ck(hipStreamSynchronize(c->stream), "sync");
+      ck(hipMemcpy(forces, c->fpart.p, sizeof(double) * NFORCE * c->nrec, hipMemcpyDeviceToHost), "download");
+    }
+  });
+}
+
+int aero_synchronize(aero_ctx* c) {
+  return guarded([&] {
+    c->set_device();
+    ck(hipStreamSynchronize(c->stream), "sync");
+  });
+}
+
+int aero_device_bytes(aero_ctx* c, int64_t* bytes) {
+  return guarded([&] {
+    const DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags,
+                            &c->lrec, &c->row_start, &c->lq, &c->fpart, &c->sigma, &c->minv_d, &c->minv_f,
+                            &c->damp, &c->planes, &c->recyc, &c->hist, &c->fsplit, &c->usplit, &c->fcache,
+                            &c->gcache, &c->tmp, &c->checkpart, &c->checkout};
+    int64_t t = 0;
+    for (const DevBuf* b : bufs) t += (int64_t)b->bytes;
+    *bytes = t;
+  });
+}
+
+}  // extern "C"

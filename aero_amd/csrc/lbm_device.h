@@ -1,0 +1,194 @@
+// Device-side building blocks of the MI355X lattice-Boltzmann step.
+//
+// Populations live in HBM as structure-of-arrays planes, f[q][z][y][x] with x
+// fastest -- the reference's own (Q, Nz, Ny, Nx) C-order layout
+// (aero/lbm/solver3d.py:239-247) -- so one wavefront of 64 lanes reads 64
+// consecutive cells of one direction and every access is coalesced.
+#pragma once
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace aero {
+
+enum Collision { COLL_BGK = 0, COLL_TRT = 1, COLL_MRT = 2 };
+enum StorageKind { ST_F64 = 0, ST_F32 = 1, ST_F16S = 2, ST_BF16S = 3 };
+enum InletMode {
+  IN_NONE = 0,
+  IN_ZOUHE = 1,      // velocity Zou-He, optional travelling sine
+  IN_PRESSURE = 2,   // 2D Zou-He pressure
+  IN_FIELD = 3,      // per-row / per-cell target velocity planes
+  IN_SEM = 4,        // 3D SEM fluctuation planes
+  IN_RECYCLING = 5,  // 3D recycling-rescaling from plane x = nx-2
+};
+enum OutletMode { OUT_NONE = 0, OUT_CONVECTIVE = 1, OUT_ZEROGRAD = 2, OUT_PRESSURE = 3 };
+enum WallMode { WALL_NONE = 0, WALL_SLIP = 1, WALL_NOSLIP = 2, WALL_MOVING = 3 };
+enum LesMode { LES_OFF = 0, LES_FD_SMAG = 1, LES_FD_WALE = 2 };
+enum CellFlag { FLAG_SOLID = 1, FLAG_LINKS = 2 };
+
+// ------------------------------------------------------------------ lattices
+// D2Q9: aero/lbm/d2q9.py:18-41.  D3Q19: aero/lbm/d3q19.py:36-73.
+struct D2Q9 {
+  static constexpr int Q = 9;
+  static constexpr int DIM = 2;
+  static constexpr int EX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+  static constexpr int EY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+  static constexpr int EZ[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  static constexpr int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+  static constexpr double W[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9,
+                                  1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+  // MRT basis rows (aero/lbm/mrt2d.py:36-46), exact small integers
+  static constexpr int M[9][9] = {
+      {1, 1, 1, 1, 1, 1, 1, 1, 1},     {-4, -1, -1, -1, -1, 2, 2, 2, 2},
+      {4, -2, -2, -2, -2, 1, 1, 1, 1}, {0, 1, 0, -1, 0, 1, -1, -1, 1},
+      {0, -2, 0, 2, 0, 1, -1, -1, 1},  {0, 0, 1, 0, -1, 1, 1, -1, -1},
+      {0, 0, -2, 0, 2, 1, 1, -1, -1},  {0, 1, -1, 1, -1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 1, -1, 1, -1}};
+  // wall pair tables (dst <- src); boundary.py:162-226
+  static constexpr int NWALL = 3;
+  static constexpr int SLIP_BOT[3][2] = {{2, 4}, {5, 8}, {6, 7}};
+  static constexpr int SLIP_TOP[3][2] = {{4, 2}, {8, 5}, {7, 6}};
+  static constexpr int NOSLIP_BOT[3][2] = {{2, 4}, {5, 7}, {6, 8}};
+  static constexpr int NOSLIP_TOP[3][2] = {{4, 2}, {7, 5}, {8, 6}};
+  static constexpr int MOVE_SGN_BOT[3] = {0, 1, -1};
+  static constexpr int MOVE_SGN_TOP[3] = {0, -1, 1};
+};
+
+struct D3Q19 {
+  static constexpr int Q = 19;
+  static constexpr int DIM = 3;
+  static constexpr int EX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+  static constexpr int EY[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+  static constexpr int EZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+  static constexpr int OPP[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+  static constexpr double W[19] = {1.0 / 3,  1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18,
+                                   1.0 / 18, 1.0 / 18, 1.0 / 36, 1.0 / 36, 1.0 / 36,
+                                   1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36,
+                                   1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+  // d'Humieres-style basis as the reference writes it (aero/lbm/mrt3d.py:46-87);
+  // rank 16, so the inverse used with it is the host-computed pseudo-inverse.
+  static constexpr int M[19][19] = {
+      {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+      {-30, -11, -11, -11, -11, -11, -11, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8},
+      {12, -4, -4, -4, -4, -4, -4, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+      {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0},
+      {0, -4, 4, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0},
+      {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1},
+      {0, 0, 0, -4, 4, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1},
+      {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1},
+      {0, 0, 0, 0, 0, -4, 4, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1},
+      {0, 2, 2, -1, -1, -1, -1, 1, 1, 1, 1, 1, 1, 1, 1, -2, -2, -2, -2},
+      {0, 0, 0, 1, 1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, -1, -1, 1},
+      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0},
+      {0, -2, 2, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0},
+      {0, 0, 0, -2, 2, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1},
+      {0, 0, 0, 0, 0, -2, 2, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1},
+      {0, 0, 0, 0, 0, 0, 0, 1, -1, 1, -1, -1, 1, -1, 1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, -1, 1, -1, 1}};
+  // boundary3d.py:205-272
+  static constexpr int NWALL = 5;
+  static constexpr int SLIP_BOT[5][2] = {{3, 4}, {7, 9}, {8, 10}, {15, 16}, {17, 18}};
+  static constexpr int SLIP_TOP[5][2] = {{4, 3}, {9, 7}, {10, 8}, {16, 15}, {18, 17}};
+  static constexpr int NOSLIP_BOT[5][2] = {{3, 4}, {7, 10}, {8, 9}, {15, 18}, {17, 16}};
+  static constexpr int NOSLIP_TOP[5][2] = {{4, 3}, {10, 7}, {9, 8}, {18, 15}, {16, 17}};
+  static constexpr int MOVE_SGN_BOT[5] = {0, 1, -1, 0, 0};
+  static constexpr int MOVE_SGN_TOP[5] = {0, -1, 1, 0, 0};
+};
+
+// ------------------------------------------------------------------ storage
+// Populations are stored as T and computed in C.  The half forms store the
+// shifted value f - w_i*s (s = 1 for f, T_ref for the scalar g) so the 10/7
+// mantissa bits resolve the deviation from rest equilibrium, not its bulk.
+template <int ST> struct Storage;
+template <> struct Storage<ST_F64> {
+  using T = double;
+  using C = double;
+  static __device__ __forceinline__ C load(const T* p, size_t k, C) { return p[k]; }
+  static __device__ __forceinline__ void store(T* p, size_t k, C v, C) { p[k] = v; }
+};
+template <> struct Storage<ST_F32> {
+  using T = float;
+  using C = float;
+  static __device__ __forceinline__ C load(const T* p, size_t k, C) { return p[k]; }
+  static __device__ __forceinline__ void store(T* p, size_t k, C v, C) { p[k] = v; }
+};
+template <> struct Storage<ST_F16S> {
+  using T = __half;
+  using C = float;
+  static __device__ __forceinline__ C load(const T* p, size_t k, C s) { return __half2float(p[k]) + s; }
+  static __device__ __forceinline__ void store(T* p, size_t k, C v, C s) { p[k] = __float2half(v - s); }
+};
+template <> struct Storage<ST_BF16S> {
+  using T = __hip_bfloat16;
+  using C = float;
+  static __device__ __forceinline__ C load(const T* p, size_t k, C s) { return __bfloat162float(p[k]) + s; }
+  static __device__ __forceinline__ void store(T* p, size_t k, C v, C s) { p[k] = __float2bfloat16(v - s); }
+};
+
+// Link-cell record: one per fluid cell that has at least one surface link.
+struct LinkCell {
+  int cell;       // linear cell index z*ny*nx + y*nx + x
+  uint32_t mask;  // bit i set: direction i is a link (neighbour along e_i solid)
+  int qoff;       // offset of this cell's q values (one per set bit, ascending i)
+  int pad;
+};
+
+constexpr int NCOEF = 20;  // per-step history row: 8 coefficients + 12 raw sums
+constexpr int NFORCE = 12; // Fx Fy Fz Mx My Mz Fpx Fpy Fpz Fvx Fvy Fvz
+constexpr int NCHECK = 5;  // nan, nonpositive rho, max|u|, sum rho, count
+constexpr int BLOCK = 256;
+
+// Everything one step launch needs; passed by value as the kernel argument.
+struct StepParams {
+  int nx, ny, nz;
+  int nrow;  // nz * ny
+  long long ncell;
+  int nblocks;       // logical blocks covering ncell
+  int nblocks_grid;  // launched blocks (multiple of 8, XCD remap)
+  const void* fin;
+  void* fout;
+  const void* gin;
+  void* gout;
+  void* fsplit;  // split mode: post-BC populations, compute type, Q x N
+  void* usplit;  // split mode: ux, uy, uz, compute type, 3 x N
+  const uint8_t* flags;
+  const LinkCell* lrec;
+  const int* row_start;
+  const float* lq;
+  double* fpart;  // NFORCE doubles per link cell
+  const double* prev_in;
+  double* prev_out;
+  const double* sigma;  // sponge sigma(x), nx entries, or null
+  const double* plane;  // inlet planes, 3 x nrow
+  const double* recyc;  // recycling: rho, ux, uy, uz per row-plane cell (4 x nrow), then scale
+  const double* minv_d;
+  const float* minv_f;
+  double mrt_s[19];
+  double omega, trt_sm, trt_lambda, u0, u_conv;
+  double inlet_amp, phase;
+  double rho_in, rho_out, u_top, u_bot;
+  double bf[3], bf_coeff;
+  double omega_T, gbeta, T_ref, T_hot, T_cold, gshift;
+  double center[3];
+  int inlet_mode, outlet_mode, wall_mode, body_force_on, buoyancy, g_bc_skip, bouzidi;
+  int commit;    // write side effects (outlet column state, scalar, forces)
+  int out_mode;  // materialize: 0 f64 field, 1 f64 macroscopic, 2 check partials
+  int check_z;   // materialize/check: restrict to this z plane (-1: all)
+  double* fmat;
+  double* check_part;
+};
+
+__device__ __forceinline__ int logical_block(const StepParams& p) {
+  // Consecutive logical blocks land on the same XCD (hardware dispatches
+  // workgroup b to XCD b % 8), so the cache lines that neighbouring blocks
+  // share through the +-1 x shifts are reused in one L2 instead of eight.
+  const int b = blockIdx.x;
+  const int per = p.nblocks_grid >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
+}  // namespace aero

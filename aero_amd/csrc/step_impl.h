@@ -1,0 +1,784 @@
+// The fused pull-scheme step for D2Q9 / D3Q19 (templated on lattice,
+// collision, storage and the scalar transport).
+//
+// State convention: the buffer B read by step k holds the post-collision
+// populations of step k (the reference's f_pre, aero/lbm/solver.py:348).  One
+// launch completes reference step k and starts step k+1:
+//   pull-stream   f_new[i](x) = B[i](x - e_i), periodic in every direction
+//                 (the inverse of the reference's push, kernels.py:94-115)
+//   [3D body force (solver3d.py:411-423)]
+//   bounce-back / Bouzidi on the cell's own links   (solver.py:379-383)
+//   inlet (x = 0) -> outlet (x = nx-1) -> sponge -> walls (y = 0, ny-1)
+//   scalar + Boussinesq force                        (solver.py:427-443)
+//   momentum-exchange partial sums (FP64) at link cells (solver.py:449-463)
+//   collision of step k+1 -> B'                     (kernels.py:42-92, ...)
+// Every boundary condition in the reference reads only the cell it writes,
+// the reference's f_pre at that cell, or (outlet) the cell at x = nx-2 in the
+// same row; that cell's pre-outlet state is re-derived by the thread at
+// x = nx-1 from B, so one launch per step is enough and no grid-wide barrier
+// is needed.
+#pragma once
+
+#include "lbm_device.h"
+
+namespace aero {
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+template <class C>
+__device__ __forceinline__ C feq_i(C w, C rho, C eu, C usq) {
+  return w * rho * (C(1) + C(3) * eu + C(4.5) * eu * eu - C(1.5) * usq);
+}
+
+// Pull gather of all Q populations of cell (x, y, z) from buffer B.
+template <class L, int ST>
+__device__ __forceinline__ void gather(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
+                                       int x, int y, int z, typename Storage<ST>::C shift_scale,
+                                       typename Storage<ST>::C fn[L::Q]) {
+  using S = Storage<ST>;
+  using C = typename S::C;
+  const size_t N = (size_t)p.ncell;
+  const int xm = x == 0 ? p.nx - 1 : x - 1, xp = x == p.nx - 1 ? 0 : x + 1;
+  const int ym = y == 0 ? p.ny - 1 : y - 1, yp = y == p.ny - 1 ? 0 : y + 1;
+  int zm = z, zp = z;
+  if (L::DIM == 3) {
+    zm = z == 0 ? p.nz - 1 : z - 1;
+    zp = z == p.nz - 1 ? 0 : z + 1;
+  }
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    const int xs = L::EX[i] == 0 ? x : (L::EX[i] > 0 ? xm : xp);
+    const int ys = L::EY[i] == 0 ? y : (L::EY[i] > 0 ? ym : yp);
+    const int zs = L::EZ[i] == 0 ? z : (L::EZ[i] > 0 ? zm : zp);
+    const unsigned src = ((unsigned)zs * (unsigned)p.ny + (unsigned)ys) * (unsigned)p.nx + (unsigned)xs;
+    fn[i] = S::load(B + (size_t)i * N, src, C(L::W[i]) * shift_scale);
+  }
+}
+
+// Locate the link-cell record of `cell` (records sorted by cell, indexed by row).
+__device__ __forceinline__ int find_link_record(const StepParams& p, int cell, int row) {
+  int lo = p.row_start[row], hi = p.row_start[row + 1] - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int c = p.lrec[mid].cell;
+    if (c == cell) return mid;
+    if (c < cell) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return -1;
+}
+
+// Bounce-back (boundary.py:283-300) or Bouzidi (boundary.py:303-352) on the
+// links of one cell.  Loads the cell's own post-collision populations into
+// fpre (also needed by the force sums).  Returns the record index.
+template <class L, int ST>
+__device__ __forceinline__ int apply_links(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
+                                           int cell, int x, int y, int z,
+                                           typename Storage<ST>::C fn[L::Q],
+                                           typename Storage<ST>::C fpre[L::Q]) {
+  using S = Storage<ST>;
+  using C = typename S::C;
+  const size_t N = (size_t)p.ncell;
+  const int rec = find_link_record(p, cell, z * p.ny + y);
+  if (rec < 0) return -1;
+  const LinkCell r = p.lrec[rec];
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) fpre[i] = S::load(B + (size_t)i * N, (size_t)cell, C(L::W[i]));
+#pragma unroll
+  for (int i = 1; i < L::Q; ++i) {
+    if ((r.mask >> i) & 1u) {
+      const int o = L::OPP[i];
+      if (!p.bouzidi) {
+        fn[o] = fpre[i];
+      } else {
+        const int k = __popc(r.mask & ((1u << i) - 1u));
+        const C q = (C)p.lq[r.qoff + k];
+        if (q < C(0.5)) {
+          fn[o] = C(2) * q * fpre[i] + (C(1) - C(2) * q) * fpre[o];
+        } else {
+          const int xn = clampi(x + L::EX[o], 0, p.nx - 1);
+          const int yn = clampi(y + L::EY[o], 0, p.ny - 1);
+          const int zn = clampi(z + L::EZ[o], 0, p.nz - 1);
+          const size_t nb = ((size_t)zn * p.ny + yn) * p.nx + xn;
+          const C fnb = S::load(B + (size_t)i * N, nb, C(L::W[i]));
+          const C inv2q = C(1) / (C(2) * q);
+          fn[o] = inv2q * fpre[i] + (C(1) - inv2q) * fnb;
+        }
+      }
+    }
+  }
+  return rec;
+}
+
+// Uniform Guo body force on a fluid cell (ibm_guo.py:264-301).
+template <class L, class C>
+__device__ __forceinline__ void body_force(const StepParams& p, C fn[L::Q]) {
+  C rho = 0, mx = 0, my = 0, mz = 0;
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    rho += fn[i];
+    mx += C(L::EX[i]) * fn[i];
+    my += C(L::EY[i]) * fn[i];
+    mz += C(L::EZ[i]) * fn[i];
+  }
+  if (!(rho > C(0))) return;
+  const C ux = mx / rho, uy = my / rho, uz = mz / rho;
+  const C fx = (C)p.bf[0], fy = (C)p.bf[1], fz = (C)p.bf[2];
+  const C cs2 = C(1.0 / 3.0);
+  const C uf = ux * fx + uy * fy + uz * fz;
+  const C coeff = (C)p.bf_coeff;
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    const C eu = C(L::EX[i]) * ux + C(L::EY[i]) * uy + C(L::EZ[i]) * uz;
+    const C ef = C(L::EX[i]) * fx + C(L::EY[i]) * fy + C(L::EZ[i]) * fz;
+    const C term = (ef - uf) / cs2 + (eu * ef) / (cs2 * cs2);
+    fn[i] += coeff * C(L::W[i]) * rho * term;
+  }
+}
+
+// ---------------------------------------------------------------- inlets
+template <class C>
+__device__ __forceinline__ void inlet_2d(const StepParams& p, int y, C f[9]) {
+  if (p.inlet_mode == IN_ZOUHE) {  // boundary.py:15-45
+    const C u0 = (C)p.u0;
+    const C rho = ((f[0] + f[2] + f[4]) + C(2) * (f[3] + f[6] + f[7])) / (C(1) - u0);
+    f[1] = f[3] + C(2.0 / 3.0) * rho * u0;
+    f[5] = f[7] - C(0.5) * (f[2] - f[4]) + C(1.0 / 6.0) * rho * u0;
+    f[8] = f[6] + C(0.5) * (f[2] - f[4]) + C(1.0 / 6.0) * rho * u0;
+    if (p.inlet_amp > 0.0) {
+      const double ny = p.ny > 1 ? p.ny : 1;
+      const C uy = (C)(p.inlet_amp * p.u0 * sin(2.0 * M_PI * (double)y / ny + p.phase));
+      f[2] += uy * C(0.25);
+      f[4] -= uy * C(0.25);
+      f[5] += uy * C(0.15);
+      f[8] -= uy * C(0.15);
+    }
+  } else if (p.inlet_mode == IN_FIELD) {  // boundary.py:48-65
+    const C ux_t = (C)p.plane[y], uy_t = (C)p.plane[p.nrow + y];
+    C den = C(1) - ux_t;
+    if (den < C(1e-8)) den = C(1e-8);
+    const C rho = ((f[0] + f[2] + f[4]) + C(2) * (f[3] + f[6] + f[7])) / den;
+    const C cy = rho * uy_t;
+    f[1] = f[3] + C(2.0 / 3.0) * rho * ux_t;
+    f[5] = f[7] - C(0.5) * cy + C(1.0 / 6.0) * rho * ux_t;
+    f[8] = f[6] + C(0.5) * cy + C(1.0 / 6.0) * rho * ux_t;
+  } else if (p.inlet_mode == IN_PRESSURE) {  // boundary.py:116-136
+    const C rin = (C)p.rho_in;
+    const C ux = C(1) - (f[0] + f[2] + f[4] + C(2) * (f[3] + f[6] + f[7])) / rin;
+    f[1] = f[3] + C(2.0 / 3.0) * rin * ux;
+    f[5] = f[7] - C(0.5) * (f[2] - f[4]) + C(1.0 / 6.0) * rin * ux;
+    f[8] = f[6] + C(0.5) * (f[2] - f[4]) + C(1.0 / 6.0) * rin * ux;
+  }
+}
+
+template <class C>
+__device__ __forceinline__ void inlet_3d(const StepParams& p, int y, int z, C f[19]) {
+  const int row = z * p.ny + y;
+  if (p.inlet_mode == IN_RECYCLING) {  // boundary3d.py:172-198
+    const double scale = p.recyc[4 * p.nrow];
+    const C rho = (C)p.recyc[row];
+    const C ux = (C)(p.recyc[p.nrow + row] * scale);
+    const C uy = (C)(p.recyc[2 * p.nrow + row] * scale);
+    const C uz = (C)(p.recyc[3 * p.nrow + row] * scale);
+    const C usq = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+      const C eu = C(D3Q19::EX[i]) * ux + C(D3Q19::EY[i]) * uy + C(D3Q19::EZ[i]) * uz;
+      f[i] = feq_i(C(D3Q19::W[i]), rho, eu, usq);
+    }
+    return;
+  }
+  const C Fm = f[2] + f[8] + f[10] + f[12] + f[14];
+  const C Fn = f[0] + f[3] + f[4] + f[5] + f[6] + f[15] + f[16] + f[17] + f[18];
+  const C cyb = f[3] - f[4] + f[15] - f[16] + f[17] - f[18];
+  const C czb = f[5] - f[6] + f[15] + f[16] - f[17] - f[18];
+  C ux, rho, cy, cz;
+  if (p.inlet_mode == IN_ZOUHE) {  // boundary3d.py:24-68 (uz sine only, solver3d.py:447-452)
+    ux = (C)p.u0;
+    rho = (C(2) * Fm + Fn) / (C(1) - ux);
+    cy = cyb;
+    cz = czb;
+    if (p.inlet_amp > 0.0) {
+      const double nz = p.nz > 1 ? p.nz : 1;
+      const C uz = (C)(p.inlet_amp * p.u0 * sin(2.0 * M_PI * (double)z / nz + p.phase));
+      cz = cz + rho * uz;
+    }
+  } else if (p.inlet_mode == IN_FIELD) {  // boundary3d.py:71-93
+    ux = (C)p.plane[row];
+    C den = C(1) - ux;
+    if (den < C(1e-8)) den = C(1e-8);
+    rho = (C(2) * Fm + Fn) / den;
+    cy = rho * (C)p.plane[p.nrow + row];
+    cz = rho * (C)p.plane[2 * p.nrow + row];
+  } else if (p.inlet_mode == IN_SEM) {  // boundary3d.py:96-137
+    ux = (C)p.u0 + (C)p.plane[row];
+    C den = C(1) - ux;
+    if (den < C(1e-10)) den = C(1e-10);
+    rho = (C(2) * Fm + Fn) / den;
+    cy = cyb - rho * (C)p.plane[p.nrow + row];
+    cz = czb - rho * (C)p.plane[2 * p.nrow + row];
+  } else {
+    return;
+  }
+  f[1] = f[2] + C(1.0 / 3.0) * rho * ux;
+  f[7] = f[10] + C(1.0 / 6.0) * rho * ux - C(0.5) * cy;
+  f[9] = f[8] + C(1.0 / 6.0) * rho * ux + C(0.5) * cy;
+  f[11] = f[14] + C(1.0 / 6.0) * rho * ux - C(0.5) * cz;
+  f[13] = f[12] + C(1.0 / 6.0) * rho * ux + C(0.5) * cz;
+}
+
+template <class L, class C>
+__device__ __forceinline__ void apply_inlet(const StepParams& p, int y, int z, C fn[L::Q]) {
+  if constexpr (L::DIM == 2) inlet_2d<C>(p, y, fn);
+  else inlet_3d<C>(p, y, z, fn);
+}
+
+// State of a cell after streaming, body force, links and inlet -- i.e. what
+// the outlet condition reads at x = nx-2 (boundary.py:68-113).
+template <class L, int ST>
+__device__ __forceinline__ int pre_outlet_state(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
+                                                int cell, int x, int y, int z, uint8_t flag,
+                                                typename Storage<ST>::C fn[L::Q],
+                                                typename Storage<ST>::C fpre[L::Q]) {
+  using C = typename Storage<ST>::C;
+  gather<L, ST>(p, B, x, y, z, C(1), fn);
+  if (L::DIM == 3 && p.body_force_on && !(flag & FLAG_SOLID)) body_force<L, C>(p, fn);
+  int rec = -1;
+  if (flag & FLAG_LINKS) rec = apply_links<L, ST>(p, B, cell, x, y, z, fn, fpre);
+  if (x == 0) apply_inlet<L, C>(p, y, z, fn);
+  return rec;
+}
+
+// Full post-boundary state of one cell (the reference's self.f before the
+// scalar step).  With commit set, the convective outlet state is advanced.
+template <class L, int ST>
+__device__ __forceinline__ int post_bc_state(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
+                                             int cell, int x, int y, int z, uint8_t flag,
+                                             typename Storage<ST>::C fn[L::Q],
+                                             typename Storage<ST>::C fpre[L::Q]) {
+  using C = typename Storage<ST>::C;
+  constexpr int Q = L::Q;
+  const int rec = pre_outlet_state<L, ST>(p, B, cell, x, y, z, flag, fn, fpre);
+  if (x == p.nx - 1 && p.outlet_mode != OUT_NONE) {
+    if (p.outlet_mode == OUT_PRESSURE) {  // 2D only, boundary.py:139-159
+      if constexpr (L::DIM == 2) {
+        const C rout = (C)p.rho_out;
+        const C ux = C(-1) + (fn[0] + fn[2] + fn[4] + C(2) * (fn[1] + fn[5] + fn[8])) / rout;
+        fn[3] = fn[1] - C(2.0 / 3.0) * rout * ux;
+        fn[7] = fn[5] + C(0.5) * (fn[2] - fn[4]) - C(1.0 / 6.0) * rout * ux;
+        fn[6] = fn[8] - C(0.5) * (fn[2] - fn[4]) - C(1.0 / 6.0) * rout * ux;
+      }
+    } else {
+      C nb[Q], nbpre[Q];
+      const int xn = p.nx - 2;
+      const int cn = cell - 1;
+      pre_outlet_state<L, ST>(p, B, cn, xn, y, z, p.flags[cn], nb, nbpre);
+      if (p.outlet_mode == OUT_ZEROGRAD) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) fn[i] = nb[i];
+      } else {  // convective, boundary.py:78-113
+        double u = p.u_conv < 0.0 ? 0.0 : (p.u_conv > 1.0 ? 1.0 : p.u_conv);
+        const C cu = (C)u;
+        const int row = z * p.ny + y;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+          const size_t k = (size_t)i * p.nrow + row;
+          const C v = (C(1) - cu) * (C)p.prev_in[k] + cu * nb[i];
+          fn[i] = v;
+          if (p.commit) p.prev_out[k] = (double)v;
+        }
+      }
+    }
+  }
+  if (p.sigma != nullptr) {  // sponge.py:24-70
+    const double s = p.sigma[x];
+    if (s > 0.0) {
+      const C cs = (C)s;
+      C rho = 0;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) rho += fn[i];
+      const C ut = (C)p.u0;
+      const C usq = ut * ut;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const C eu = C(L::EX[i]) * ut;
+        fn[i] = (C(1) - cs) * fn[i] + cs * feq_i(C(L::W[i]), rho, eu, usq);
+      }
+    }
+  }
+  if (p.wall_mode != WALL_NONE && (y == 0 || y == p.ny - 1)) {  // boundary.py:162-226
+    const bool moving = p.wall_mode == WALL_MOVING;
+    const bool slip = p.wall_mode == WALL_SLIP;
+    C rsum = 0;
+    if (moving) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) rsum += fn[i];
+      rsum = rsum > C(1e-12) ? rsum : C(1e-12);
+    }
+    if (y == 0) {
+      const C add = moving ? C(6.0 * (1.0 / 36.0)) * rsum * (C)p.u_bot : C(0);
+#pragma unroll
+      for (int k = 0; k < L::NWALL; ++k) {
+        const int d = slip ? L::SLIP_BOT[k][0] : L::NOSLIP_BOT[k][0];
+        const int s = slip ? L::SLIP_BOT[k][1] : L::NOSLIP_BOT[k][1];
+        fn[d] = L::MOVE_SGN_BOT[k] == 0 || !moving ? fn[s] : fn[s] + C(L::MOVE_SGN_BOT[k]) * add;
+      }
+    }
+    if (y == p.ny - 1) {
+      const C add = moving ? C(6.0 * (1.0 / 36.0)) * rsum * (C)p.u_top : C(0);
+#pragma unroll
+      for (int k = 0; k < L::NWALL; ++k) {
+        const int d = slip ? L::SLIP_TOP[k][0] : L::NOSLIP_TOP[k][0];
+        const int s = slip ? L::SLIP_TOP[k][1] : L::NOSLIP_TOP[k][1];
+        fn[d] = L::MOVE_SGN_TOP[k] == 0 || !moving ? fn[s] : fn[s] + C(L::MOVE_SGN_TOP[k]) * add;
+      }
+    }
+  }
+  return rec;
+}
+
+// Scalar transport + Boussinesq force (thermal.py:47-124, 149-222).  Pulls
+// the previous scalar post-collision state from G, applies the y-wall
+// Dirichlet values (unless the state was just set from a raw field), adds the
+// buoyancy force to fn and, with commit, writes the new post-collision g.
+template <class L, int ST>
+__device__ __forceinline__ void scalar_step(const StepParams& p, int cell, int x, int y, int z, bool solid,
+                                            typename Storage<ST>::C fn[L::Q], bool write_g) {
+  using S = Storage<ST>;
+  using C = typename S::C;
+  constexpr int Q = L::Q;
+  const size_t N = (size_t)p.ncell;
+  C g[Q];
+  gather<L, ST>(p, (const typename S::T*)p.gin, x, y, z, (C)p.gshift, g);
+  if (!p.g_bc_skip) {
+    if (y == 0) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) g[i] = C(L::W[i]) * (C)p.T_hot;
+    }
+    if (y == p.ny - 1) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) g[i] = C(L::W[i]) * (C)p.T_cold;
+    }
+  }
+  C T = 0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) T += g[i];
+  C rho = 0, mx = 0, my = 0, mz = 0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    rho += fn[i];
+    mx += C(L::EX[i]) * fn[i];
+    my += C(L::EY[i]) * fn[i];
+    mz += C(L::EZ[i]) * fn[i];
+  }
+  const C inv = rho > C(0) ? C(1) / rho : C(0);
+  const C ux = inv * mx, uy = inv * my, uz = inv * mz;
+  if (p.buoyancy) {
+    const C buoy = (C)p.gbeta * (T - (C)p.T_ref);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) fn[i] += C(L::W[i]) * C(3) * rho * buoy * C(L::EY[i]);
+  }
+  if (write_g) {
+    const C om = (C)p.omega_T;
+    typename S::T* gout = (typename S::T*)p.gout;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const C eu = C(L::EX[i]) * ux + C(L::EY[i]) * uy + C(L::EZ[i]) * uz;
+      const C geq = C(L::W[i]) * T * (C(1) + C(3) * eu);
+      const C gp = solid ? C(L::W[i]) * T : g[i] - om * (g[i] - geq);
+      S::store(gout + (size_t)i * N, (size_t)cell, gp, C(L::W[i]) * (C)p.gshift);
+    }
+  }
+}
+
+// Momentum exchange at one link cell (forces.py:15-75, forces3d.py:8-66 and
+// the moment sums, forces.py:106-139 / forces3d.py:398-425), FP64.
+template <class L, class C>
+__device__ __forceinline__ void link_forces(const StepParams& p, const LinkCell& r, int x, int y, int z,
+                                            const C fpre[L::Q], const C fn[L::Q], double out[NFORCE]) {
+  double rho = 0, mx = 0, my = 0, mz = 0;
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    const double v = (double)fpre[i];
+    rho += v;
+    mx += L::EX[i] * v;
+    my += L::EY[i] * v;
+    mz += L::EZ[i] * v;
+  }
+  const double inv = rho > 0.0 ? 1.0 / rho : 0.0;
+  const double ux = inv * mx, uy = inv * my, uz = inv * mz;
+  const double usq = ux * ux + uy * uy + uz * uz;
+  const double rx = (double)x - p.center[0], ry = (double)y - p.center[1], rz = (double)z - p.center[2];
+#pragma unroll
+  for (int k = 0; k < NFORCE; ++k) out[k] = 0.0;
+#pragma unroll
+  for (int i = 1; i < L::Q; ++i) {
+    if ((r.mask >> i) & 1u) {
+      const int o = L::OPP[i];
+      const double fo = (double)fpre[i], fi = (double)fn[o];
+      const double mom = fo + fi;
+      const double dfx = L::EX[i] * mom, dfy = L::EY[i] * mom, dfz = L::EZ[i] * mom;
+      out[0] += dfx;
+      out[1] += dfy;
+      out[2] += dfz;
+      out[3] += ry * dfz - rz * dfy;
+      out[4] += rz * dfx - rx * dfz;
+      out[5] += rx * dfy - ry * dfx;
+      const double feo = feq_i<double>(L::W[i], rho, L::EX[i] * ux + L::EY[i] * uy + L::EZ[i] * uz, usq);
+      const double fei = feq_i<double>(L::W[o], rho, L::EX[o] * ux + L::EY[o] * uy + L::EZ[o] * uz, usq);
+      const double mp = feo + fei;
+      const double mv = (fo - feo) + (fi - fei);
+      out[6] += L::EX[i] * mp;
+      out[7] += L::EY[i] * mp;
+      out[8] += L::EZ[i] * mp;
+      out[9] += L::EX[i] * mv;
+      out[10] += L::EY[i] * mv;
+      out[11] += L::EZ[i] * mv;
+    }
+  }
+}
+
+// Collision (BGK kernels.py:42-92, TRT kernels_trt.py:228-277,
+// MRT kernels_mrt.py:30-82 / kernels3d_mrt.py:158-217).  u = 0 in solids.
+template <class L, int COLL, class C>
+__device__ __forceinline__ void collide(const double* mrt_s, const double* minv_d, const float* minv_f,
+                                        bool solid, C om, C sm, const C fn[L::Q], C out[L::Q]) {
+  constexpr int Q = L::Q;
+  C rho = 0, mx = 0, my = 0, mz = 0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    rho += fn[i];
+    mx += C(L::EX[i]) * fn[i];
+    my += C(L::EY[i]) * fn[i];
+    mz += C(L::EZ[i]) * fn[i];
+  }
+  const C inv = rho > C(0) ? C(1) / rho : C(0);
+  C ux = inv * mx, uy = inv * my, uz = inv * mz;
+  if (solid) {
+    ux = C(0);
+    uy = C(0);
+    uz = C(0);
+  }
+  const C usq = ux * ux + uy * uy + uz * uz;
+  C fe[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const C eu = C(L::EX[i]) * ux + C(L::EY[i]) * uy + C(L::EZ[i]) * uz;
+    fe[i] = feq_i(C(L::W[i]), rho, eu, usq);
+  }
+  if constexpr (COLL == COLL_BGK) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) out[i] = (C(1) - om) * fn[i] + om * fe[i];
+  } else if constexpr (COLL == COLL_TRT) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int j = L::OPP[i];
+      const C fp = C(0.5) * (fn[i] + fn[j]);
+      const C fm = C(0.5) * (fn[i] - fn[j]);
+      const C ep = C(0.5) * (fe[i] + fe[j]);
+      const C em = C(0.5) * (fe[i] - fe[j]);
+      out[i] = fe[i] + (C(1) - om) * (fp - ep) + (C(1) - sm) * (fm - em);
+    }
+  } else {
+    C mp[Q];
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+      C m = 0, me = 0;
+#pragma unroll
+      for (int b = 0; b < Q; ++b) {
+        if (L::M[a][b] != 0) {
+          m += C(L::M[a][b]) * fn[b];
+          me += C(L::M[a][b]) * fe[b];
+        }
+      }
+      mp[a] = m - C(mrt_s[a]) * (m - me);
+    }
+    const C* minv;
+    if constexpr (sizeof(C) == 8) minv = (const C*)minv_d;
+    else minv = (const C*)minv_f;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      C acc = 0;
+#pragma unroll
+      for (int a = 0; a < Q; ++a) acc += minv[i * Q + a] * mp[a];
+      out[i] = acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- kernels
+// KMODE 0: fused step (collide into fout).  KMODE 1: split step -- commit the
+// boundary state and write the post-BC populations + velocity for the LES
+// finite-difference pass.
+template <class L, int COLL, int ST, bool THERMAL, int KMODE>
+__global__ __launch_bounds__(BLOCK) void step_kernel(StepParams p) {
+  using S = Storage<ST>;
+  using C = typename S::C;
+  using T = typename S::T;
+  constexpr int Q = L::Q;
+  const int lb = logical_block(p);
+  if (lb >= p.nblocks) return;
+  const long long c64 = (long long)lb * BLOCK + threadIdx.x;
+  if (c64 >= p.ncell) return;
+  const int cell = (int)c64;
+  const int x = cell % p.nx;
+  const int yz = cell / p.nx;
+  const int y = yz % p.ny;
+  const int z = yz / p.ny;
+  const uint8_t flag = p.flags[cell];
+  const bool solid = flag & FLAG_SOLID;
+  const T* B = (const T*)p.fin;
+  C fn[Q], fpre[Q];
+  const int rec = post_bc_state<L, ST>(p, B, cell, x, y, z, flag, fn, fpre);
+  if constexpr (THERMAL) scalar_step<L, ST>(p, cell, x, y, z, solid, fn, true);
+  if (rec >= 0) {
+    double fo[NFORCE];
+    link_forces<L, C>(p, p.lrec[rec], x, y, z, fpre, fn, fo);
+#pragma unroll
+    for (int k = 0; k < NFORCE; ++k) p.fpart[(size_t)rec * NFORCE + k] = fo[k];
+  }
+  const size_t N = (size_t)p.ncell;
+  if constexpr (KMODE == 0) {
+    C out[Q];
+    collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, (C)p.omega, (C)p.trt_sm, fn, out);
+    T* Bo = (T*)p.fout;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) S::store(Bo + (size_t)i * N, (size_t)cell, out[i], C(L::W[i]));
+  } else {
+    C* F = (C*)p.fsplit;
+    C* U = (C*)p.usplit;
+    C rho = 0, mx = 0, my = 0, mz = 0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      F[(size_t)i * N + cell] = fn[i];
+      rho += fn[i];
+      mx += C(L::EX[i]) * fn[i];
+      my += C(L::EY[i]) * fn[i];
+      mz += C(L::EZ[i]) * fn[i];
+    }
+    const C inv = rho > C(0) ? C(1) / rho : C(0);
+    U[cell] = inv * mx;
+    U[N + cell] = inv * my;
+    U[2 * N + cell] = inv * mz;
+  }
+}
+
+// Side-effect-free re-derivation of the reference's post-boundary state of
+// the last completed step (reads the previous buffer side).  out_mode 0
+// writes the f64 field, 1 writes rho/ux/uy/uz, 2 reduces the stability check.
+template <class L, int ST, bool THERMAL>
+__global__ __launch_bounds__(BLOCK) void materialize_kernel(StepParams p) {
+  using S = Storage<ST>;
+  using C = typename S::C;
+  using T = typename S::T;
+  constexpr int Q = L::Q;
+  __shared__ double red[BLOCK / 64][NCHECK];
+  const int lb = logical_block(p);
+  if (lb >= p.nblocks) return;
+  const long long c64 = (long long)lb * BLOCK + threadIdx.x;
+  double v_nan = 0, v_neg = 0, v_umax = 0, v_rho = 0, v_cnt = 0;
+  if (c64 < p.ncell) {
+    const int cell = (int)c64;
+    const int x = cell % p.nx;
+    const int yz = cell / p.nx;
+    const int y = yz % p.ny;
+    const int z = yz / p.ny;
+    const bool in_plane = p.check_z < 0 || z == p.check_z;
+    if (in_plane) {
+      const uint8_t flag = p.flags[cell];
+      C fn[Q], fpre[Q];
+      post_bc_state<L, ST>(p, (const T*)p.fin, cell, x, y, z, flag, fn, fpre);
+      if constexpr (THERMAL) scalar_step<L, ST>(p, cell, x, y, z, flag & FLAG_SOLID, fn, false);
+      const size_t N = (size_t)p.ncell;
+      if (p.out_mode == 0) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) p.fmat[(size_t)i * N + cell] = (double)fn[i];
+      } else {
+        double rho = 0, mx = 0, my = 0, mz = 0;
+        bool nan = false;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+          const double v = (double)fn[i];
+          nan |= isnan(v);
+          rho += v;
+          mx += L::EX[i] * v;
+          my += L::EY[i] * v;
+          mz += L::EZ[i] * v;
+        }
+        const double inv = rho > 0.0 ? 1.0 / rho : 0.0;
+        if (p.out_mode == 1) {
+          p.fmat[cell] = rho;
+          p.fmat[N + cell] = inv * mx;
+          p.fmat[2 * N + cell] = inv * my;
+          p.fmat[3 * N + cell] = inv * mz;
+        } else {
+          // check_stability (diagnostics.py:32-71): |u| uses ux, uy only
+          v_nan = nan ? 1.0 : 0.0;
+          v_neg = rho <= 0.0 ? 1.0 : 0.0;
+          const double ux = inv * mx, uy = inv * my;
+          v_umax = sqrt(ux * ux + uy * uy);
+          if (isnan(v_umax)) v_umax = 0.0;
+          v_rho = rho;
+          v_cnt = 1.0;
+        }
+      }
+    }
+  }
+  if (p.out_mode == 2) {
+    double v[NCHECK] = {v_nan, v_neg, v_umax, v_rho, v_cnt};
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      v[0] = fmax(v[0], __shfl_xor(v[0], off));
+      v[1] = fmax(v[1], __shfl_xor(v[1], off));
+      v[2] = fmax(v[2], __shfl_xor(v[2], off));
+      v[3] += __shfl_xor(v[3], off);
+      v[4] += __shfl_xor(v[4], off);
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+      for (int k = 0; k < NCHECK; ++k) red[wv][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double r[NCHECK] = {0, 0, 0, 0, 0};
+      for (int w = 0; w < BLOCK / 64; ++w) {
+        r[0] = fmax(r[0], red[w][0]);
+        r[1] = fmax(r[1], red[w][1]);
+        r[2] = fmax(r[2], red[w][2]);
+        r[3] += red[w][3];
+        r[4] += red[w][4];
+      }
+#pragma unroll
+      for (int k = 0; k < NCHECK; ++k) p.check_part[(size_t)lb * NCHECK + k] = r[k];
+    }
+  }
+}
+
+// Collision from an explicit post-boundary field F (compute type): used after
+// set_f and by the LES split step.  les: LES_OFF or a finite-difference
+// model on U (les.py:273-344).
+struct CollideParams {
+  int nx, ny, nz, dim;
+  long long ncell;
+  int nblocks, nblocks_grid;
+  const void* F;
+  const void* U;
+  void* fout;
+  const uint8_t* flags;
+  const double* damp;
+  const double* minv_d;
+  const float* minv_f;
+  double mrt_s[19];
+  double omega, trt_sm, trt_lambda, base_nu, les_cs;
+  int les;
+};
+
+template <class C>
+__device__ __forceinline__ C les_omega_fd(const CollideParams& p, const C* U, int cell, int x, int y, int z,
+                                          bool solid) {
+  const size_t N = (size_t)p.ncell;
+  C nu_sgs = C(0);
+  bool interior = x >= 1 && x <= p.nx - 2 && y >= 1 && y <= p.ny - 2;
+  if (p.dim == 3) interior = interior && z >= 1 && z <= p.nz - 2;
+  if (interior && !solid) {
+    const C csl = p.damp ? (C)(p.les_cs * p.damp[cell]) : (C)p.les_cs;
+    C g[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const size_t sy = (size_t)p.nx, sz = (size_t)p.nx * p.ny;
+    for (int a = 0; a < p.dim; ++a) {
+      const C* u = U + (size_t)a * N;
+      g[a][0] = C(0.5) * (u[cell + 1] - u[cell - 1]);
+      g[a][1] = C(0.5) * (u[cell + sy] - u[cell - sy]);
+      if (p.dim == 3) g[a][2] = C(0.5) * (u[cell + sz] - u[cell - sz]);
+    }
+    if (p.les == LES_FD_SMAG) {
+      C smag;
+      if (p.dim == 2) {
+        const C s11 = g[0][0], s22 = g[1][1], s12 = C(0.5) * (g[0][1] + g[1][0]);
+        smag = sqrt(C(2) * (s11 * s11 + s22 * s22 + C(2) * s12 * s12));
+      } else {
+        const C s11 = g[0][0], s22 = g[1][1], s33 = g[2][2];
+        const C s12 = C(0.5) * (g[0][1] + g[1][0]);
+        const C s13 = C(0.5) * (g[0][2] + g[2][0]);
+        const C s23 = C(0.5) * (g[1][2] + g[2][1]);
+        smag = sqrt(C(2) * (s11 * s11 + s22 * s22 + s33 * s33) + C(4) * (s12 * s12 + s13 * s13 + s23 * s23));
+      }
+      nu_sgs = (csl * csl) * smag;
+    } else {
+      C s_sq, sd_sq;
+      if (p.dim == 2) {
+        const C g11 = g[0][0], g12 = g[0][1], g21 = g[1][0], g22 = g[1][1];
+        const C s12 = C(0.5) * (g12 + g21);
+        s_sq = g11 * g11 + g22 * g22 + C(2) * s12 * s12;
+        const C q11 = g11 * g11 + g12 * g21, q12 = g11 * g12 + g12 * g22;
+        const C q21 = g21 * g11 + g22 * g21, q22 = g21 * g12 + g22 * g22;
+        const C tr = C(0.5) * (q11 + q22);
+        const C d11 = q11 - tr, d22 = q22 - tr, d12 = C(0.5) * (q12 + q21);
+        sd_sq = d11 * d11 + d22 * d22 + C(2) * d12 * d12;
+      } else {
+        C sq = 0;
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) {
+            const C sab = C(0.5) * (g[a][b] + g[b][a]);
+            sq += sab * sab;
+          }
+        s_sq = sq;
+        C g2[3][3];
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) g2[a][b] = g[a][0] * g[0][b] + g[a][1] * g[1][b] + g[a][2] * g[2][b];
+        const C tr = (g2[0][0] + g2[1][1] + g2[2][2]) / C(3);
+        C sd = 0;
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) {
+            const C v = C(0.5) * (g2[a][b] + g2[b][a]) - (a == b ? tr : C(0));
+            sd += v * v;
+          }
+        sd_sq = sd;
+      }
+      const C num = pow(sd_sq, C(1.5));
+      const C den = pow(s_sq, C(2.5)) + pow(sd_sq, C(1.25)) + C(1e-30);
+      nu_sgs = (csl * csl) * num / den;
+    }
+  }
+  const C nu_eff = solid ? (C)p.base_nu : (C)p.base_nu + nu_sgs;
+  return C(1) / (C(3) * nu_eff + C(0.5));
+}
+
+template <class L, int COLL, int ST>
+__global__ __launch_bounds__(BLOCK) void collide_kernel(CollideParams p) {
+  using S = Storage<ST>;
+  using C = typename S::C;
+  using T = typename S::T;
+  constexpr int Q = L::Q;
+  const int b = blockIdx.x;
+  const int per = p.nblocks_grid >> 3;
+  const int lb = (b & 7) * per + (b >> 3);
+  if (lb >= p.nblocks) return;
+  const long long c64 = (long long)lb * BLOCK + threadIdx.x;
+  if (c64 >= p.ncell) return;
+  const int cell = (int)c64;
+  const size_t N = (size_t)p.ncell;
+  const C* F = (const C*)p.F;
+  C fn[Q], out[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) fn[i] = F[(size_t)i * N + cell];
+  const bool solid = p.flags[cell] & FLAG_SOLID;
+  C om = (C)p.omega, sm = (C)p.trt_sm;
+  if (p.les != LES_OFF && COLL != COLL_MRT) {
+    const int x = cell % p.nx, yz = cell / p.nx, y = yz % p.ny, z = yz / p.ny;
+    om = les_omega_fd<C>(p, (const C*)p.U, cell, x, y, z, solid);
+    if (COLL == COLL_TRT) {
+      const C tau = C(1) / om;
+      const C lam = (C)p.trt_lambda;
+      const C tp = lam * tau + (C(1) - lam);
+      const C tm = (tp * tau - C(0.5)) / (tp - C(0.5));
+      sm = C(1) / tm;
+    }
+  }
+  collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, om, sm, fn, out);
+  T* Bo = (T*)p.fout;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) S::store(Bo + (size_t)i * N, (size_t)cell, out[i], C(L::W[i]));
+}
+
+}  // namespace aero

@@ -1,0 +1,59 @@
+"""Surface-link lists for bounce-back (host, setup only).
+
+Same definition and order as the reference (aero/lbm/boundary.py:232-280,
+aero/lbm/boundary3d.py:279-330): a link (i, cell) exists when the cell is
+fluid and its neighbour along e_i -- with the index CLIPPED to the domain,
+not wrapped -- is solid; links are ordered by direction i = 1..Q-1 and then
+row-major by cell.  q = clip(float32(phi[cell]), 1e-4, 1) when an SDF is
+given, else 0.5.  Vectorised: no per-link Python loop.
+"""
+
+from __future__ import annotations
+
+from typing import Optional, Tuple
+
+import numpy as np
+
+from .lattice import E, E3
+
+
+def _links(solid3: np.ndarray, E_: np.ndarray) -> np.ndarray:
+    nz, ny, nx = solid3.shape
+    fluid = ~solid3
+    zz, yy, xx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    out = []
+    for i in range(1, len(E_)):
+        ex = int(E_[i, 0])
+        ey = int(E_[i, 1])
+        ez = int(E_[i, 2]) if E_.shape[1] > 2 else 0
+        zn = np.clip(zz + ez, 0, nz - 1)
+        yn = np.clip(yy + ey, 0, ny - 1)
+        xn = np.clip(xx + ex, 0, nx - 1)
+        mask = fluid & solid3[zn, yn, xn]
+        z, y, x = np.nonzero(mask)
+        if z.size:
+            out.append(np.stack([np.full(z.size, i), z, y, x], axis=1))
+    if not out:
+        return np.empty((0, 4), dtype=np.int32)
+    return np.concatenate(out).astype(np.int32)
+
+
+def _q(links_cells: Tuple[np.ndarray, ...], n: int, phi: Optional[np.ndarray]) -> np.ndarray:
+    if phi is None or n == 0:
+        return np.full(n, 0.5, dtype=np.float32)
+    return np.clip(phi[links_cells].astype(np.float32), 1e-4, 1.0)
+
+
+def build_surface_links(solid: np.ndarray, phi: Optional[np.ndarray] = None):
+    """2D links (L, 3) = (i, y, x) and q (L,) float32."""
+    solid = np.asarray(solid, dtype=bool)
+    l4 = _links(solid[None], E)
+    links = np.ascontiguousarray(l4[:, [0, 2, 3]])
+    return links, _q((links[:, 1], links[:, 2]), len(links), phi)
+
+
+def build_surface_links_3d(solid: np.ndarray, phi: Optional[np.ndarray] = None):
+    """3D links (L, 4) = (i, z, y, x) and q (L,) float32."""
+    solid = np.asarray(solid, dtype=bool)
+    links = _links(solid, E3)
+    return links, _q((links[:, 1], links[:, 2], links[:, 3]), len(links), phi)

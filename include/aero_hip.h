@@ -1,0 +1,153 @@
+/*
+ * aero_hip.h -- C ABI of the MI355X (gfx950) lattice-Boltzmann step library.
+ *
+ * The reference drives its step loop from Python (aero/lbm/solver.py,
+ * aero/lbm/solver3d.py); its "native" layer is a set of Numba kernels behind
+ * plain NumPy arrays.  This header is the boundary a Python host binds with
+ * ctypes in place of those kernels: plain C types, caller-owned host buffers
+ * in the reference's own float64 layouts, no torch or HIP types.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a nonzero code on failure; the
+ *    message is available from aero_last_error() on the same thread.
+ *  - Host buffers are borrowed for the duration of the call only.
+ *  - Device memory is owned by the context.  A context is not thread-safe;
+ *    calls on one context must be serialised by the caller.
+ *  - Field layouts are the reference's C-order arrays:
+ *      f, g           (Q, Nz, Ny, Nx) float64  (2D: Nz = 1)
+ *      f_outlet_prev  (Q, Nz, Ny)     float64
+ *      solid          (Nz, Ny, Nx)    uint8 (0/1)
+ *      links          (L, 4) int32 rows (i, z, y, x)   (2D: z = 0)
+ *      q_vals         (L,)   float32
+ */
+#ifndef AERO_HIP_H
+#define AERO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AERO_ABI_VERSION 1
+
+/* collision (aero/lbm/solver.py:186-187) */
+#define AERO_BGK 0
+#define AERO_TRT 1
+#define AERO_MRT 2
+/* population storage */
+#define AERO_STORE_F64 0
+#define AERO_STORE_F32 1
+#define AERO_STORE_F16S 2  /* float16 holding f - w_i, float32 arithmetic */
+#define AERO_STORE_BF16S 3 /* bfloat16 holding f - w_i, float32 arithmetic */
+/* inlet at x = 0 */
+#define AERO_IN_NONE 0
+#define AERO_IN_ZOUHE 1     /* velocity Zou-He (+ travelling sine)   boundary.py:15-45, boundary3d.py:24-68 */
+#define AERO_IN_PRESSURE 2  /* 2D Zou-He pressure                    boundary.py:116-136 */
+#define AERO_IN_FIELD 3     /* target velocity planes                boundary.py:48-65, boundary3d.py:71-93 */
+#define AERO_IN_SEM 4       /* SEM fluctuation planes (3D)           boundary3d.py:96-137 */
+#define AERO_IN_RECYCLING 5 /* recycling-rescaling (3D)              boundary3d.py:172-198 */
+/* outlet at x = Nx-1 */
+#define AERO_OUT_NONE 0
+#define AERO_OUT_CONVECTIVE 1 /* boundary.py:78-113 */
+#define AERO_OUT_ZEROGRAD 2   /* boundary.py:68-75 */
+#define AERO_OUT_PRESSURE 3   /* 2D, boundary.py:139-159 */
+/* walls at y = 0 and y = Ny-1 */
+#define AERO_WALL_NONE 0
+#define AERO_WALL_SLIP 1
+#define AERO_WALL_NOSLIP 2
+#define AERO_WALL_MOVING 3
+/* LES (aero/lbm/les.py:273-344), finite differences on the pre-collision velocity */
+#define AERO_LES_OFF 0
+#define AERO_LES_SMAGORINSKY 1
+#define AERO_LES_WALE 2
+
+/* Number of doubles per step in the coefficient history written by aero_step:
+ * Cd, Cl_y, Cl_z, Cm_x, Cm_y, Cm_z, Cd_p, Cd_v, then the raw FP64 sums
+ * Fx Fy Fz Mx My Mz Fpx Fpy Fpz Fvx Fvy Fvz.  2D uses Cl_y as Cl and Cm_z as Cm
+ * (aero/forces.py:78-139); 3D normalises by D^2 and D^3 (aero/forces3d.py:364-439). */
+#define AERO_NCOEF 20
+/* aero_check output: any NaN, any rho <= 0, max |(ux, uy)|, sum rho, cell count
+ * (aero/diagnostics.py:32-71; 3D checks the z-midplane, solver3d.py:608-615). */
+#define AERO_NCHECK 5
+
+typedef struct aero_ctx aero_ctx;
+
+/* Everything the constructors take (aero/lbm/solver.py:75-116,
+ * aero/lbm/solver3d.py:71-119), resolved to numbers by the host. */
+typedef struct aero_desc {
+  int32_t q;  /* 9 or 19 */
+  int32_t nx, ny, nz;
+  int32_t collision;
+  int32_t storage;
+  int32_t inlet;
+  int32_t outlet;
+  int32_t wall;
+  int32_t bouzidi;
+  int32_t sponge_thickness;
+  int32_t les;
+  int32_t thermal;
+  int32_t buoyancy;
+  int32_t body_force_on;
+  int32_t reserved;
+  double omega, u0, rho0, D, trt_lambda;
+  double rho_in, rho_out, inlet_perturbation;
+  double sponge_strength;
+  double les_cs;
+  double u_wall_top, u_wall_bottom;
+  double body_force[3];
+  double alpha_T, T_hot, T_cold, T_ref, g_gravity, beta;
+  double center[3];    /* moment reference point (x, y, z): mean of solid cells */
+  double mrt_s[19];    /* MRT relaxation vector (mrt2d.py:51-77, mrt3d.py:94-135) */
+  double mrt_minv[361];/* inv(M9) or pinv(M19), row-major Q x Q */
+} aero_desc;
+
+/* Library / device */
+const char* aero_last_error(void);
+int aero_version(void);
+int aero_device_count(int32_t* n);
+
+/* Context lifecycle.  Replaces Solver.__init__ / Solver3D.__init__ state
+ * allocation (solver.py:143-257, solver3d.py:136-295). */
+int aero_create(const aero_desc* desc, aero_ctx** out);
+int aero_destroy(aero_ctx* ctx);
+
+/* Geometry: solid mask and the surface links + q values built by
+ * build_surface_links(_3d) (boundary.py:232-280, boundary3d.py:279-330). */
+int aero_set_geometry(aero_ctx* ctx, const uint8_t* solid, int64_t nlinks, const int32_t* links,
+                      const float* q_vals);
+/* Van Driest factor (1 - exp(-|phi|/A))^2 per cell (les.py:512-514), or NULL. */
+int aero_set_damping(aero_ctx* ctx, const double* damp);
+
+/* State: the reference's self.f (post-boundary, pre-collision), self.g and
+ * self._f_outlet_prev, read and written in the reference layout. */
+int aero_set_f(aero_ctx* ctx, const double* f);
+int aero_get_f(aero_ctx* ctx, double* f);
+int aero_set_g(aero_ctx* ctx, const double* g);
+int aero_get_g(aero_ctx* ctx, double* g);
+int aero_set_outlet_prev(aero_ctx* ctx, const double* prev);
+int aero_get_outlet_prev(aero_ctx* ctx, double* prev);
+
+/* Advance nsteps reference steps (Solver._step, solver.py:276-464;
+ * Solver3D._step, solver3d.py:325-541).  step0 is the step count before the
+ * call (the travelling inlet sine uses step0 + k + 1).  planes: for
+ * AERO_IN_FIELD / AERO_IN_SEM, nsteps x 3 x (Nz*Ny) doubles (2D: ux, uy, 0 per
+ * row; 3D: the target or fluctuation planes (u, v, w)), else NULL.
+ * hist: nsteps x AERO_NCOEF doubles, or NULL. */
+int aero_step(aero_ctx* ctx, int32_t nsteps, int64_t step0, const double* planes, double* hist);
+
+/* Observables of the current state (compute_macroscopic, d2q9.py:44-63). */
+int aero_macroscopic(aero_ctx* ctx, double* rho, double* ux, double* uy, double* uz);
+int aero_check(aero_ctx* ctx, double* stats);
+/* Per-link-cell FP64 momentum-exchange sums of the last step (nrec x 12) and
+ * the cells they belong to; *nrec returns the count (pass NULL arrays to query). */
+int aero_link_forces(aero_ctx* ctx, int32_t* cells, double* forces, int32_t* nrec);
+int aero_synchronize(aero_ctx* ctx);
+/* Bytes of device memory held by the context. */
+int aero_device_bytes(aero_ctx* ctx, int64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AERO_HIP_H */

@@ -1,0 +1,130 @@
+"""GPU parity: the HIP step path against the reference's own outputs.
+
+Every golden case (tests/golden/, written from the reference's NumPy path)
+is replayed through the product API -- Solver / Solver3D backed by
+libaero_hip.so -- with float64 storage and compared at 1e-10 relative on the
+full fields f, g, f_outlet_prev and on every per-step coefficient.  The same
+cases in float32 storage are compared with the float64 oracle at a stated
+float32 tolerance.  These run on an MI355X only.
+"""
+
+import numpy as np
+import pytest
+
+from tests import _golden as G
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-10      # relative, fields and coefficients (float64 storage)
+F32_FIELD_TOL = 2e-6  # max |f - f_ref| / max |f_ref| after the last snapshot (float32 storage)
+F32_COEF_TOL = 2e-3   # max |C - C_ref| / max |C_ref| over the history (float32 storage)
+
+
+def _aero():
+    import aero_amd
+    return aero_amd
+
+
+def make_solver(dim, case, storage="f64"):
+    A = _aero()
+    kw = dict(case["kw"])
+    use_phi = kw.get("bouzidi", False) or kw.get("van_driest", False)
+    phi = case.get("phi") if use_phi else None
+    solid = case["solid"]
+    args = dict(omega=float(case["omega"]), u0=float(case["u0"]), D=float(case["D"]), phi=phi,
+                storage=storage)
+    if dim == 2:
+        s = A.Solver(Ny=solid.shape[0], Nx=solid.shape[1], solid=solid, **args, **kw)
+    else:
+        s = A.Solver3D(Nz=solid.shape[0], Ny=solid.shape[1], Nx=solid.shape[2], solid=solid, **args, **kw)
+    f0 = G.initial_f(case)
+    s.f = f0
+    s._f_outlet_prev = f0[..., -1].copy()
+    return s
+
+
+def replay(dim, name, storage="f64"):
+    case = G.load(dim, name)
+    s = make_solver(dim, case, storage)
+    np.testing.assert_array_equal(s.surface_links, case["links"])
+    np.testing.assert_array_equal(s.q_vals, case["q_vals"])
+    snaps = G.SNAP2 if dim == 2 else G.SNAP3
+    rows = []
+    fields = {}
+    done = 0
+    for snap in snaps:
+        rows.append(s._advance(snap - done))
+        done = snap
+        fields[snap] = (s.f, s._f_outlet_prev, s.g)
+    return case, s, np.concatenate(rows), fields
+
+
+def _cols(dim):
+    if dim == 2:
+        return dict(Cd=0, Cl=1, Cm=5, Cd_p=6, Cd_v=7)
+    return dict(Cd=0, Cly=1, Clz=2, Cmx=3, Cmy=4, Cmz=5, Cd_p=6, Cd_v=7)
+
+
+@pytest.mark.parametrize("dim,name", [(2, n) for n in G.CASES_2D] + [(3, n) for n in G.CASES_3D])
+def test_f64_matches_reference(dim, name):
+    print(f"[gpu] f64 parity {dim}d {name}", flush=True)
+    case, s, rows, fields = replay(dim, name)
+    for snap, (f, prev, g) in fields.items():
+        assert G.rel_err(f, case[f"f_{snap}"]) < F64_TOL, (name, snap)
+        assert G.rel_err(prev, case[f"fprev_{snap}"]) < F64_TOL, (name, snap)
+        if g is not None:
+            assert G.rel_err(g, case[f"g_{snap}"]) < F64_TOL, (name, snap)
+    for key, col in _cols(dim).items():
+        ref = case[f"{key}_hist"]
+        scale = max(np.max(np.abs(ref)), 1e-3)
+        assert np.max(np.abs(rows[:, col] - ref)) / scale < F64_TOL * 10, (name, key)
+    prof = s._last_force_profile if dim == 2 else s._last_spanwise_profile
+    for k in ("fx", "fy") + (("fz",) if dim == 3 else ()):
+        np.testing.assert_allclose(prof[k], case[f"profile_{k}"], rtol=1e-9, atol=1e-12)
+
+
+F32_CASES = [(2, "bgk_bb_conv_slip"), (2, "trt_bouzidi_conv_slip"), (2, "mrt_bouzidi_zg_sponge"),
+             (2, "bgk_thermal_buoy"), (2, "bgk_les_smag"), (3, "trt_bouzidi_conv_slip"),
+             (3, "mrt_bouzidi_zg_noslip"), (3, "bgk_thermal_buoy"), (3, "trt_recycling")]
+
+
+@pytest.mark.parametrize("dim,name", F32_CASES)
+def test_f32_matches_reference_within_tolerance(dim, name):
+    print(f"[gpu] f32 parity {dim}d {name}", flush=True)
+    case, s, rows, fields = replay(dim, name, storage="f32")
+    last = max(fields)
+    f, _, g = fields[last]
+    assert G.rel_err(f, case[f"f_{last}"]) < F32_FIELD_TOL, name
+    if g is not None:
+        assert G.rel_err(g, case[f"g_{last}"]) < F32_FIELD_TOL, name
+    ref = case["Cd_hist"]
+    assert np.max(np.abs(rows[:, 0] - ref)) / np.max(np.abs(ref)) < F32_COEF_TOL, name
+
+
+def test_set_get_roundtrip_and_checkpoint(tmp_path):
+    """f written through the ABI reads back bit-exact; a checkpoint restart
+    continues the run (tests/test_tier3.py:148-163 of the reference)."""
+    A = _aero()
+    case = G.load(2, "bgk_bouzidi_zg_sponge")
+    s = make_solver(2, case)
+    np.testing.assert_array_equal(s.f, G.initial_f(case))
+    s.run(steps=20, check_every=10, verbose=False)
+    path = str(tmp_path / "c.npz")
+    s.save_checkpoint(path)
+    s.run(steps=20, check_every=10, verbose=False)
+    r = A.Solver.from_checkpoint(path)
+    assert r.step_count == 20
+    r.run(steps=20, check_every=10, verbose=False)
+    np.testing.assert_allclose(r.f, s.f, rtol=0, atol=1e-13)
+    assert r.Cd_history[-1] == pytest.approx(s.Cd_history[-1], rel=1e-10)
+
+
+def test_divergence_raises_runtime_error():
+    """check_stability semantics (aero/diagnostics.py:43-52) on device statistics."""
+    case = G.load(2, "bgk_bb_conv_slip")
+    s = make_solver(2, case)
+    f = s.f
+    f[3, 5, 7] = np.nan
+    s.f = f
+    with pytest.raises(RuntimeError, match="NaN"):
+        s.run(steps=2, check_every=1, verbose=False)
