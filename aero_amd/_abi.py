@@ -31,7 +31,7 @@ class AeroDesc(ctypes.Structure):
         ("collision", ctypes.c_int32), ("storage", ctypes.c_int32), ("inlet", ctypes.c_int32),
         ("outlet", ctypes.c_int32), ("wall", ctypes.c_int32), ("bouzidi", ctypes.c_int32),
         ("sponge_thickness", ctypes.c_int32), ("les", ctypes.c_int32), ("thermal", ctypes.c_int32),
-        ("buoyancy", ctypes.c_int32), ("body_force_on", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("buoyancy", ctypes.c_int32), ("body_force_on", ctypes.c_int32), ("trt_variant", ctypes.c_int32),
         ("omega", ctypes.c_double), ("u0", ctypes.c_double), ("rho0", ctypes.c_double),
         ("D", ctypes.c_double), ("trt_lambda", ctypes.c_double),
         ("rho_in", ctypes.c_double), ("rho_out", ctypes.c_double),
@@ -67,6 +67,8 @@ _SIGS = {
     "aero_link_forces": ([_P, _P, _P, _P], ctypes.c_int),
     "aero_synchronize": ([_P], ctypes.c_int),
     "aero_device_bytes": ([_P, _P], ctypes.c_int),
+    "aero_set_timing": ([_P, ctypes.c_int32], ctypes.c_int),
+    "aero_get_timing": ([_P, _P, _P], ctypes.c_int),
 }
 EXPORTED = tuple(_SIGS)
 

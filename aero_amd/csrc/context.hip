@@ -59,6 +59,17 @@ struct DevBuf {
   T* as() const { return (T*)p; }
 };
 
+// Antisymmetric TRT rate.  Variant 0 is the reference's formula
+// (trt2d.py:8-18: tau+ = lam tau + 1 - lam, tau- = (tau+ tau - 1/2)/(tau+ - 1/2));
+// variant 1 is the standard magic-parameter TRT, tau- = lam/(tau - 1/2) + 1/2.
+double trt_s_minus(double omega, double lam, int variant) {
+  const double tau = 1.0 / omega;
+  if (variant == 1) return 1.0 / (lam / (tau - 0.5) + 0.5);
+  const double tp = lam * tau + (1.0 - lam);
+  const double tm = (tp * tau - 0.5) / (tp - 0.5);
+  return 1.0 / tm;
+}
+
 size_t storage_bytes(int st) {
   switch (st) {
     case ST_F64: return 8;
@@ -100,6 +111,11 @@ struct aero_ctx {
   DevBuf fcache, gcache, tmp;
   bool fcache_valid = false, gcache_valid = false;
   DevBuf checkpart, checkout;
+  // optional event timing of the step kernel
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  double timed_ms = 0.0;
+  long long timed_launches = 0;
 
   bool les_split() const { return d.les != AERO_LES_OFF && d.collision != AERO_MRT; }
   bool convective() const { return d.outlet == AERO_OUT_CONVECTIVE; }
@@ -127,13 +143,7 @@ struct aero_ctx {
     for (int a = 0; a < 19; ++a) p.mrt_s[a] = d.mrt_s[a];
     p.omega = d.omega;
     p.trt_lambda = d.trt_lambda;
-    {
-      // trt2d.py:8-18: s- = 1/tau-, tau+ = lam tau + 1 - lam, tau- = (tau+ tau - 1/2)/(tau+ - 1/2)
-      const double tau = 1.0 / d.omega;
-      const double tp = d.trt_lambda * tau + (1.0 - d.trt_lambda);
-      const double tm = (tp * tau - 0.5) / (tp - 0.5);
-      p.trt_sm = 1.0 / tm;
-    }
+    p.trt_sm = trt_s_minus(d.omega, d.trt_lambda, d.trt_variant);
     p.u0 = d.u0;
     p.u_conv = d.u0;
     p.inlet_amp = d.inlet_perturbation > 0.0 ? d.inlet_perturbation : 0.0;
@@ -201,6 +211,7 @@ struct aero_ctx {
     cp.omega = d.omega;
     cp.trt_sm = bp.trt_sm;
     cp.trt_lambda = d.trt_lambda;
+    cp.trt_variant = d.trt_variant;
     cp.base_nu = (1.0 / d.omega - 0.5) / 3.0;  // physics.py:27-29
     cp.les_cs = d.les_cs;
     cp.les = U ? d.les : AERO_LES_OFF;
@@ -381,6 +392,7 @@ int aero_destroy(aero_ctx* c) {
                     &c->damp, &c->planes, &c->recyc, &c->hist, &c->fsplit, &c->usplit, &c->fcache,
                     &c->gcache, &c->tmp, &c->checkpart, &c->checkout};
   for (DevBuf* b : bufs) b->release();
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -547,6 +559,13 @@ int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, 
          "upload planes");
     }
     c->hist.alloc(sizeof(double) * NCOEF * (size_t)nsteps);
+    if (c->timing) {
+      while (c->ev_pool.size() < 2 * (size_t)nsteps) {
+        hipEvent_t e;
+        ck(hipEventCreate(&e), "hipEventCreate");
+        c->ev_pool.push_back(e);
+      }
+    }
     const double fd = c->force_denom(), md = c->moment_denom();
     for (int k = 0; k < nsteps; ++k) {
       const long long step_no = step0 + k + 1;
@@ -563,6 +582,7 @@ int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, 
       p.phase = 0.17 * (double)step_no;
       p.g_bc_skip = c->g_skip_pending;
       p.commit = 1;
+      if (c->timing) ck(hipEventRecord(c->ev_pool[2 * k], c->stream), "event");
       if (c->les_split()) {
         p.fsplit = c->fsplit.p;
         p.usplit = c->usplit.p;
@@ -571,6 +591,7 @@ int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, 
       } else {
         c->launch_step_kernel(0, p);
       }
+      if (c->timing) ck(hipEventRecord(c->ev_pool[2 * k + 1], c->stream), "event");
       ck(launch_reduce_forces(c->fpart.as<double>(), c->nrec, fd, md, c->hist.as<double>() + (size_t)NCOEF * k,
                               c->stream),
          "force reduction");
@@ -588,6 +609,30 @@ int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, 
          "download history");
     }
     ck(hipStreamSynchronize(c->stream), "step sync");
+    if (c->timing) {
+      for (int k = 0; k < nsteps; ++k) {
+        float ms = 0.f;
+        ck(hipEventElapsedTime(&ms, c->ev_pool[2 * k], c->ev_pool[2 * k + 1]), "event time");
+        c->timed_ms += ms;
+      }
+      c->timed_launches += nsteps;
+    }
+  });
+}
+
+int aero_set_timing(aero_ctx* c, int32_t enabled) {
+  return guarded([&] {
+    c->set_device();
+    c->timing = enabled != 0;
+    c->timed_ms = 0.0;
+    c->timed_launches = 0;
+  });
+}
+
+int aero_get_timing(aero_ctx* c, double* kernel_ms, int64_t* launches) {
+  return guarded([&] {
+    *kernel_ms = c->timed_ms;
+    *launches = c->timed_launches;
   });
 }
 

@@ -672,6 +672,7 @@ struct CollideParams {
   double mrt_s[19];
   double omega, trt_sm, trt_lambda, base_nu, les_cs;
   int les;
+  int trt_variant;
 };
 
 template <class C>
@@ -767,12 +768,16 @@ __global__ __launch_bounds__(BLOCK) void collide_kernel(CollideParams p) {
   if (p.les != LES_OFF && COLL != COLL_MRT) {
     const int x = cell % p.nx, yz = cell / p.nx, y = yz % p.ny, z = yz / p.ny;
     om = les_omega_fd<C>(p, (const C*)p.U, cell, x, y, z, solid);
-    if (COLL == COLL_TRT) {
+    if (COLL == COLL_TRT) {  // per-cell s- from the local omega (kernels_trt.py:14-18)
       const C tau = C(1) / om;
       const C lam = (C)p.trt_lambda;
-      const C tp = lam * tau + (C(1) - lam);
-      const C tm = (tp * tau - C(0.5)) / (tp - C(0.5));
-      sm = C(1) / tm;
+      if (p.trt_variant == 1) {
+        sm = C(1) / (lam / (tau - C(0.5)) + C(0.5));
+      } else {
+        const C tp = lam * tau + (C(1) - lam);
+        const C tm = (tp * tau - C(0.5)) / (tp - C(0.5));
+        sm = C(1) / tm;
+      }
     }
   }
   collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, om, sm, fn, out);

@@ -18,8 +18,20 @@ from .lattice import E, E3
 
 
 def _links(solid3: np.ndarray, E_: np.ndarray) -> np.ndarray:
-    nz, ny, nx = solid3.shape
-    fluid = ~solid3
+    if not solid3.any():
+        return np.empty((0, 4), dtype=np.int32)
+    # A link cell lies within one cell of a solid cell on every axis, so only
+    # the solid's bounding box dilated by one needs scanning.  Cells on the
+    # box faces are fluid, so clipping inside the box equals clipping inside
+    # the domain wherever the box stops short of the domain edge.
+    lo, hi = [], []
+    for ax, n in enumerate(solid3.shape):
+        idx = np.nonzero(solid3.any(axis=tuple(a for a in range(3) if a != ax)))[0]
+        lo.append(max(int(idx[0]) - 1, 0))
+        hi.append(min(int(idx[-1]) + 1, n - 1))
+    sub = solid3[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1]
+    nz, ny, nx = sub.shape
+    fluid = ~sub
     zz, yy, xx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     out = []
     for i in range(1, len(E_)):
@@ -29,10 +41,10 @@ def _links(solid3: np.ndarray, E_: np.ndarray) -> np.ndarray:
         zn = np.clip(zz + ez, 0, nz - 1)
         yn = np.clip(yy + ey, 0, ny - 1)
         xn = np.clip(xx + ex, 0, nx - 1)
-        mask = fluid & solid3[zn, yn, xn]
+        mask = fluid & sub[zn, yn, xn]
         z, y, x = np.nonzero(mask)
         if z.size:
-            out.append(np.stack([np.full(z.size, i), z, y, x], axis=1))
+            out.append(np.stack([np.full(z.size, i), z + lo[0], y + lo[1], x + lo[2]], axis=1))
     if not out:
         return np.empty((0, 4), dtype=np.int32)
     return np.concatenate(out).astype(np.int32)

@@ -39,10 +39,16 @@ class Solver(DeviceSolver):
                  synthetic_inflow_intensity: float = 0.03, synthetic_inflow_seed: int = 12345,
                  thermal: bool = False, T_hot: float = 1.0, T_cold: float = 0.0, alpha_T: float = 1e-3,
                  buoyancy: bool = False, g_gravity: float = 0.0, beta: float = 1e-3, T_ref: float = 0.5,
-                 storage: str = "f64"):
-        """Arguments as the reference (solver.py:75-142); ``storage`` selects the
-        population format on the device: "f64" (reference precision), "f32",
-        "f16" or "bf16" (shifted half storage, FP32 arithmetic)."""
+                 storage: str = "f64", trt_variant: str = "reference"):
+        """Arguments as the reference (solver.py:75-142), plus:
+
+        storage     -- population format on the device: "f64" (reference
+                       precision), "f32", "f16" or "bf16" (shifted half storage,
+                       FP32 arithmetic).
+        trt_variant -- "reference" uses the reference's TRT rates
+                       (trt2d.py:8-18), which are unstable for tau below ~0.76;
+                       "lambda" uses the standard magic-parameter TRT
+                       (tau+ = tau, tau- = trt_lambda/(tau - 1/2) + 1/2)."""
         self.Ny, self.Nx = int(Ny), int(Nx)
         self.solid = np.ascontiguousarray(solid, dtype=np.bool_)
         if self.solid.shape != (self.Ny, self.Nx):
@@ -81,6 +87,9 @@ class Solver(DeviceSolver):
         if storage not in _abi.STORE:
             raise ValueError(f"Unknown storage '{storage}'. Choose one of {sorted(_abi.STORE)}.")
         self.storage = storage
+        if trt_variant not in ("reference", "lambda"):
+            raise ValueError("trt_variant must be 'reference' or 'lambda'.")
+        self.trt_variant = trt_variant
         self._raise_on_step = None
         if self.les and self.les_model == "wale" and self.van_driest and phi is not None:
             # the reference's WALE loop cannot take a damped (per-cell) constant
@@ -109,6 +118,7 @@ class Solver(DeviceSolver):
         d.wall = {"slip": _abi.WALL_SLIP, "noslip": _abi.WALL_NOSLIP,
                   "moving": _abi.WALL_MOVING}.get(wall_bc, _abi.WALL_NONE)
         d.bouzidi = int(self.bouzidi)
+        d.trt_variant = 1 if self.trt_variant == "lambda" else 0
         d.sponge_thickness = self.sponge_thickness
         d.les = (_abi.LES_OFF if not self.les else
                  (_abi.LES_WALE if self.les_model == "wale" else _abi.LES_SMAGORINSKY))
@@ -264,7 +274,7 @@ class Solver(DeviceSolver):
             synthetic_inflow=self.synthetic_inflow, synthetic_inflow_intensity=self.synthetic_inflow_intensity,
             thermal=self.thermal, T_hot=self.T_hot, T_cold=self.T_cold, alpha_T=self.alpha_T,
             buoyancy=self.buoyancy, g_gravity=self.g_gravity, beta=self.beta, T_ref=self.T_ref,
-            collision=self.collision, backend=self.backend, storage=self.storage)
+            collision=self.collision, backend=self.backend, storage=self.storage, trt_variant=self.trt_variant)
 
     def save_checkpoint(self, path: str) -> None:
         """Same keys and layout as the reference (solver.py:638-686)."""

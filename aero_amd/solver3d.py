@@ -39,8 +39,9 @@ class Solver3D(DeviceSolver):
                  synthetic_inflow_seed: int = 12345, sem_inlet: bool = False, sem_Tu: float = 0.05,
                  sem_L_int: float = 10.0, sem_N: int = 200, thermal: bool = False, T_hot: float = 1.0,
                  T_cold: float = 0.0, alpha_T: float = 1e-3, buoyancy: bool = False, g_gravity: float = 0.0,
-                 beta: float = 1e-3, T_ref: float = 0.5, storage: str = "f64"):
-        """Arguments as the reference (solver3d.py:71-135) plus ``storage``."""
+                 beta: float = 1e-3, T_ref: float = 0.5, storage: str = "f64", trt_variant: str = "reference"):
+        """Arguments as the reference (solver3d.py:71-135) plus ``storage`` and
+        ``trt_variant`` (see Solver)."""
         self.Nz, self.Ny, self.Nx = int(Nz), int(Ny), int(Nx)
         self.solid = np.ascontiguousarray(solid, dtype=np.bool_)
         if self.solid.shape != (self.Nz, self.Ny, self.Nx):
@@ -86,6 +87,9 @@ class Solver3D(DeviceSolver):
         if storage not in _abi.STORE:
             raise ValueError(f"Unknown storage '{storage}'. Choose one of {sorted(_abi.STORE)}.")
         self.storage = storage
+        if trt_variant not in ("reference", "lambda"):
+            raise ValueError("trt_variant must be 'reference' or 'lambda'.")
+        self.trt_variant = trt_variant
         self._raise_on_step = None
         if self.les and self.les_model == "wale" and self.van_driest and phi is not None:
             self._raise_on_step = ValueError("setting an array element with a sequence.")
@@ -112,6 +116,7 @@ class Solver3D(DeviceSolver):
         d.wall = {"slip": _abi.WALL_SLIP, "noslip": _abi.WALL_NOSLIP,
                   "moving": _abi.WALL_MOVING}.get(wall_bc, _abi.WALL_NONE)
         d.bouzidi = int(self.bouzidi)
+        d.trt_variant = 1 if self.trt_variant == "lambda" else 0
         d.sponge_thickness = self.sponge_thickness
         d.les = (_abi.LES_OFF if not self.les else
                  (_abi.LES_WALE if self.les_model == "wale" else _abi.LES_SMAGORINSKY))
@@ -289,7 +294,7 @@ class Solver3D(DeviceSolver):
             synthetic_inflow=self.synthetic_inflow, synthetic_inflow_intensity=self.synthetic_inflow_intensity,
             thermal=self.thermal, T_hot=self.T_hot, T_cold=self.T_cold, alpha_T=self.alpha_T,
             buoyancy=self.buoyancy, g_gravity=self.g_gravity, beta=self.beta, T_ref=self.T_ref,
-            storage=self.storage)
+            storage=self.storage, trt_variant=self.trt_variant)
 
     def save_checkpoint(self, path: str) -> None:
         """Same keys and layout as the reference (solver3d.py:727-783)."""

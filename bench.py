@@ -1,0 +1,187 @@
+"""Throughput benchmark of the MI355X lattice-Boltzmann step.
+
+Workload (one step = one lattice update of every cell, solid included):
+3D D3Q19 TRT sphere, 128 x 64 x 64 (Nx x Ny x Nz), radius 8 centred at
+(32, 32, 32), Bouzidi links, u0 = 0.05, Re = 100 (tau = 0.524), slip walls,
+convective outlet, equilibrium start plus seeded 1e-4 transverse velocity
+noise, float32 population storage.  At tau = 0.524 the reference's TRT rates
+are linearly unstable (s- = -18), so the timed run uses the standard
+magic-parameter TRT (Lambda = 3/16), labelled "trt_variant": "lambda".
+
+Prints ONE JSON line.  ``value`` is whole-job MLUPS over the timed steps
+(all launches included, populations resident in HBM).  ``roofline`` is the
+fused step kernel's algorithmic HBM rate (2 Q sizeof(float) bytes per cell)
+over its average HIP-event duration.  ``cpu_baseline`` times the float64 C
+oracle (OpenMP, all host cores) on a bounded number of full steps of the same
+workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def make_case(nx, ny, nz, radius, cx, storage, steps_hint=0, seed=0):
+    import aero_amd
+    from aero_amd.geometry import Sphere
+    from aero_amd.lattice import compute_feq_3d
+    g = Sphere(radius=radius, cx_frac=cx / nx, cy_frac=0.5, cz_frac=0.5)
+    solid, phi = g.mark_solid(nz, ny, nx), g.sdf_field(nz, ny, nx)
+    D = 2.0 * radius
+    u0, re = 0.05, 100.0
+    omega = 1.0 / (3.0 * u0 * D / re + 0.5)
+    s = aero_amd.Solver3D(Nz=nz, Ny=ny, Nx=nx, solid=solid, omega=omega, u0=u0, D=D, phi=phi, bouzidi=True,
+                          collision="trt", trt_lambda=0.1875, trt_variant="lambda", storage=storage)
+    rng = np.random.default_rng(seed)
+    sh = (nz, ny, nx)
+    f0 = compute_feq_3d(np.ones(sh), np.full(sh, u0), rng.uniform(-1e-4, 1e-4, sh), rng.uniform(-1e-4, 1e-4, sh))
+    s.f = f0
+    s._f_outlet_prev = f0[:, :, :, -1].copy()
+    return s, dict(solid=solid, phi=phi, omega=omega, u0=u0, D=D, f0=f0)
+
+
+def timed(solver, steps, warmup):
+    import ctypes
+    from aero_amd import _abi
+    solver._advance(warmup)
+    solver._ctx.call("aero_set_timing", ctypes.c_int32(1))
+    solver._ctx.call("aero_synchronize")
+    t0 = time.perf_counter()
+    rows = solver._advance(steps)
+    t1 = time.perf_counter()
+    ms = ctypes.c_double(0.0)
+    n = ctypes.c_int64(0)
+    solver._ctx.call("aero_get_timing", ctypes.byref(ms), ctypes.byref(n))
+    solver._ctx.call("aero_set_timing", ctypes.c_int32(0))
+    return t1 - t0, ms.value / max(n.value, 1), rows
+
+
+def cpu_baseline(case, nx, ny, nz, budget_s=15.0):
+    """Float64 C oracle (OpenMP) on the same workload, full reference steps."""
+    from oracle import oracle as orc
+    o = orc.OracleSolver(case["solid"], omega=case["omega"], u0=case["u0"], D=case["D"], phi=case["phi"],
+                         bouzidi=True, collision="trt", trt_lambda=0.1875, trt_variant="lambda")
+    o.set_state(case["f0"])
+    o.step()  # warm-up
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < budget_s:
+        o.step()
+        n += 1
+    dt = time.perf_counter() - t0
+    cells = nx * ny * nz
+    kt = orc.kernel_only_seconds(o, max(1, n // 2))
+    return {"value": round(cells * n / dt / 1e6, 3), "unit": "MLUPS", "cores": orc.lib().orc_num_threads(),
+            "kind": "port",
+            "sample": f"{n} full float64 steps (collide, copy, stream, Bouzidi, inlet, outlet, walls, forces) "
+                      f"of the {nx}x{ny}x{nz} D3Q19 TRT sphere; kernel-only (collide+copy+stream) "
+                      f"{cells * max(1, n // 2) / kt / 1e6:.3f} MLUPS",
+            "cpu_model": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def traffic_from_profiles(name):
+    p = ROOT / "profiles" / f"{name}.json"
+    if p.exists():
+        try:
+            return json.loads(p.read_text()).get("hbm_bytes_per_launch")
+        except (ValueError, OSError):
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the extra storage forms")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs one resident domain per device and "
+              "has no multi-GPU path (see DESIGN.md for how it would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    nx, ny, nz = 128, 64, 64
+    cells = nx * ny * nz
+    s, case = make_case(nx, ny, nz, 8.0, 32.0, "f32")
+    wall, kern_ms, rows = timed(s, a.steps, a.warmup)
+    if not np.all(np.isfinite(rows[:, 0])):
+        raise RuntimeError("non-finite drag in the timed run")
+    mlups = cells * a.steps / wall / 1e6
+    bytes_per_launch = 2 * 19 * 4 * cells
+    achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+    out = {
+        "metric": "MLUPS + %HBM roofline, D3Q19 FP32 vs mixed",
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: analytic sphere, equilibrium + seeded 1e-4 noise",
+        "config": {"workload": "D3Q19 TRT sphere 128x64x64 r=8 Bouzidi, u0=0.05 Re=100, fp32 storage",
+                   "nx": nx, "ny": ny, "nz": nz, "collision": "trt", "trt_variant": "lambda",
+                   "trt_lambda": 0.1875, "storage": "f32", "links": int(len(s.surface_links))},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles("step_d3q19_trt_f32_128x64x64"),
+                     "kernel": "aero::step_kernel<D3Q19, TRT, F32>", "kernel_ms": round(kern_ms, 5),
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "cd_last": float(rows[-1, 0]),
+    }
+    if not a.no_extra:
+        extra = {}
+        for st in ("f16", "bf16", "f64"):
+            s2, _ = make_case(nx, ny, nz, 8.0, 32.0, st)
+            w2, k2, r2 = timed(s2, max(200, a.steps // 2), a.warmup)
+            n2 = max(200, a.steps // 2)
+            bpl = 2 * 19 * {"f16": 2, "bf16": 2, "f64": 8}[st] * cells
+            extra[st] = {"MLUPS": round(cells * n2 / w2 / 1e6, 2), "kernel_ms": round(k2, 5),
+                         "roofline_frac": round(bpl / (k2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "cd_last": float(r2[-1, 0])}
+            s2.close()
+        # the 512 x 256 x 256 sphere (radius 32) the throughput target is quoted on
+        for st in ("f32", "f16"):
+            s3, _ = make_case(512, 256, 256, 32.0, 128.0, st)
+            n3 = 40
+            w3, k3, r3 = timed(s3, n3, 10)
+            c3 = 512 * 256 * 256
+            bpl = 2 * 19 * (4 if st == "f32" else 2) * c3
+            extra[f"{st}_512x256x256"] = {"MLUPS": round(c3 * n3 / w3 / 1e6, 2), "kernel_ms": round(k3, 4),
+                                          "roofline_frac": round(bpl / (k3 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            s3.close()
+        out["forms"] = extra
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(case, nx, ny, nz)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

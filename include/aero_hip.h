@@ -89,7 +89,8 @@ typedef struct aero_desc {
   int32_t thermal;
   int32_t buoyancy;
   int32_t body_force_on;
-  int32_t reserved;
+  int32_t trt_variant; /* 0: reference TRT rates (trt2d.py:8-18); 1: standard Lambda-TRT,
+                          tau+ = tau, tau- = Lambda/(tau+ - 1/2) + 1/2 with Lambda = trt_lambda */
   double omega, u0, rho0, D, trt_lambda;
   double rho_in, rho_out, inlet_perturbation;
   double sponge_strength;
@@ -145,6 +146,11 @@ int aero_link_forces(aero_ctx* ctx, int32_t* cells, double* forces, int32_t* nre
 int aero_synchronize(aero_ctx* ctx);
 /* Bytes of device memory held by the context. */
 int aero_device_bytes(aero_ctx* ctx, int64_t* bytes);
+/* Kernel timing with HIP events on the context's stream: when enabled,
+ * aero_step brackets every step-kernel launch with events; aero_get_timing
+ * returns the summed kernel milliseconds and the launch count since enabling. */
+int aero_set_timing(aero_ctx* ctx, int32_t enabled);
+int aero_get_timing(aero_ctx* ctx, double* kernel_ms, int64_t* launches);
 
 #ifdef __cplusplus
 }

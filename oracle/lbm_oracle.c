@@ -40,7 +40,7 @@ int orc_collide(int Q, int nz, int ny, int nx,
                 const int *ex, const int *ey, const int *ez, const double *w,
                 const int *opp, int coll, double omega, const double *omega_field,
                 double trt_lambda, const double *M, const double *Minv,
-                const double *s) {
+                const double *s, int trt_variant) {
     const size_t N = (size_t)nz * ny * nx;
 #pragma omp parallel for schedule(static)
     for (long long c = 0; c < (long long)N; ++c) {
@@ -70,9 +70,16 @@ int orc_collide(int Q, int nz, int ny, int nx,
             /* trt2d.py:8-18 and kernels_trt.py:14-18: s- = 1/tau-,
                tau+ = lam*tau + (1-lam), tau- = (tau+ tau - 1/2)/(tau+ - 1/2) */
             double tau = 1.0 / om;
-            double tp = trt_lambda * tau + (1.0 - trt_lambda);
-            double tm = (tp * tau - 0.5) / (tp - 0.5);
-            double sm = 1.0 / tm;
+            double sm;
+            if (trt_variant == 1) {
+                /* standard magic-parameter TRT (not the reference operator):
+                   tau+ = tau, tau- = Lambda/(tau - 1/2) + 1/2 */
+                sm = 1.0 / (trt_lambda / (tau - 0.5) + 0.5);
+            } else {
+                double tp = trt_lambda * tau + (1.0 - trt_lambda);
+                double tm = (tp * tau - 0.5) / (tp - 0.5);
+                sm = 1.0 / tm;
+            }
             for (int i = 0; i < Q; ++i) {
                 int j = opp[i];
                 double fp = 0.5 * (fl[i] + fl[j]);

@@ -203,7 +203,7 @@ class OracleSolver:
                  bouzidi=False, van_driest=False, van_driest_A=25.0, body_force=(0.0, 0.0, 0.0),
                  wall_velocity_top=0.0, wall_velocity_bottom=0.0, inlet_mode="zouhe",
                  thermal=False, T_hot=1.0, T_cold=0.0, alpha_T=1e-3, buoyancy=False,
-                 g_gravity=0.0, beta=1e-3, T_ref=0.5, links=None, q_vals=None):
+                 g_gravity=0.0, beta=1e-3, T_ref=0.5, links=None, q_vals=None, trt_variant="reference"):
         self.is3d = solid.ndim == 3
         self.solid = np.ascontiguousarray(solid, dtype=bool)
         s3 = self.solid if self.is3d else self.solid[None]
@@ -222,6 +222,7 @@ class OracleSolver:
         self.collision = {"bgk": 0, "trt": 1, "mrt": 2}[collision]
         self.inlet_perturbation = max(float(inlet_perturbation), 0.0)
         self.trt_lambda = float(trt_lambda)
+        self.trt_variant = {"reference": 0, "lambda": 1}[trt_variant]
         self.sigma = sponge_sigma(self.nx, int(sponge_thickness), float(sponge_strength))
         self.sponge_on = int(sponge_thickness) > 0
         self.les, self.les_cs = bool(les), float(les_cs)
@@ -338,7 +339,8 @@ class OracleSolver:
         lib().orc_collide(Q, nz, ny, nx, _p(f), _p(out), _p(self.solid_u8), _p(self.ex),
                           _p(self.ey), _p(self.ez), _p(self.w), _p(self.opp), self.collision,
                           ctypes.c_double(self.omega), _p(omega_field),
-                          ctypes.c_double(self.trt_lambda), _p(self.M), _p(self.Minv), _p(self.s))
+                          ctypes.c_double(self.trt_lambda), _p(self.M), _p(self.Minv), _p(self.s),
+                          self.trt_variant)
         return out
 
     def stream(self, src):

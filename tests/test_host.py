@@ -1,0 +1,187 @@
+"""Host-side pieces of the product (CPU only): geometry, links, inflow
+generators, diagnostics and the solver's argument checking.
+
+Each is compared with the reference's recorded outputs (golden fixtures) or,
+where the reference's own tests fix a behaviour, with the same known-answer
+inputs those tests use (tests/test_diagnostics_stationarity.py,
+tests/test_phase2.py:424-453, tests/test_observables.py of the reference).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from aero_amd import diagnostics as dg
+from aero_amd.geometry import Box, Cylinder, Cylinder3D, Polygon, Rectangle, Sphere
+from aero_amd.inflow import SEMInlet, SyntheticInflow2D, SyntheticInflow3D
+from aero_amd.links import build_surface_links, build_surface_links_3d
+from oracle import oracle as orc
+from tests import _golden as G
+
+
+# ---------------------------------------------------------------- geometry
+def test_geometry_masks_and_sdf_match_fixtures():
+    shapes2 = {"cyl": Cylinder(radius=5.0, cx_frac=0.3, cy_frac=0.5),
+               "rect": Rectangle(width=5.0, height=7.0, cx_frac=0.3, cy_frac=0.45),
+               "poly": Polygon([(0.25, 0.35), (0.33, 0.42), (0.30, 0.62), (0.22, 0.60), (0.19, 0.45)])}
+    by_case = {"bgk_bb_conv_slip": "cyl", "trt316_bb_zg_noslip": "rect", "bgk_polygon": "poly"}
+    for name, shape in by_case.items():
+        case = G.load(2, name)
+        g = shapes2[shape]
+        np.testing.assert_array_equal(g.mark_solid(20, 40), case["solid"])
+        if "phi" in case:
+            np.testing.assert_array_equal(g.sdf_field(20, 40), case["phi"])
+    shapes3 = {"bgk_bouzidi_conv_slip": Sphere(radius=2.3, cx_frac=0.3, cy_frac=0.5, cz_frac=0.5),
+               "mrt_bb_conv_sponge": Box(width=3.0, height=4.0, depth=3.0, cx_frac=0.3, cy_frac=0.45),
+               "bgk_periodic_bodyforce": Cylinder3D(radius=2.0, length=4.0, cx_frac=0.4)}
+    for name, g in shapes3.items():
+        case = G.load(3, name)
+        np.testing.assert_array_equal(g.mark_solid(6, 8, 16), case["solid"])
+        if "phi" in case:
+            np.testing.assert_array_equal(g.sdf_field(6, 8, 16), case["phi"])
+
+
+# ------------------------------------------------------------------- links
+def test_links_match_reference_fixtures():
+    with np.load(G.GOLDEN / "links_c1_c2.npz") as d:
+        g = Cylinder(radius=20.0, cx_frac=0.25, cy_frac=0.5)
+        links, q = build_surface_links(g.mark_solid(200, 400), g.sdf_field(200, 400))
+        np.testing.assert_array_equal(links, d["c1_links"])
+        np.testing.assert_array_equal(q, d["c1_q"])
+        g3 = Sphere(radius=8.0, cx_frac=0.25)
+        links3, q3 = build_surface_links_3d(g3.mark_solid(64, 64, 128), g3.sdf_field(64, 64, 128))
+        np.testing.assert_array_equal(links3, d["c2_links"])
+        np.testing.assert_array_equal(q3, d["c2_q"])
+    for dim in (2, 3):
+        for name in (G.CASES_2D if dim == 2 else G.CASES_3D):
+            case = G.load(dim, name)
+            fn = build_surface_links if dim == 2 else build_surface_links_3d
+            links, _ = fn(case["solid"])
+            np.testing.assert_array_equal(links, case["links"])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_links_brute_force(seed):
+    """Against the oracle's scan (itself checked by brute force); solids touching
+    the domain edges exercise the clipped neighbour index."""
+    rng = np.random.default_rng(seed)
+    sol3 = rng.random((5, 6, 7)) < 0.25
+    np.testing.assert_array_equal(build_surface_links_3d(sol3)[0], orc.build_links(sol3)[0])
+    sol2 = rng.random((6, 9)) < 0.25
+    np.testing.assert_array_equal(build_surface_links(sol2)[0], orc.build_links(sol2)[0])
+
+
+# ----------------------------------------------------------------- inflow
+def test_sem_matches_reference_planes():
+    with np.load(G.GOLDEN / "sem_inlet.npz") as d:
+        sem = SEMInlet(Ny=20, Nz=16, u0=0.05, Tu=0.05, L_int=4.0, N_eddies=30, rng_seed=0)
+        planes = [np.stack(sem.fluctuation())]
+        for _ in range(5):
+            sem.step()
+            planes.append(np.stack(sem.fluctuation()))
+        np.testing.assert_allclose(np.array(planes), d["planes"], rtol=0, atol=1e-15)
+        np.testing.assert_array_equal(sem.xk, d["xk"])
+        np.testing.assert_array_equal(sem.eps, d["eps"])
+
+
+def test_sem_solver_planes_match_reference_step_planes():
+    """The per-step planes the 3D SEM case fed its steps (solver3d.py:434-437)."""
+    case = G.load(3, "bgk_sem")
+    kw = case["kw"]
+    sem = SEMInlet(8, 6, float(case["u0"]), kw["sem_Tu"], kw["sem_L_int"], kw["sem_N"])
+    for k in range(len(case["sem_planes"])):
+        sem.step()
+        np.testing.assert_allclose(np.stack(sem.fluctuation()), case["sem_planes"][k], rtol=0, atol=1e-15)
+
+
+def test_sem_divergence_free_and_energy():
+    """Known properties the reference tests (tests/test_sem_inlet.py:46-70)."""
+    sem = SEMInlet(Ny=32, Nz=32, u0=0.05, Tu=0.1, L_int=8.0, N_eddies=300, rng_seed=99)
+    _, v, w = sem.fluctuation()
+    vh, wh = np.fft.fft2(v), np.fft.fft2(w)
+    kz = (np.fft.fftfreq(32) * 2 * np.pi)[:, None]
+    ky = (np.fft.fftfreq(32) * 2 * np.pi)[None, :]
+    div = ky * vh + kz * wh
+    assert np.linalg.norm(div) / (np.linalg.norm(vh) + np.linalg.norm(wh)) < 1e-10
+    assert SEMInlet(16, 16, 0.05, 0.0, 5.0, 50).fluctuation()[0].max() == 0.0
+
+
+def test_synthetic_inflow_statistics():
+    g2 = SyntheticInflow2D(64, 0.05, 0.03)
+    for _ in range(5):
+        uy = g2.next()
+    assert abs(float(np.mean(uy))) < 1e-15
+    assert abs(float(np.sqrt(np.mean(uy ** 2))) - 0.03 * 0.05) < 1e-12
+    g3 = SyntheticInflow3D(8, 12, 0.05, 0.04)
+    uy, uz = g3.next()
+    assert abs(float(np.sqrt(np.mean(uy ** 2 + uz ** 2))) - 0.04 * 0.05) < 1e-12
+
+
+# ------------------------------------------------------------ diagnostics
+def test_convergence_and_strouhal_known_answers():
+    hist = [1.0 + 1e-4 * math.sin(i) for i in range(600)]
+    r = dg.analyze_convergence(hist, window=500, tol=0.01)
+    assert r.converged and r.ratio < r.threshold
+    D, u0 = 20.0, 0.05
+    t = np.arange(8192.0)
+    freq = 4 / t.size
+    rep = dg.analyze_strouhal(list(0.2 * np.sin(2 * math.pi * freq * t)), D=D, u0=u0, window=8192)
+    assert abs(rep.strouhal - freq * D / u0) / (freq * D / u0) < 0.02
+    st = 0.164
+    cl = 0.3 * np.sin(2 * math.pi * (st * u0 / 40.0) * np.arange(5000.0))
+    assert abs(dg.compute_strouhal(list(cl), D=40.0, u0=u0) - st) / st < 0.05
+    assert dg.compute_strouhal([0.1] * 100, D=40.0, u0=0.05) is None
+    assert dg.compute_strouhal([0.0] * 2000, D=40.0, u0=0.05) is None
+
+
+def test_stationarity_requires_both_metrics():
+    D, u0 = 20.0, 0.05
+    freq = 0.17 * u0 / D
+    t = np.arange(5000.0)
+    cd = list(1.5 + 1e-4 * np.sin(0.01 * t))
+    cl = list(0.15 * np.sin(2 * math.pi * freq * t))
+    cdr, str_, stop = dg.detect_statistical_stationarity(cd, cl, D=D, u0=u0, convergence_window=2000,
+                                                         strouhal_window=2048)
+    assert cdr.converged and str_.strouhal is not None and stop
+    # lift whose frequency jumps from bin 10 to bin 14 (of 1024) between the two half windows
+    f1, f2 = 10 / 1024, 14 / 1024
+    drifting = list(np.where(t < 5000 - 1024, 0.15 * np.sin(2 * math.pi * f1 * t),
+                             0.15 * np.sin(2 * math.pi * f2 * t)))
+    _, st2, stop2 = dg.detect_statistical_stationarity(cd, drifting, D=D, u0=u0, convergence_window=2000,
+                                                       strouhal_window=2048)
+    assert not stop2 and st2.relative_drift is not None and st2.relative_drift > 0.05
+
+
+def test_coefficient_spectrum_and_validation():
+    vals = [0.2 * math.sin(2 * math.pi * 0.025 * i) for i in range(512)]
+    rep = dg.coefficient_spectrum(vals, discard_fraction=0.0, min_samples=128)
+    assert abs(rep["dominant_frequency"] - 0.025) < 0.005
+    assert dg.coefficient_spectrum([1.0] * 50) is None
+    msgs = dg.validate_parameters(tau=0.5, u0=0.1, D=90.0, Ny=200)
+    assert msgs[0].startswith("ERROR") and any("Ma=" in m for m in msgs) and any("blockage" in m for m in msgs)
+
+
+def test_check_stability_semantics():
+    with pytest.raises(RuntimeError, match="NaN"):
+        dg.check_stability_stats(np.array([1.0, 0.0, 0.0, 1.0, 1.0]), 3)
+    with pytest.raises(RuntimeError, match="Non-positive"):
+        dg.check_stability_stats(np.array([0.0, 1.0, 0.0, 1.0, 1.0]), 3)
+    with pytest.warns(RuntimeWarning, match="Max velocity"):
+        dg.check_stability_stats(np.array([0.0, 0.0, 0.5, 10.0, 10.0]), 3)
+    with pytest.warns(RuntimeWarning, match="Mean density"):
+        dg.check_stability_stats(np.array([0.0, 0.0, 0.0, 12.0, 10.0]), 3)
+
+
+# --------------------------------------------------------- argument checks
+def test_solver_argument_errors_match_reference():
+    from aero_amd import Solver, Solver3D
+    solid = np.zeros((8, 16), bool)
+    with pytest.raises(ValueError, match="Unknown collision"):
+        Solver(8, 16, solid, 1.0, 0.05, 4.0, collision="xyz")
+    with pytest.raises(ValueError, match="Unknown backend"):
+        Solver(8, 16, solid, 1.0, 0.05, 4.0, backend="cuda")
+    with pytest.raises(ValueError, match="les_model"):
+        Solver(8, 16, solid, 1.0, 0.05, 4.0, les_model="dynamic")
+    with pytest.raises(ValueError, match="streamwise_bc"):
+        Solver3D(4, 8, 16, np.zeros((4, 8, 16), bool), 1.0, 0.05, 4.0, streamwise_bc="inflow")
