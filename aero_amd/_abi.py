@@ -69,6 +69,7 @@ _SIGS = {
     "aero_device_bytes": ([_P, _P], ctypes.c_int),
     "aero_set_timing": ([_P, ctypes.c_int32], ctypes.c_int),
     "aero_get_timing": ([_P, _P, _P], ctypes.c_int),
+    "aero_cell_counts": ([_P, _P, _P], ctypes.c_int),
 }
 EXPORTED = tuple(_SIGS)
 

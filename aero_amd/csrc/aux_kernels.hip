@@ -22,7 +22,7 @@ hipError_t launch_step_d3q19(int coll, int st, bool thermal, int kmode, const St
 // ------------------------------------------------------------ materialize
 template <class L>
 static hipError_t mat_l(int st, bool thermal, const StepParams& p, hipStream_t s) {
-  const dim3 g((unsigned)p.nblocks_grid);
+  const dim3 g((unsigned)p.nblocks);
 #define AERO_CASE(ST)                                                                  \
   case ST:                                                                             \
     if (thermal) materialize_kernel<L, ST, true><<<g, BLOCK, 0, s>>>(p);               \
@@ -45,7 +45,7 @@ hipError_t launch_materialize(int q, int st, bool thermal, const StepParams& p, 
 // ---------------------------------------------------------------- collide
 template <class L, int COLL>
 static hipError_t col_lc(int st, const CollideParams& p, hipStream_t s) {
-  const dim3 g((unsigned)p.nblocks_grid);
+  const dim3 g((unsigned)p.nblocks);
   switch (st) {
     case ST_F64: collide_kernel<L, COLL, ST_F64><<<g, BLOCK, 0, s>>>(p); break;
     case ST_F32: collide_kernel<L, COLL, ST_F32><<<g, BLOCK, 0, s>>>(p); break;
@@ -82,8 +82,8 @@ __global__ __launch_bounds__(BLOCK) void recycling_plane_kernel(StepParams p, do
   const int y = row % p.ny, z = row / p.ny;
   const int x = p.nx - 2;
   const int cell = row * p.nx + x;
-  C fn[19], fpre[19];
-  pre_outlet_state<D3Q19, ST>(p, (const typename S::T*)p.fin, cell, x, y, z, p.flags[cell], fn, fpre);
+  C fn[19];
+  pre_outlet<D3Q19, ST>(p, (const typename S::T*)p.fin, cell, x, y, z, p.flags[cell], fn);
   double rho = 0, mx = 0, my = 0, mz = 0;
 #pragma unroll
   for (int i = 0; i < 19; ++i) {
@@ -353,46 +353,11 @@ hipError_t launch_check_f64(int q, int nx, int ny, int nz, int zc, const double*
 }
 
 // ------------------------------------------------------------- reductions
-// Momentum-exchange totals in a fixed order: thread t sums link cells
-// t, t+256, ... sequentially, then a fixed tree combines the 256 partials,
-// so a case gives the same bits whatever else shares the device.
-__global__ __launch_bounds__(BLOCK) void reduce_forces_kernel(const double* fpart, int nrec, double fd, double md,
-                                                              double* row) {
-  __shared__ double red[NFORCE][BLOCK];
-  double acc[NFORCE];
-#pragma unroll
-  for (int k = 0; k < NFORCE; ++k) acc[k] = 0.0;
-  for (int r = threadIdx.x; r < nrec; r += BLOCK) {
-#pragma unroll
-    for (int k = 0; k < NFORCE; ++k) acc[k] += fpart[(size_t)r * NFORCE + k];
-  }
-#pragma unroll
-  for (int k = 0; k < NFORCE; ++k) red[k][threadIdx.x] = acc[k];
-  __syncthreads();
-  for (int w = BLOCK / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-#pragma unroll
-      for (int k = 0; k < NFORCE; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    double F[NFORCE];
-#pragma unroll
-    for (int k = 0; k < NFORCE; ++k) F[k] = red[k][0];
-    // forces.py:78-103 / forces3d.py:364-395 and the moment coefficients
-    const bool ok = fd != 0.0;
-    row[0] = ok ? F[0] / fd : 0.0;
-    row[1] = ok ? F[1] / fd : 0.0;
-    row[2] = ok ? F[2] / fd : 0.0;
-    row[3] = md != 0.0 ? F[3] / md : 0.0;
-    row[4] = md != 0.0 ? F[4] / md : 0.0;
-    row[5] = md != 0.0 ? F[5] / md : 0.0;
-    row[6] = ok ? F[6] / fd : 0.0;
-    row[7] = ok ? F[9] / fd : 0.0;
-#pragma unroll
-    for (int k = 0; k < NFORCE; ++k) row[8 + k] = F[k];
-  }
+// Momentum-exchange totals of one step in the same fixed order the step
+// kernel's block 0 uses (reduce_forces_block), for the last step of a call.
+__global__ __launch_bounds__(BLOCK) void reduce_forces_kernel(const double* fpart, int nrec, double fd,
+                                                                      double md, double* row) {
+  reduce_forces_block(fpart, nrec, fd, md, row);
 }
 
 hipError_t launch_reduce_forces(const double* fpart, int nrec, double fd, double md, double* row, hipStream_t s) {

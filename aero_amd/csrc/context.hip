@@ -87,7 +87,7 @@ struct aero_ctx {
   int Q = 0;
   long long N = 0;
   int nrow = 0;
-  int nblocks = 0, nblocks_grid = 0;
+  int nblocks = 0;
   size_t esz = 8;  // storage element size
   size_t csz = 8;  // compute element size
   // state: ping-pong sides, `side` is the one the next step reads
@@ -95,7 +95,8 @@ struct aero_ctx {
   int side = 0;
   int prev_side = 0;
   // geometry
-  DevBuf flags, lrec, row_start, lq, fpart, sigma, minv_d, minv_f, damp;
+  DevBuf flags, lrec, row_start, lq, fpart[2], sigma, minv_d, minv_f, damp;
+  int fpart_side = 0;  // buffer holding the last completed step's per-link-cell force sums
   int nrec = 0;
   std::vector<int> rec_cells;
   // per-step inputs
@@ -131,12 +132,14 @@ struct aero_ctx {
     p.nrow = nrow;
     p.ncell = N;
     p.nblocks = nblocks;
-    p.nblocks_grid = nblocks_grid;
     p.flags = flags.as<uint8_t>();
     p.lrec = lrec.as<LinkCell>();
     p.row_start = row_start.as<int>();
     p.lq = lq.as<float>();
-    p.fpart = fpart.as<double>();
+    p.fpart = fpart[fpart_side].as<double>();
+    p.nrec = nrec;
+    p.fdenom = force_denom();
+    p.mdenom = moment_denom();
     p.sigma = d.sponge_thickness > 0 ? sigma.as<double>() : nullptr;
     p.minv_d = minv_d.as<double>();
     p.minv_f = minv_f.as<float>();
@@ -198,7 +201,6 @@ struct aero_ctx {
     cp.dim = Q == 19 ? 3 : 2;
     cp.ncell = N;
     cp.nblocks = nblocks;
-    cp.nblocks_grid = nblocks_grid;
     cp.F = F;
     cp.U = U;
     cp.fout = B[dst].p;
@@ -289,6 +291,9 @@ void inverse_stream(const aero_ctx* c, const double* g, std::vector<double>& out
 
 }  // namespace
 
+static void set_geometry_impl(aero_ctx* c, const uint8_t* solid, int64_t nlinks, const int32_t* links,
+                              const float* q);
+
 extern "C" {
 
 const char* aero_last_error(void) { return g_err.c_str(); }
@@ -325,7 +330,6 @@ int aero_create(const aero_desc* desc, aero_ctx** out) {
     c->N = N;
     c->nrow = d.nz * d.ny;
     c->nblocks = (int)((N + BLOCK - 1) / BLOCK);
-    c->nblocks_grid = ((c->nblocks + 7) / 8) * 8;
     c->esz = storage_bytes(d.storage);
     c->csz = d.storage == ST_F64 ? 8 : 4;
     ck(hipGetDevice(&c->device), "hipGetDevice");
@@ -366,14 +370,7 @@ int aero_create(const aero_desc* desc, aero_ctx** out) {
     }
     // default geometry: no solid, no links
     std::vector<uint8_t> none((size_t)N, 0);
-    c->flags.alloc((size_t)N);
-    ck(hipMemcpy(c->flags.p, none.data(), (size_t)N, hipMemcpyHostToDevice), "copy flags");
-    std::vector<int> rs(c->nrow + 1, 0);
-    c->row_start.alloc(sizeof(int) * (c->nrow + 1));
-    ck(hipMemcpy(c->row_start.p, rs.data(), sizeof(int) * (c->nrow + 1), hipMemcpyHostToDevice), "copy rows");
-    c->lrec.alloc(sizeof(LinkCell));
-    c->lq.alloc(sizeof(float));
-    c->fpart.alloc(sizeof(double) * NFORCE);
+    set_geometry_impl(c, none.data(), 0, nullptr, nullptr);
   });
   if (rc) {
     aero_destroy(c);
@@ -388,7 +385,7 @@ int aero_destroy(aero_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags,
-                    &c->lrec, &c->row_start, &c->lq, &c->fpart, &c->sigma, &c->minv_d, &c->minv_f,
+                    &c->lrec, &c->row_start, &c->lq, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
                     &c->damp, &c->planes, &c->recyc, &c->hist, &c->fsplit, &c->usplit, &c->fcache,
                     &c->gcache, &c->tmp, &c->checkpart, &c->checkout};
   for (DevBuf* b : bufs) b->release();
@@ -398,10 +395,15 @@ int aero_destroy(aero_ctx* c) {
   return 0;
 }
 
-int aero_set_geometry(aero_ctx* c, const uint8_t* solid, int64_t nlinks, const int32_t* links, const float* q) {
-  return guarded([&] {
+}  // extern "C"
+
+// Cell flags and link-cell records for one geometry.
+static void set_geometry_impl(aero_ctx* c, const uint8_t* solid, int64_t nlinks, const int32_t* links,
+                              const float* q) {
     c->set_device();
     const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
+    c->flags.alloc((size_t)c->N);
+    c->row_start.alloc(sizeof(int) * (c->nrow + 1));
     std::vector<uint8_t> fl((size_t)c->N);
     for (long long k = 0; k < c->N; ++k) fl[k] = solid[k] ? FLAG_SOLID : 0;
     // group links by cell: mask + q per direction
@@ -437,15 +439,21 @@ int aero_set_geometry(aero_ctx* c, const uint8_t* solid, int64_t nlinks, const i
     ck(hipMemcpy(c->flags.p, fl.data(), (size_t)c->N, hipMemcpyHostToDevice), "copy flags");
     c->lrec.alloc(sizeof(LinkCell) * std::max<size_t>(1, recs.size()));
     c->lq.alloc(sizeof(float) * std::max<size_t>(1, qv.size()));
-    c->fpart.alloc(sizeof(double) * NFORCE * std::max<size_t>(1, recs.size()));
+    for (int k = 0; k < 2; ++k) c->fpart[k].alloc(sizeof(double) * NFORCE * std::max<size_t>(1, recs.size()));
+    c->fpart_side = 0;
     if (!recs.empty()) {
       ck(hipMemcpy(c->lrec.p, recs.data(), sizeof(LinkCell) * recs.size(), hipMemcpyHostToDevice), "copy recs");
       ck(hipMemcpy(c->lq.p, qv.data(), sizeof(float) * qv.size(), hipMemcpyHostToDevice), "copy q");
     }
-    ck(hipMemset(c->fpart.p, 0, c->fpart.bytes), "memset fpart");
+    for (int k = 0; k < 2; ++k) ck(hipMemset(c->fpart[k].p, 0, c->fpart[k].bytes), "memset fpart");
     ck(hipMemcpy(c->row_start.p, rs.data(), sizeof(int) * (c->nrow + 1), hipMemcpyHostToDevice), "copy rows");
     c->invalidate();
-  });
+}
+
+extern "C" {
+
+int aero_set_geometry(aero_ctx* c, const uint8_t* solid, int64_t nlinks, const int32_t* links, const float* q) {
+  return guarded([&] { set_geometry_impl(c, solid, nlinks, links, q); });
 }
 
 int aero_set_damping(aero_ctx* c, const double* damp) {
@@ -582,25 +590,38 @@ int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, 
       p.phase = 0.17 * (double)step_no;
       p.g_bc_skip = c->g_skip_pending;
       p.commit = 1;
-      if (c->timing) ck(hipEventRecord(c->ev_pool[2 * k], c->stream), "event");
-      if (c->les_split()) {
+      const int kmode = c->les_split() ? 1 : 0;
+      if (kmode) {
         p.fsplit = c->fsplit.p;
         p.usplit = c->usplit.p;
-        c->launch_step_kernel(1, p);
-        c->collide_field(c->fsplit.p, c->usplit.p, wr);
-      } else {
-        c->launch_step_kernel(0, p);
       }
-      if (c->timing) ck(hipEventRecord(c->ev_pool[2 * k + 1], c->stream), "event");
-      ck(launch_reduce_forces(c->fpart.as<double>(), c->nrec, fd, md, c->hist.as<double>() + (size_t)NCOEF * k,
-                              c->stream),
-         "force reduction");
+      // force sums of this step go to the other buffer; the launch's block 0
+      // reduces the previous step's into that step's coefficient row
+      const int fs = c->fpart_side ^ 1;
+      p.fpart = c->fpart[fs].as<double>();
+      p.fpart_prev = c->fpart[c->fpart_side].as<double>();
+      p.reduce_prev = (k > 0 && c->nrec > 0) ? 1 : 0;
+      p.hist_prev = k > 0 ? c->hist.as<double>() + (size_t)NCOEF * (k - 1) : nullptr;
+      if (c->timing) {
+        p.ev_start = c->ev_pool[2 * k];
+        p.ev_stop = c->ev_pool[2 * k + 1];
+      }
+      c->launch_step_kernel(kmode, p);
+      if (kmode) c->collide_field(c->fsplit.p, c->usplit.p, wr);
+      c->fpart_side = fs;
       c->last_plane = p.plane;
       c->last_step_no = step_no;
       c->last_g_skip = c->g_skip_pending;
       c->g_skip_pending = 0;
       c->side = wr;
       if (c->convective()) c->prev_side ^= 1;
+    }
+    if (c->nrec > 0) {
+      ck(launch_reduce_forces(c->fpart[c->fpart_side].as<double>(), c->nrec, fd, md,
+                              c->hist.as<double>() + (size_t)NCOEF * (nsteps - 1), c->stream),
+         "force reduction");
+    } else {
+      ck(hipMemsetAsync(c->hist.p, 0, sizeof(double) * NCOEF * (size_t)nsteps, c->stream), "zero history");
     }
     c->invalidate();
     if (hist) {
@@ -626,6 +647,13 @@ int aero_set_timing(aero_ctx* c, int32_t enabled) {
     c->timing = enabled != 0;
     c->timed_ms = 0.0;
     c->timed_launches = 0;
+  });
+}
+
+int aero_cell_counts(aero_ctx* c, int64_t* total, int64_t* link_cells) {
+  return guarded([&] {
+    *total = c->N;
+    *link_cells = c->nrec;
   });
 }
 
@@ -676,7 +704,7 @@ int aero_link_forces(aero_ctx* c, int32_t* cells, double* forces, int32_t* nrec)
     if (cells) std::memcpy(cells, c->rec_cells.data(), sizeof(int) * c->nrec);
     if (forces && c->nrec) {
       ck(hipStreamSynchronize(c->stream), "sync");
-      ck(hipMemcpy(forces, c->fpart.p, sizeof(double) * NFORCE * c->nrec, hipMemcpyDeviceToHost), "download");
+      ck(hipMemcpy(forces, c->fpart[c->fpart_side].p, sizeof(double) * NFORCE * c->nrec, hipMemcpyDeviceToHost), "download");
     }
   });
 }
@@ -691,7 +719,7 @@ int aero_synchronize(aero_ctx* c) {
 int aero_device_bytes(aero_ctx* c, int64_t* bytes) {
   return guarded([&] {
     const DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags,
-                            &c->lrec, &c->row_start, &c->lq, &c->fpart, &c->sigma, &c->minv_d, &c->minv_f,
+                            &c->lrec, &c->row_start, &c->lq, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
                             &c->damp, &c->planes, &c->recyc, &c->hist, &c->fsplit, &c->usplit, &c->fcache,
                             &c->gcache, &c->tmp, &c->checkpart, &c->checkout};
     int64_t t = 0;

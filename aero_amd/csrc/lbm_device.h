@@ -26,6 +26,7 @@ enum InletMode {
 enum OutletMode { OUT_NONE = 0, OUT_CONVECTIVE = 1, OUT_ZEROGRAD = 2, OUT_PRESSURE = 3 };
 enum WallMode { WALL_NONE = 0, WALL_SLIP = 1, WALL_NOSLIP = 2, WALL_MOVING = 3 };
 enum LesMode { LES_OFF = 0, LES_FD_SMAG = 1, LES_FD_WALE = 2 };
+// Per-cell flags: solid, or a fluid cell with at least one surface link.
 enum CellFlag { FLAG_SOLID = 1, FLAG_LINKS = 2 };
 
 // ------------------------------------------------------------------ lattices
@@ -147,8 +148,7 @@ struct StepParams {
   int nx, ny, nz;
   int nrow;  // nz * ny
   long long ncell;
-  int nblocks;       // logical blocks covering ncell
-  int nblocks_grid;  // launched blocks (multiple of 8, XCD remap)
+  int nblocks;  // blocks covering ncell (the step launch adds one reduction block)
   const void* fin;
   void* fout;
   const void* gin;
@@ -159,7 +159,12 @@ struct StepParams {
   const LinkCell* lrec;
   const int* row_start;
   const float* lq;
-  double* fpart;  // NFORCE doubles per link cell
+  double* fpart;             // NFORCE doubles per link cell, this step
+  const double* fpart_prev;  // the previous step's sums, reduced by the extra block
+  double* hist_prev;         // coefficient row the previous step's sums go to
+  int reduce_prev;
+  int nrec;  // link cells
+  double fdenom, mdenom;
   const double* prev_in;
   double* prev_out;
   const double* sigma;  // sponge sigma(x), nx entries, or null
@@ -178,17 +183,10 @@ struct StepParams {
   int commit;    // write side effects (outlet column state, scalar, forces)
   int out_mode;  // materialize: 0 f64 field, 1 f64 macroscopic, 2 check partials
   int check_z;   // materialize/check: restrict to this z plane (-1: all)
+  void* ev_start;  // host side only: events around the step launch (timing)
+  void* ev_stop;
   double* fmat;
   double* check_part;
 };
-
-__device__ __forceinline__ int logical_block(const StepParams& p) {
-  // Consecutive logical blocks land on the same XCD (hardware dispatches
-  // workgroup b to XCD b % 8), so the cache lines that neighbouring blocks
-  // share through the +-1 x shifts are reused in one L2 instead of eight.
-  const int b = blockIdx.x;
-  const int per = p.nblocks_grid >> 3;
-  return (b & 7) * per + (b >> 3);
-}
 
 }  // namespace aero

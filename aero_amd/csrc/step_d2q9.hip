@@ -1,23 +1,19 @@
-// D2Q9 instantiations of the fused step, the materialisation pass and the
-// field collision.
+// D2Q9 instantiations of the step kernels (one fused launch per step).
 #include "launch.h"
 #include "step_impl.h"
 
 namespace aero {
 
 namespace {
-dim3 grid_of(const StepParams& p) { return dim3((unsigned)p.nblocks_grid); }
+template <int COLL, int ST, bool TH, int KM>
+hipError_t pair(const StepParams& p, hipStream_t s) {
+  return launch_step<D2Q9, COLL, ST, TH, KM>(p, s);
+}
 
 template <int COLL, int ST>
 hipError_t step_t(bool thermal, int kmode, const StepParams& p, hipStream_t s) {
-  if (thermal) {
-    if (kmode == 0) step_kernel<D2Q9, COLL, ST, true, 0><<<grid_of(p), BLOCK, 0, s>>>(p);
-    else step_kernel<D2Q9, COLL, ST, true, 1><<<grid_of(p), BLOCK, 0, s>>>(p);
-  } else {
-    if (kmode == 0) step_kernel<D2Q9, COLL, ST, false, 0><<<grid_of(p), BLOCK, 0, s>>>(p);
-    else step_kernel<D2Q9, COLL, ST, false, 1><<<grid_of(p), BLOCK, 0, s>>>(p);
-  }
-  return hipGetLastError();
+  if (thermal) return kmode == 0 ? pair<COLL, ST, true, 0>(p, s) : pair<COLL, ST, true, 1>(p, s);
+  return kmode == 0 ? pair<COLL, ST, false, 0>(p, s) : pair<COLL, ST, false, 1>(p, s);
 }
 
 template <int COLL>
@@ -36,7 +32,7 @@ hipError_t launch_step_d2q9(int coll, int st, bool thermal, int kmode, const Ste
   switch (coll) {
     case COLL_BGK: return step_c<COLL_BGK>(st, thermal, kmode, p, s);
     case COLL_TRT: return step_c<COLL_TRT>(st, thermal, kmode, p, s);
-    case COLL_MRT: return step_c<COLL_MRT>(st, thermal, kmode, p, s);
+    case COLL_MRT: return step_c<COLL_MRT>(st, thermal, 0, p, s);  // MRT never takes the LES field
   }
   return hipErrorInvalidValue;
 }

@@ -1,28 +1,29 @@
-// D3Q19 TRT instantiations of the fused step.
+// D3Q19 TRT instantiations of the step kernels (one fused launch per step).
 #include "launch.h"
 #include "step_impl.h"
 
 namespace aero {
 
+namespace {
+template <int ST, bool TH, int KM>
+hipError_t pair(const StepParams& p, hipStream_t s) {
+  return launch_step<D3Q19, COLL_TRT, ST, TH, KM>(p, s);
+}
+
+template <int ST>
+hipError_t step_t(bool thermal, int kmode, const StepParams& p, hipStream_t s) {
+  return thermal ? (kmode == 0 ? pair<ST, true, 0>(p, s) : pair<ST, true, 1>(p, s))
+                 : (kmode == 0 ? pair<ST, false, 0>(p, s) : pair<ST, false, 1>(p, s));
+}
+}  // namespace
+
 hipError_t launch_step_d3q19_trt(int st, bool thermal, int kmode, const StepParams& p, hipStream_t s) {
-  const dim3 g((unsigned)p.nblocks_grid);
-#define AERO_CASE(ST)                                                                    \
-  case ST:                                                                               \
-    if (thermal) {                                                                       \
-      if (kmode == 0) step_kernel<D3Q19, COLL_TRT, ST, true, 0><<<g, BLOCK, 0, s>>>(p);  \
-      else step_kernel<D3Q19, COLL_TRT, ST, true, 1><<<g, BLOCK, 0, s>>>(p);             \
-    } else {                                                                             \
-      if (kmode == 0) step_kernel<D3Q19, COLL_TRT, ST, false, 0><<<g, BLOCK, 0, s>>>(p); \
-      else step_kernel<D3Q19, COLL_TRT, ST, false, 1><<<g, BLOCK, 0, s>>>(p);            \
-    }                                                                                    \
-    return hipGetLastError();
   switch (st) {
-    AERO_CASE(ST_F64)
-    AERO_CASE(ST_F32)
-    AERO_CASE(ST_F16S)
-    AERO_CASE(ST_BF16S)
+    case ST_F64: return step_t<ST_F64>(thermal, kmode, p, s);
+    case ST_F32: return step_t<ST_F32>(thermal, kmode, p, s);
+    case ST_F16S: return step_t<ST_F16S>(thermal, kmode, p, s);
+    case ST_BF16S: return step_t<ST_BF16S>(thermal, kmode, p, s);
   }
-#undef AERO_CASE
   return hipErrorInvalidValue;
 }
 

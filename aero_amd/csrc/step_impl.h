@@ -18,6 +18,7 @@
 // x = nx-1 from B, so one launch per step is enough and no grid-wide barrier
 // is needed.
 #pragma once
+#include <hip/hip_ext.h>
 
 #include "lbm_device.h"
 
@@ -68,46 +69,109 @@ __device__ __forceinline__ int find_link_record(const StepParams& p, int cell, i
   return -1;
 }
 
-// Bounce-back (boundary.py:283-300) or Bouzidi (boundary.py:303-352) on the
-// links of one cell.  Loads the cell's own post-collision populations into
-// fpre (also needed by the force sums).  Returns the record index.
+// Bounce-back (boundary.py:283-300, boundary3d.py:333-350) or Bouzidi
+// (boundary.py:303-352, boundary3d.py:353-404) on the links of record `rec`.
+// For link i the pulled fn[OPP[i]] is replaced by the cell's own outgoing
+// f_pre[i] (plain bounce-back), or interpolated with q: bouzidi 1 follows the
+// reference (q < 1/2 uses f_pre[OPP[i]] at the cell, q >= 1/2 f_pre[i] at the
+// clipped x - e_i), bouzidi 2 is the linear rule (q < 1/2 takes f_pre[i] at
+// x - e_i, q >= 1/2 f_pre[OPP[i]] at the cell).  The cell's f_pre is not held
+// in registers: each link reloads the two values it needs (cache hits, the
+// lines were just pulled), so the common path keeps its small register budget.
 template <class L, int ST>
-__device__ __forceinline__ int apply_links(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
-                                           int cell, int x, int y, int z,
-                                           typename Storage<ST>::C fn[L::Q],
-                                           typename Storage<ST>::C fpre[L::Q]) {
+__device__ __forceinline__ void apply_links_light(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
+                                                  int rec, int cell, int x, int y, int z,
+                                                  typename Storage<ST>::C fn[L::Q]) {
   using S = Storage<ST>;
   using C = typename S::C;
   const size_t N = (size_t)p.ncell;
-  const int rec = find_link_record(p, cell, z * p.ny + y);
-  if (rec < 0) return -1;
   const LinkCell r = p.lrec[rec];
-#pragma unroll
-  for (int i = 0; i < L::Q; ++i) fpre[i] = S::load(B + (size_t)i * N, (size_t)cell, C(L::W[i]));
 #pragma unroll
   for (int i = 1; i < L::Q; ++i) {
     if ((r.mask >> i) & 1u) {
       const int o = L::OPP[i];
+      const C fi = S::load(B + (size_t)i * N, (size_t)cell, C(L::W[i]));
       if (!p.bouzidi) {
-        fn[o] = fpre[i];
+        fn[o] = fi;
       } else {
         const int k = __popc(r.mask & ((1u << i) - 1u));
         const C q = (C)p.lq[r.qoff + k];
-        if (q < C(0.5)) {
-          fn[o] = C(2) * q * fpre[i] + (C(1) - C(2) * q) * fpre[o];
-        } else {
+        const bool upstream = p.bouzidi == 1 ? !(q < C(0.5)) : (q < C(0.5));
+        C partner;
+        if (upstream) {
           const int xn = clampi(x + L::EX[o], 0, p.nx - 1);
           const int yn = clampi(y + L::EY[o], 0, p.ny - 1);
           const int zn = clampi(z + L::EZ[o], 0, p.nz - 1);
-          const size_t nb = ((size_t)zn * p.ny + yn) * p.nx + xn;
-          const C fnb = S::load(B + (size_t)i * N, nb, C(L::W[i]));
+          partner = S::load(B + (size_t)i * N, ((size_t)zn * p.ny + yn) * p.nx + xn, C(L::W[i]));
+        } else {
+          partner = S::load(B + (size_t)o * N, (size_t)cell, C(L::W[o]));
+        }
+        if (q < C(0.5)) {
+          fn[o] = C(2) * q * fi + (C(1) - C(2) * q) * partner;
+        } else {
           const C inv2q = C(1) / (C(2) * q);
-          fn[o] = inv2q * fpre[i] + (C(1) - inv2q) * fnb;
+          fn[o] = inv2q * fi + (C(1) - inv2q) * partner;
         }
       }
     }
   }
-  return rec;
+}
+
+// Momentum exchange of one link cell (forces.py:15-75 and 106-139,
+// forces3d.py:8-66 and 398-425) streaming f_pre from memory, FP64 sums
+// written to the cell's record slot.
+template <class L, int ST>
+__device__ __forceinline__ void link_forces_light(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
+                                                  int rec, int cell, int x, int y, int z,
+                                                  const typename Storage<ST>::C fn[L::Q]) {
+  using S = Storage<ST>;
+  using C = typename S::C;
+  const size_t N = (size_t)p.ncell;
+  double rho = 0, mx = 0, my = 0, mz = 0;
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    const double v = (double)S::load(B + (size_t)i * N, (size_t)cell, C(L::W[i]));
+    rho += v;
+    mx += L::EX[i] * v;
+    my += L::EY[i] * v;
+    mz += L::EZ[i] * v;
+  }
+  const double inv = rho > 0.0 ? 1.0 / rho : 0.0;
+  const double ux = inv * mx, uy = inv * my, uz = inv * mz;
+  const double usq = ux * ux + uy * uy + uz * uz;
+  const double rx = (double)x - p.center[0], ry = (double)y - p.center[1], rz = (double)z - p.center[2];
+  const uint32_t mask = p.lrec[rec].mask;
+  double out[NFORCE];
+#pragma unroll
+  for (int k = 0; k < NFORCE; ++k) out[k] = 0.0;
+#pragma unroll
+  for (int i = 1; i < L::Q; ++i) {
+    if ((mask >> i) & 1u) {
+      const int o = L::OPP[i];
+      const double fo = (double)S::load(B + (size_t)i * N, (size_t)cell, C(L::W[i]));
+      const double fi = (double)fn[o];
+      const double mom = fo + fi;
+      const double dfx = L::EX[i] * mom, dfy = L::EY[i] * mom, dfz = L::EZ[i] * mom;
+      out[0] += dfx;
+      out[1] += dfy;
+      out[2] += dfz;
+      out[3] += ry * dfz - rz * dfy;
+      out[4] += rz * dfx - rx * dfz;
+      out[5] += rx * dfy - ry * dfx;
+      const double feo = feq_i<double>(L::W[i], rho, L::EX[i] * ux + L::EY[i] * uy + L::EZ[i] * uz, usq);
+      const double fei = feq_i<double>(L::W[o], rho, L::EX[o] * ux + L::EY[o] * uy + L::EZ[o] * uz, usq);
+      const double mp = feo + fei;
+      const double mv = (fo - feo) + (fi - fei);
+      out[6] += L::EX[i] * mp;
+      out[7] += L::EY[i] * mp;
+      out[8] += L::EZ[i] * mp;
+      out[9] += L::EX[i] * mv;
+      out[10] += L::EY[i] * mv;
+      out[11] += L::EZ[i] * mv;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NFORCE; ++k) p.fpart[(size_t)rec * NFORCE + k] = out[k];
 }
 
 // Uniform Guo body force on a fluid cell (ibm_guo.py:264-301).
@@ -233,63 +297,11 @@ __device__ __forceinline__ void apply_inlet(const StepParams& p, int y, int z, C
   else inlet_3d<C>(p, y, z, fn);
 }
 
-// State of a cell after streaming, body force, links and inlet -- i.e. what
-// the outlet condition reads at x = nx-2 (boundary.py:68-113).
-template <class L, int ST>
-__device__ __forceinline__ int pre_outlet_state(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
-                                                int cell, int x, int y, int z, uint8_t flag,
-                                                typename Storage<ST>::C fn[L::Q],
-                                                typename Storage<ST>::C fpre[L::Q]) {
-  using C = typename Storage<ST>::C;
-  gather<L, ST>(p, B, x, y, z, C(1), fn);
-  if (L::DIM == 3 && p.body_force_on && !(flag & FLAG_SOLID)) body_force<L, C>(p, fn);
-  int rec = -1;
-  if (flag & FLAG_LINKS) rec = apply_links<L, ST>(p, B, cell, x, y, z, fn, fpre);
-  if (x == 0) apply_inlet<L, C>(p, y, z, fn);
-  return rec;
-}
-
-// Full post-boundary state of one cell (the reference's self.f before the
-// scalar step).  With commit set, the convective outlet state is advanced.
-template <class L, int ST>
-__device__ __forceinline__ int post_bc_state(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
-                                             int cell, int x, int y, int z, uint8_t flag,
-                                             typename Storage<ST>::C fn[L::Q],
-                                             typename Storage<ST>::C fpre[L::Q]) {
-  using C = typename Storage<ST>::C;
+// Sponge relaxation (sponge.py:24-70) and the y-wall conditions
+// (boundary.py:162-226), in the reference's order; both read only the cell.
+template <class L, class C>
+__device__ __forceinline__ void sponge_and_walls(const StepParams& p, int x, int y, C fn[L::Q]) {
   constexpr int Q = L::Q;
-  const int rec = pre_outlet_state<L, ST>(p, B, cell, x, y, z, flag, fn, fpre);
-  if (x == p.nx - 1 && p.outlet_mode != OUT_NONE) {
-    if (p.outlet_mode == OUT_PRESSURE) {  // 2D only, boundary.py:139-159
-      if constexpr (L::DIM == 2) {
-        const C rout = (C)p.rho_out;
-        const C ux = C(-1) + (fn[0] + fn[2] + fn[4] + C(2) * (fn[1] + fn[5] + fn[8])) / rout;
-        fn[3] = fn[1] - C(2.0 / 3.0) * rout * ux;
-        fn[7] = fn[5] + C(0.5) * (fn[2] - fn[4]) - C(1.0 / 6.0) * rout * ux;
-        fn[6] = fn[8] - C(0.5) * (fn[2] - fn[4]) - C(1.0 / 6.0) * rout * ux;
-      }
-    } else {
-      C nb[Q], nbpre[Q];
-      const int xn = p.nx - 2;
-      const int cn = cell - 1;
-      pre_outlet_state<L, ST>(p, B, cn, xn, y, z, p.flags[cn], nb, nbpre);
-      if (p.outlet_mode == OUT_ZEROGRAD) {
-#pragma unroll
-        for (int i = 0; i < Q; ++i) fn[i] = nb[i];
-      } else {  // convective, boundary.py:78-113
-        double u = p.u_conv < 0.0 ? 0.0 : (p.u_conv > 1.0 ? 1.0 : p.u_conv);
-        const C cu = (C)u;
-        const int row = z * p.ny + y;
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-          const size_t k = (size_t)i * p.nrow + row;
-          const C v = (C(1) - cu) * (C)p.prev_in[k] + cu * nb[i];
-          fn[i] = v;
-          if (p.commit) p.prev_out[k] = (double)v;
-        }
-      }
-    }
-  }
   if (p.sigma != nullptr) {  // sponge.py:24-70
     const double s = p.sigma[x];
     if (s > 0.0) {
@@ -334,6 +346,68 @@ __device__ __forceinline__ int post_bc_state(const StepParams& p, const typename
       }
     }
   }
+}
+
+// State of a cell after streaming, body force, links and inlet: what the
+// outlet and the recycling inlet read at x = nx-2 (boundary.py:68-113,
+// boundary3d.py:185-190).  Returns the cell's link record or -1.
+template <class L, int ST>
+__device__ __forceinline__ int pre_outlet(const StepParams& p, const typename Storage<ST>::T* __restrict__ B, int cell,
+                                          int x, int y, int z, uint8_t flag, typename Storage<ST>::C fn[L::Q]) {
+  using C = typename Storage<ST>::C;
+  gather<L, ST>(p, B, x, y, z, C(1), fn);
+  if (L::DIM == 3 && p.body_force_on && !(flag & FLAG_SOLID)) body_force<L, C>(p, fn);
+  int rec = -1;
+  if (flag & FLAG_LINKS) {
+    rec = find_link_record(p, cell, z * p.ny + y);
+    if (rec >= 0) apply_links_light<L, ST>(p, B, rec, cell, x, y, z, fn);
+  }
+  if (x == 0) apply_inlet<L, C>(p, y, z, fn);
+  return rec;
+}
+
+// Post-boundary state of one cell (the reference's self.f before the scalar
+// step): pull, [3D body force], links, inlet, outlet, sponge, walls.  The
+// outlet overwrites every population of the x = nx-1 column with the state of
+// x = nx-2 (boundary.py:68-113), so that column pulls the x = nx-2 state
+// instead of its own.  With commit set the convective outlet's state is
+// advanced.  Returns the cell's own link record or -1.
+template <class L, int ST>
+__device__ __forceinline__ int post_bc(const StepParams& p, const typename Storage<ST>::T* __restrict__ B, int cell,
+                                       int x, int y, int z, uint8_t flag, bool commit,
+                                       typename Storage<ST>::C fn[L::Q]) {
+  using C = typename Storage<ST>::C;
+  constexpr int Q = L::Q;
+  const int row = z * p.ny + y;
+  const bool redirect = x == p.nx - 1 && (p.outlet_mode == OUT_ZEROGRAD || p.outlet_mode == OUT_CONVECTIVE);
+  const int xg = redirect ? p.nx - 2 : x;
+  const int cg = redirect ? cell - 1 : cell;
+  const uint8_t fg = redirect ? p.flags[cg] : flag;
+  int rec = pre_outlet<L, ST>(p, B, cg, xg, y, z, fg, fn);
+  if (redirect) {
+    if (p.outlet_mode == OUT_CONVECTIVE) {  // boundary.py:78-113
+      const double u = p.u_conv < 0.0 ? 0.0 : (p.u_conv > 1.0 ? 1.0 : p.u_conv);
+      const C cu = (C)u;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const size_t k = (size_t)i * p.nrow + row;
+        const C v = (C(1) - cu) * (C)p.prev_in[k] + cu * fn[i];
+        fn[i] = v;
+        if (commit) p.prev_out[k] = (double)v;
+      }
+    }  // zero gradient (boundary.py:68-75): fn already is the x = nx-2 state
+    rec = (flag & FLAG_LINKS) ? find_link_record(p, cell, row) : -1;
+  }
+  if constexpr (L::DIM == 2) {
+    if (x == p.nx - 1 && p.outlet_mode == OUT_PRESSURE) {  // boundary.py:139-159
+      const C rout = (C)p.rho_out;
+      const C ux = C(-1) + (fn[0] + fn[2] + fn[4] + C(2) * (fn[1] + fn[5] + fn[8])) / rout;
+      fn[3] = fn[1] - C(2.0 / 3.0) * rout * ux;
+      fn[7] = fn[5] + C(0.5) * (fn[2] - fn[4]) - C(1.0 / 6.0) * rout * ux;
+      fn[6] = fn[8] - C(0.5) * (fn[2] - fn[4]) - C(1.0 / 6.0) * rout * ux;
+    }
+  }
+  sponge_and_walls<L, C>(p, x, y, fn);
   return rec;
 }
 
@@ -387,53 +461,6 @@ __device__ __forceinline__ void scalar_step(const StepParams& p, int cell, int x
       const C geq = C(L::W[i]) * T * (C(1) + C(3) * eu);
       const C gp = solid ? C(L::W[i]) * T : g[i] - om * (g[i] - geq);
       S::store(gout + (size_t)i * N, (size_t)cell, gp, C(L::W[i]) * (C)p.gshift);
-    }
-  }
-}
-
-// Momentum exchange at one link cell (forces.py:15-75, forces3d.py:8-66 and
-// the moment sums, forces.py:106-139 / forces3d.py:398-425), FP64.
-template <class L, class C>
-__device__ __forceinline__ void link_forces(const StepParams& p, const LinkCell& r, int x, int y, int z,
-                                            const C fpre[L::Q], const C fn[L::Q], double out[NFORCE]) {
-  double rho = 0, mx = 0, my = 0, mz = 0;
-#pragma unroll
-  for (int i = 0; i < L::Q; ++i) {
-    const double v = (double)fpre[i];
-    rho += v;
-    mx += L::EX[i] * v;
-    my += L::EY[i] * v;
-    mz += L::EZ[i] * v;
-  }
-  const double inv = rho > 0.0 ? 1.0 / rho : 0.0;
-  const double ux = inv * mx, uy = inv * my, uz = inv * mz;
-  const double usq = ux * ux + uy * uy + uz * uz;
-  const double rx = (double)x - p.center[0], ry = (double)y - p.center[1], rz = (double)z - p.center[2];
-#pragma unroll
-  for (int k = 0; k < NFORCE; ++k) out[k] = 0.0;
-#pragma unroll
-  for (int i = 1; i < L::Q; ++i) {
-    if ((r.mask >> i) & 1u) {
-      const int o = L::OPP[i];
-      const double fo = (double)fpre[i], fi = (double)fn[o];
-      const double mom = fo + fi;
-      const double dfx = L::EX[i] * mom, dfy = L::EY[i] * mom, dfz = L::EZ[i] * mom;
-      out[0] += dfx;
-      out[1] += dfy;
-      out[2] += dfz;
-      out[3] += ry * dfz - rz * dfy;
-      out[4] += rz * dfx - rx * dfz;
-      out[5] += rx * dfy - ry * dfx;
-      const double feo = feq_i<double>(L::W[i], rho, L::EX[i] * ux + L::EY[i] * uy + L::EZ[i] * uz, usq);
-      const double fei = feq_i<double>(L::W[o], rho, L::EX[o] * ux + L::EY[o] * uy + L::EZ[o] * uz, usq);
-      const double mp = feo + fei;
-      const double mv = (fo - feo) + (fi - fei);
-      out[6] += L::EX[i] * mp;
-      out[7] += L::EY[i] * mp;
-      out[8] += L::EZ[i] * mp;
-      out[9] += L::EX[i] * mv;
-      out[10] += L::EY[i] * mv;
-      out[11] += L::EZ[i] * mv;
     }
   }
 }
@@ -507,36 +534,16 @@ __device__ __forceinline__ void collide(const double* mrt_s, const double* minv_
 }
 
 // ---------------------------------------------------------------- kernels
-// KMODE 0: fused step (collide into fout).  KMODE 1: split step -- commit the
-// boundary state and write the post-BC populations + velocity for the LES
-// finite-difference pass.
-template <class L, int COLL, int ST, bool THERMAL, int KMODE>
-__global__ __launch_bounds__(BLOCK) void step_kernel(StepParams p) {
+// Tail of a step for one cell: collide into fout (KMODE 0), or -- split step
+// for the LES finite-difference pass -- write the post-boundary populations
+// and velocity (KMODE 1).
+template <class L, int COLL, int ST, int KMODE>
+__device__ __forceinline__ void finish_cell(const StepParams& p, int cell, bool solid,
+                                            const typename Storage<ST>::C fn[L::Q]) {
   using S = Storage<ST>;
   using C = typename S::C;
   using T = typename S::T;
   constexpr int Q = L::Q;
-  const int lb = logical_block(p);
-  if (lb >= p.nblocks) return;
-  const long long c64 = (long long)lb * BLOCK + threadIdx.x;
-  if (c64 >= p.ncell) return;
-  const int cell = (int)c64;
-  const int x = cell % p.nx;
-  const int yz = cell / p.nx;
-  const int y = yz % p.ny;
-  const int z = yz / p.ny;
-  const uint8_t flag = p.flags[cell];
-  const bool solid = flag & FLAG_SOLID;
-  const T* B = (const T*)p.fin;
-  C fn[Q], fpre[Q];
-  const int rec = post_bc_state<L, ST>(p, B, cell, x, y, z, flag, fn, fpre);
-  if constexpr (THERMAL) scalar_step<L, ST>(p, cell, x, y, z, solid, fn, true);
-  if (rec >= 0) {
-    double fo[NFORCE];
-    link_forces<L, C>(p, p.lrec[rec], x, y, z, fpre, fn, fo);
-#pragma unroll
-    for (int k = 0; k < NFORCE; ++k) p.fpart[(size_t)rec * NFORCE + k] = fo[k];
-  }
   const size_t N = (size_t)p.ncell;
   if constexpr (KMODE == 0) {
     C out[Q];
@@ -563,6 +570,106 @@ __global__ __launch_bounds__(BLOCK) void step_kernel(StepParams p) {
   }
 }
 
+__device__ __forceinline__ void write_coefficients(const double F[NFORCE], double fd, double md, double* row) {
+  // forces.py:78-139 / forces3d.py:364-439
+  const bool ok = fd != 0.0;
+  row[0] = ok ? F[0] / fd : 0.0;
+  row[1] = ok ? F[1] / fd : 0.0;
+  row[2] = ok ? F[2] / fd : 0.0;
+  row[3] = md != 0.0 ? F[3] / md : 0.0;
+  row[4] = md != 0.0 ? F[4] / md : 0.0;
+  row[5] = md != 0.0 ? F[5] / md : 0.0;
+  row[6] = ok ? F[6] / fd : 0.0;
+  row[7] = ok ? F[9] / fd : 0.0;
+#pragma unroll
+  for (int k = 0; k < NFORCE; ++k) row[8 + k] = F[k];
+}
+
+// Fixed-order FP64 reduction of the per-link-cell momentum-exchange sums by
+// one block of BLOCK threads: thread t sums link cells t, t+256, ...
+// sequentially, then a fixed tree combines the partials.  The order never
+// depends on what else runs on the device, so a case reproduces its own bits
+// alone or in a batch, and whichever launch performs it.
+__device__ __forceinline__ void reduce_forces_block(const double* fpart, int nrec, double fd, double md,
+                                                    double* row) {
+  __shared__ double red[NFORCE][BLOCK];
+  double acc[NFORCE];
+#pragma unroll
+  for (int k = 0; k < NFORCE; ++k) acc[k] = 0.0;
+  for (int r = threadIdx.x; r < nrec; r += BLOCK) {
+#pragma unroll
+    for (int k = 0; k < NFORCE; ++k) acc[k] += fpart[(size_t)r * NFORCE + k];
+  }
+#pragma unroll
+  for (int k = 0; k < NFORCE; ++k) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int k = 0; k < NFORCE; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double F[NFORCE];
+#pragma unroll
+    for (int k = 0; k < NFORCE; ++k) F[k] = red[k][0];
+    write_coefficients(F, fd, md, row);
+  }
+}
+
+// One reference step for every cell in a single launch.  Each thread pulls
+// its cell's populations and applies, in the reference's order, the
+// conditions that concern it: links, inlet, outlet (the outlet column pulls
+// the x = nx-2 state instead of its own, since the outlet overwrites all of
+// it), sponge, walls, scalar + buoyancy; link cells then write their FP64
+// momentum-exchange sums, and the cell is collided for the next step.
+// Rare work (links, outlet) sits behind flag / column branches and streams
+// its operands from memory, so the register budget -- and the occupancy that
+// hides HBM latency -- is set by the common pull + collide path.  One extra
+// block (index 0, dispatched first so it overlaps the cell blocks) reduces the
+// previous step's force sums, written by the previous launch, into that
+// step's coefficient row.
+template <class L, int COLL, int ST, bool THERMAL, int KMODE>
+__global__ __launch_bounds__(BLOCK) void step_kernel(StepParams p) {
+  using S = Storage<ST>;
+  using C = typename S::C;
+  using T = typename S::T;
+  constexpr int Q = L::Q;
+  if (blockIdx.x == 0) {
+    if (p.reduce_prev) reduce_forces_block(p.fpart_prev, p.nrec, p.fdenom, p.mdenom, p.hist_prev);
+    return;
+  }
+  const long long c64 = (long long)(blockIdx.x - 1) * BLOCK + threadIdx.x;
+  if (c64 >= p.ncell) return;
+  const int cell = (int)c64;
+  const int x = cell % p.nx;
+  const int yz = cell / p.nx;
+  const int y = yz % p.ny;
+  const int z = yz / p.ny;
+  const uint8_t flag = p.flags[cell];
+  const bool solid = flag & FLAG_SOLID;
+  const T* B = (const T*)p.fin;
+  C fn[Q];
+  const int rec = post_bc<L, ST>(p, B, cell, x, y, z, flag, true, fn);
+  if constexpr (THERMAL) scalar_step<L, ST>(p, cell, x, y, z, solid, fn, true);
+  if (rec >= 0) link_forces_light<L, ST>(p, B, rec, cell, x, y, z, fn);
+  finish_cell<L, COLL, ST, KMODE>(p, cell, solid, fn);
+}
+
+// One step = one launch of nblocks + 1 blocks; with timing events set they
+// are recorded by the packet processor at the kernel's start and end.
+template <class L, int COLL, int ST, bool THERMAL, int KMODE>
+hipError_t launch_step(const StepParams& p, hipStream_t s) {
+  const dim3 grid((unsigned)(p.nblocks + 1));
+  if (p.ev_start)
+    hipExtLaunchKernelGGL(step_kernel<L, COLL, ST, THERMAL, KMODE>, grid, dim3(BLOCK), 0, s,
+                          (hipEvent_t)p.ev_start, (hipEvent_t)p.ev_stop, 0, p);
+  else
+    step_kernel<L, COLL, ST, THERMAL, KMODE><<<grid, BLOCK, 0, s>>>(p);
+  return hipGetLastError();
+}
+
 // Side-effect-free re-derivation of the reference's post-boundary state of
 // the last completed step (reads the previous buffer side).  out_mode 0
 // writes the f64 field, 1 writes rho/ux/uy/uz, 2 reduces the stability check.
@@ -573,8 +680,7 @@ __global__ __launch_bounds__(BLOCK) void materialize_kernel(StepParams p) {
   using T = typename S::T;
   constexpr int Q = L::Q;
   __shared__ double red[BLOCK / 64][NCHECK];
-  const int lb = logical_block(p);
-  if (lb >= p.nblocks) return;
+  const int lb = blockIdx.x;
   const long long c64 = (long long)lb * BLOCK + threadIdx.x;
   double v_nan = 0, v_neg = 0, v_umax = 0, v_rho = 0, v_cnt = 0;
   if (c64 < p.ncell) {
@@ -586,8 +692,8 @@ __global__ __launch_bounds__(BLOCK) void materialize_kernel(StepParams p) {
     const bool in_plane = p.check_z < 0 || z == p.check_z;
     if (in_plane) {
       const uint8_t flag = p.flags[cell];
-      C fn[Q], fpre[Q];
-      post_bc_state<L, ST>(p, (const T*)p.fin, cell, x, y, z, flag, fn, fpre);
+      C fn[Q];
+      post_bc<L, ST>(p, (const T*)p.fin, cell, x, y, z, flag, false, fn);
       if constexpr (THERMAL) scalar_step<L, ST>(p, cell, x, y, z, flag & FLAG_SOLID, fn, false);
       const size_t N = (size_t)p.ncell;
       if (p.out_mode == 0) {
@@ -661,7 +767,7 @@ __global__ __launch_bounds__(BLOCK) void materialize_kernel(StepParams p) {
 struct CollideParams {
   int nx, ny, nz, dim;
   long long ncell;
-  int nblocks, nblocks_grid;
+  int nblocks;
   const void* F;
   const void* U;
   void* fout;
@@ -751,11 +857,7 @@ __global__ __launch_bounds__(BLOCK) void collide_kernel(CollideParams p) {
   using C = typename S::C;
   using T = typename S::T;
   constexpr int Q = L::Q;
-  const int b = blockIdx.x;
-  const int per = p.nblocks_grid >> 3;
-  const int lb = (b & 7) * per + (b >> 3);
-  if (lb >= p.nblocks) return;
-  const long long c64 = (long long)lb * BLOCK + threadIdx.x;
+  const long long c64 = (long long)blockIdx.x * BLOCK + threadIdx.x;
   if (c64 >= p.ncell) return;
   const int cell = (int)c64;
   const size_t N = (size_t)p.ncell;

@@ -29,7 +29,8 @@ class Geometry:
     def reference_length(self) -> float:
         raise NotImplementedError
 
-    def sdf_field(self, Ny: int, Nx: int) -> Optional[np.ndarray]:
+    def sdf_field(self, Ny: int, Nx: int, offset: float = 0.5) -> Optional[np.ndarray]:
+        """SDF at cell coordinates + offset (the reference samples +0.5)."""
         return None
 
 
@@ -50,9 +51,9 @@ class Cylinder(Geometry):
     def reference_length(self):
         return 2.0 * self.radius
 
-    def sdf_field(self, Ny, Nx):
+    def sdf_field(self, Ny, Nx, offset=0.5):
         cx, cy = self.center(Ny, Nx)
-        xx, yy = np.meshgrid(np.arange(Nx, dtype=np.float64) + 0.5, np.arange(Ny, dtype=np.float64) + 0.5)
+        xx, yy = np.meshgrid(np.arange(Nx, dtype=np.float64) + offset, np.arange(Ny, dtype=np.float64) + offset)
         return np.sqrt((xx - cx) ** 2 + (yy - cy) ** 2) - self.radius
 
 
@@ -73,9 +74,9 @@ class Rectangle(Geometry):
     def reference_length(self):
         return self.height
 
-    def sdf_field(self, Ny, Nx):
+    def sdf_field(self, Ny, Nx, offset=0.5):
         cx, cy = self.center(Ny, Nx)
-        xx, yy = np.meshgrid(np.arange(Nx, dtype=np.float64) + 0.5, np.arange(Ny, dtype=np.float64) + 0.5)
+        xx, yy = np.meshgrid(np.arange(Nx, dtype=np.float64) + offset, np.arange(Ny, dtype=np.float64) + offset)
         dx = np.abs(xx - cx) - self.width / 2.0
         dy = np.abs(yy - cy) - self.height / 2.0
         return np.sqrt(np.maximum(dx, 0.0) ** 2 + np.maximum(dy, 0.0) ** 2) + np.minimum(np.maximum(dx, dy), 0.0)
@@ -125,7 +126,8 @@ class Geometry3D:
     def reference_length(self) -> float:
         raise NotImplementedError
 
-    def sdf_field(self, Nz: int, Ny: int, Nx: int) -> Optional[np.ndarray]:
+    def sdf_field(self, Nz: int, Ny: int, Nx: int, offset: float = 0.5) -> Optional[np.ndarray]:
+        """SDF at cell coordinates + offset (the reference samples +0.5)."""
         return None
 
 
@@ -153,9 +155,9 @@ class Sphere(Geometry3D):
     def reference_length(self):
         return 2.0 * self.radius
 
-    def sdf_field(self, Nz, Ny, Nx):
+    def sdf_field(self, Nz, Ny, Nx, offset=0.5):
         cx, cy, cz = self.center(Nz, Ny, Nx)
-        zz, yy, xx = _grid3(Nz, Ny, Nx, 0.5)
+        zz, yy, xx = _grid3(Nz, Ny, Nx, offset)
         return np.sqrt((xx - cx) ** 2 + (yy - cy) ** 2 + (zz - cz) ** 2) - self.radius
 
 
@@ -179,9 +181,9 @@ class Box(Geometry3D):
     def reference_length(self):
         return self.height
 
-    def sdf_field(self, Nz, Ny, Nx):
+    def sdf_field(self, Nz, Ny, Nx, offset=0.5):
         cx, cy, cz = self.center(Nz, Ny, Nx)
-        zz, yy, xx = _grid3(Nz, Ny, Nx, 0.5)
+        zz, yy, xx = _grid3(Nz, Ny, Nx, offset)
         dx = np.abs(xx - cx) - self.width / 2.0
         dy = np.abs(yy - cy) - self.height / 2.0
         dz = np.abs(zz - cz) - self.depth / 2.0
@@ -208,9 +210,9 @@ class Cylinder3D(Geometry3D):
     def reference_length(self):
         return 2.0 * self.radius
 
-    def sdf_field(self, Nz, Ny, Nx):
+    def sdf_field(self, Nz, Ny, Nx, offset=0.5):
         cx, cy, cz = self.center(Nz, Ny, Nx)
-        zz, yy, xx = _grid3(Nz, Ny, Nx, 0.5)
+        zz, yy, xx = _grid3(Nz, Ny, Nx, offset)
         dr = np.sqrt((xx - cx) ** 2 + (yy - cy) ** 2) - self.radius
         dz = np.abs(zz - cz) - self.length / 2.0
         return np.sqrt(np.maximum(dr, 0.0) ** 2 + np.maximum(dz, 0.0) ** 2) + np.minimum(np.maximum(dr, dz), 0.0)

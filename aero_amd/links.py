@@ -69,3 +69,33 @@ def build_surface_links_3d(solid: np.ndarray, phi: Optional[np.ndarray] = None):
     solid = np.asarray(solid, dtype=bool)
     links = _links(solid, E3)
     return links, _q((links[:, 1], links[:, 2], links[:, 3]), len(links), phi)
+
+
+def link_fractions(links: np.ndarray, phi: np.ndarray) -> np.ndarray:
+    """Wall position along each link for the linear Bouzidi rule.
+
+    ``phi`` is a signed distance sampled at the integer cell coordinates the
+    solid mask is tested on (negative inside).  Along link (i, cell) the wall
+    sits where the linear interpolant of phi between the fluid cell and its
+    solid neighbour crosses zero: q = phi(x) / (phi(x) - phi(x + e_i)),
+    clipped to [1e-4, 1].  (The reference instead takes q = phi at the fluid
+    cell of a +0.5-offset SDF, aero/lbm/boundary.py:273-278.)
+    """
+    E_ = E3 if links.shape[1] == 4 else E
+    ph = phi if phi.ndim == 3 else phi[None]
+    nz, ny, nx = ph.shape
+    if links.shape[1] == 4:
+        i, z, y, x = (links[:, k] for k in range(4))
+        ez = E_[i, 2].astype(int)
+    else:
+        i, y, x = (links[:, k] for k in range(3))
+        z = np.zeros_like(i)
+        ez = np.zeros(len(i), int)
+    zn = np.clip(z + ez, 0, nz - 1)
+    yn = np.clip(y + E_[i, 1].astype(int), 0, ny - 1)
+    xn = np.clip(x + E_[i, 0].astype(int), 0, nx - 1)
+    a = ph[z, y, x]
+    b = ph[zn, yn, xn]
+    den = np.where(a - b != 0.0, a - b, 1.0)
+    q = np.where(a - b != 0.0, a / den, 0.5)
+    return np.clip(q, 1e-4, 1.0).astype(np.float32)

@@ -20,7 +20,7 @@ from ._base import DeviceSolver, resolve_backend, run_loop, warn_hdf5
 from .diagnostics import coefficient_spectrum, detect_statistical_stationarity
 from .inflow import SyntheticInflow2D
 from .lattice import M9_inv, build_s_vec, compute_feq, base_nu_from_omega
-from .links import build_surface_links
+from .links import build_surface_links, link_fractions
 
 
 class Solver(DeviceSolver):
@@ -39,7 +39,8 @@ class Solver(DeviceSolver):
                  synthetic_inflow_intensity: float = 0.03, synthetic_inflow_seed: int = 12345,
                  thermal: bool = False, T_hot: float = 1.0, T_cold: float = 0.0, alpha_T: float = 1e-3,
                  buoyancy: bool = False, g_gravity: float = 0.0, beta: float = 1e-3, T_ref: float = 0.5,
-                 storage: str = "f64", trt_variant: str = "reference"):
+                 storage: str = "f64", trt_variant: str = "reference",
+                 bouzidi_variant: str = "reference"):
         """Arguments as the reference (solver.py:75-142), plus:
 
         storage     -- population format on the device: "f64" (reference
@@ -48,7 +49,12 @@ class Solver(DeviceSolver):
         trt_variant -- "reference" uses the reference's TRT rates
                        (trt2d.py:8-18), which are unstable for tau below ~0.76;
                        "lambda" uses the standard magic-parameter TRT
-                       (tau+ = tau, tau- = trt_lambda/(tau - 1/2) + 1/2)."""
+                       (tau+ = tau, tau- = trt_lambda/(tau - 1/2) + 1/2).
+        bouzidi_variant -- "reference" keeps the reference's interpolated
+                       bounce-back (q = phi at the fluid cell, boundary.py:303-352);
+                       "linear" is the Bouzidi-Firdaouss-Lallemand rule with q the
+                       link fraction from ``phi`` sampled at the integer cell
+                       coordinates (e.g. ``geometry.sdf_field(..., offset=0.0)``)."""
         self.Ny, self.Nx = int(Ny), int(Nx)
         self.solid = np.ascontiguousarray(solid, dtype=np.bool_)
         if self.solid.shape != (self.Ny, self.Nx):
@@ -90,6 +96,9 @@ class Solver(DeviceSolver):
         if trt_variant not in ("reference", "lambda"):
             raise ValueError("trt_variant must be 'reference' or 'lambda'.")
         self.trt_variant = trt_variant
+        if bouzidi_variant not in ("reference", "linear"):
+            raise ValueError("bouzidi_variant must be 'reference' or 'linear'.")
+        self.bouzidi_variant = bouzidi_variant
         self._raise_on_step = None
         if self.les and self.les_model == "wale" and self.van_driest and phi is not None:
             # the reference's WALE loop cannot take a damped (per-cell) constant
@@ -98,6 +107,8 @@ class Solver(DeviceSolver):
 
         _phi_q = phi if (bouzidi and phi is not None) else None
         self.surface_links, self.q_vals = build_surface_links(self.solid, _phi_q)
+        if bouzidi and bouzidi_variant == "linear" and phi is not None:
+            self.q_vals = link_fractions(self.surface_links, phi)
 
         ys, xs = np.where(self.solid)
         self._ref_center_y = float(np.mean(ys)) if ys.size else 0.5 * (Ny - 1)
@@ -117,7 +128,7 @@ class Solver(DeviceSolver):
                     "zerogradient": _abi.OUT_ZEROGRAD}.get(outlet_bc, _abi.OUT_NONE)
         d.wall = {"slip": _abi.WALL_SLIP, "noslip": _abi.WALL_NOSLIP,
                   "moving": _abi.WALL_MOVING}.get(wall_bc, _abi.WALL_NONE)
-        d.bouzidi = int(self.bouzidi)
+        d.bouzidi = 0 if not self.bouzidi else (2 if self.bouzidi_variant == "linear" else 1)
         d.trt_variant = 1 if self.trt_variant == "lambda" else 0
         d.sponge_thickness = self.sponge_thickness
         d.les = (_abi.LES_OFF if not self.les else
@@ -274,7 +285,8 @@ class Solver(DeviceSolver):
             synthetic_inflow=self.synthetic_inflow, synthetic_inflow_intensity=self.synthetic_inflow_intensity,
             thermal=self.thermal, T_hot=self.T_hot, T_cold=self.T_cold, alpha_T=self.alpha_T,
             buoyancy=self.buoyancy, g_gravity=self.g_gravity, beta=self.beta, T_ref=self.T_ref,
-            collision=self.collision, backend=self.backend, storage=self.storage, trt_variant=self.trt_variant)
+            collision=self.collision, backend=self.backend, storage=self.storage, trt_variant=self.trt_variant,
+            bouzidi_variant=self.bouzidi_variant)
 
     def save_checkpoint(self, path: str) -> None:
         """Same keys and layout as the reference (solver.py:638-686)."""

@@ -19,7 +19,7 @@ from ._base import DeviceSolver, resolve_backend, run_loop, warn_hdf5
 from .diagnostics import coefficient_spectrum, detect_statistical_stationarity
 from .inflow import SEMInlet, SyntheticInflow3D
 from .lattice import M19_inv, W3, build_s3_vec, compute_feq_3d, base_nu_from_omega
-from .links import build_surface_links_3d
+from .links import build_surface_links_3d, link_fractions
 
 
 class Solver3D(DeviceSolver):
@@ -39,9 +39,10 @@ class Solver3D(DeviceSolver):
                  synthetic_inflow_seed: int = 12345, sem_inlet: bool = False, sem_Tu: float = 0.05,
                  sem_L_int: float = 10.0, sem_N: int = 200, thermal: bool = False, T_hot: float = 1.0,
                  T_cold: float = 0.0, alpha_T: float = 1e-3, buoyancy: bool = False, g_gravity: float = 0.0,
-                 beta: float = 1e-3, T_ref: float = 0.5, storage: str = "f64", trt_variant: str = "reference"):
-        """Arguments as the reference (solver3d.py:71-135) plus ``storage`` and
-        ``trt_variant`` (see Solver)."""
+                 beta: float = 1e-3, T_ref: float = 0.5, storage: str = "f64", trt_variant: str = "reference",
+                 bouzidi_variant: str = "reference"):
+        """Arguments as the reference (solver3d.py:71-135) plus ``storage``,
+        ``trt_variant`` and ``bouzidi_variant`` (see Solver)."""
         self.Nz, self.Ny, self.Nx = int(Nz), int(Ny), int(Nx)
         self.solid = np.ascontiguousarray(solid, dtype=np.bool_)
         if self.solid.shape != (self.Nz, self.Ny, self.Nx):
@@ -90,12 +91,17 @@ class Solver3D(DeviceSolver):
         if trt_variant not in ("reference", "lambda"):
             raise ValueError("trt_variant must be 'reference' or 'lambda'.")
         self.trt_variant = trt_variant
+        if bouzidi_variant not in ("reference", "linear"):
+            raise ValueError("bouzidi_variant must be 'reference' or 'linear'.")
+        self.bouzidi_variant = bouzidi_variant
         self._raise_on_step = None
         if self.les and self.les_model == "wale" and self.van_driest and phi is not None:
             self._raise_on_step = ValueError("setting an array element with a sequence.")
 
         _phi_q = phi if (bouzidi and phi is not None) else None
         self.surface_links, self.q_vals = build_surface_links_3d(self.solid, _phi_q)
+        if bouzidi and bouzidi_variant == "linear" and phi is not None:
+            self.q_vals = link_fractions(self.surface_links, phi)
         zs, ys, xs = np.where(self.solid)
         self._ref_center_z = float(np.mean(zs)) if zs.size else 0.5 * (Nz - 1)
         self._ref_center_y = float(np.mean(ys)) if ys.size else 0.5 * (Ny - 1)
@@ -115,7 +121,7 @@ class Solver3D(DeviceSolver):
             d.inlet = _abi.IN_RECYCLING
         d.wall = {"slip": _abi.WALL_SLIP, "noslip": _abi.WALL_NOSLIP,
                   "moving": _abi.WALL_MOVING}.get(wall_bc, _abi.WALL_NONE)
-        d.bouzidi = int(self.bouzidi)
+        d.bouzidi = 0 if not self.bouzidi else (2 if self.bouzidi_variant == "linear" else 1)
         d.trt_variant = 1 if self.trt_variant == "lambda" else 0
         d.sponge_thickness = self.sponge_thickness
         d.les = (_abi.LES_OFF if not self.les else
@@ -294,7 +300,8 @@ class Solver3D(DeviceSolver):
             synthetic_inflow=self.synthetic_inflow, synthetic_inflow_intensity=self.synthetic_inflow_intensity,
             thermal=self.thermal, T_hot=self.T_hot, T_cold=self.T_cold, alpha_T=self.alpha_T,
             buoyancy=self.buoyancy, g_gravity=self.g_gravity, beta=self.beta, T_ref=self.T_ref,
-            storage=self.storage, trt_variant=self.trt_variant)
+            storage=self.storage, trt_variant=self.trt_variant,
+            bouzidi_variant=self.bouzidi_variant)
 
     def save_checkpoint(self, path: str) -> None:
         """Same keys and layout as the reference (solver3d.py:727-783)."""

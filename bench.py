@@ -5,8 +5,10 @@ Workload (one step = one lattice update of every cell, solid included):
 (32, 32, 32), Bouzidi links, u0 = 0.05, Re = 100 (tau = 0.524), slip walls,
 convective outlet, equilibrium start plus seeded 1e-4 transverse velocity
 noise, float32 population storage.  At tau = 0.524 the reference's TRT rates
-are linearly unstable (s- = -18), so the timed run uses the standard
-magic-parameter TRT (Lambda = 3/16), labelled "trt_variant": "lambda".
+are linearly unstable (s- = -18) and its interpolated bounce-back diverges
+within ~100 steps even in float64, so the timed run uses the standard
+magic-parameter TRT (Lambda = 3/16) and the linear Bouzidi rule with link
+fractions, labelled "trt_variant": "lambda", "bouzidi_variant": "linear".
 
 Prints ONE JSON line.  ``value`` is whole-job MLUPS over the timed steps
 (all launches included, populations resident in HBM).  ``roofline`` is the
@@ -38,12 +40,13 @@ def make_case(nx, ny, nz, radius, cx, storage, steps_hint=0, seed=0):
     from aero_amd.geometry import Sphere
     from aero_amd.lattice import compute_feq_3d
     g = Sphere(radius=radius, cx_frac=cx / nx, cy_frac=0.5, cz_frac=0.5)
-    solid, phi = g.mark_solid(nz, ny, nx), g.sdf_field(nz, ny, nx)
+    solid, phi = g.mark_solid(nz, ny, nx), g.sdf_field(nz, ny, nx, offset=0.0)
     D = 2.0 * radius
     u0, re = 0.05, 100.0
     omega = 1.0 / (3.0 * u0 * D / re + 0.5)
     s = aero_amd.Solver3D(Nz=nz, Ny=ny, Nx=nx, solid=solid, omega=omega, u0=u0, D=D, phi=phi, bouzidi=True,
-                          collision="trt", trt_lambda=0.1875, trt_variant="lambda", storage=storage)
+                          collision="trt", trt_lambda=0.1875, trt_variant="lambda", bouzidi_variant="linear",
+                          storage=storage)
     rng = np.random.default_rng(seed)
     sh = (nz, ny, nx)
     f0 = compute_feq_3d(np.ones(sh), np.full(sh, u0), rng.uniform(-1e-4, 1e-4, sh), rng.uniform(-1e-4, 1e-4, sh))
@@ -68,11 +71,20 @@ def timed(solver, steps, warmup):
     return t1 - t0, ms.value / max(n.value, 1), rows
 
 
+def cell_counts(solver):
+    """(cells, link cells) of the solver's domain."""
+    import ctypes
+    tot, links = ctypes.c_int64(), ctypes.c_int64()
+    solver._ctx.call("aero_cell_counts", ctypes.byref(tot), ctypes.byref(links))
+    return int(tot.value), int(links.value)
+
+
 def cpu_baseline(case, nx, ny, nz, budget_s=15.0):
     """Float64 C oracle (OpenMP) on the same workload, full reference steps."""
     from oracle import oracle as orc
     o = orc.OracleSolver(case["solid"], omega=case["omega"], u0=case["u0"], D=case["D"], phi=case["phi"],
-                         bouzidi=True, collision="trt", trt_lambda=0.1875, trt_variant="lambda")
+                         bouzidi=True, collision="trt", trt_lambda=0.1875, trt_variant="lambda",
+                         bouzidi_variant="linear")
     o.set_state(case["f0"])
     o.step()  # warm-up
     t0 = time.perf_counter()
@@ -131,7 +143,8 @@ def main():
     if not np.all(np.isfinite(rows[:, 0])):
         raise RuntimeError("non-finite drag in the timed run")
     mlups = cells * a.steps / wall / 1e6
-    bytes_per_launch = 2 * 19 * 4 * cells
+    ncell, nlink = cell_counts(s)
+    bytes_per_launch = 2 * 19 * 4 * ncell
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
     out = {
         "metric": "MLUPS + %HBM roofline, D3Q19 FP32 vs mixed",
@@ -148,12 +161,14 @@ def main():
         "data": "synthetic: analytic sphere, equilibrium + seeded 1e-4 noise",
         "config": {"workload": "D3Q19 TRT sphere 128x64x64 r=8 Bouzidi, u0=0.05 Re=100, fp32 storage",
                    "nx": nx, "ny": ny, "nz": nz, "collision": "trt", "trt_variant": "lambda",
+                   "bouzidi_variant": "linear",
                    "trt_lambda": 0.1875, "storage": "f32", "links": int(len(s.surface_links))},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles("step_d3q19_trt_f32_128x64x64"),
                      "kernel": "aero::step_kernel<D3Q19, TRT, F32>", "kernel_ms": round(kern_ms, 5),
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "algorithmic_bytes_per_launch": bytes_per_launch, "cells": ncell, "link_cells": nlink,
+                     "bytes_per_cell": 2 * 19 * 4},
         "cd_last": float(rows[-1, 0]),
     }
     if not a.no_extra:
@@ -173,7 +188,7 @@ def main():
             n3 = 40
             w3, k3, r3 = timed(s3, n3, 10)
             c3 = 512 * 256 * 256
-            bpl = 2 * 19 * (4 if st == "f32" else 2) * c3
+            bpl = 2 * 19 * (4 if st == "f32" else 2) * 512 * 256 * 256
             extra[f"{st}_512x256x256"] = {"MLUPS": round(c3 * n3 / w3 / 1e6, 2), "kernel_ms": round(k3, 4),
                                           "roofline_frac": round(bpl / (k3 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
             s3.close()

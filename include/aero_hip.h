@@ -83,7 +83,8 @@ typedef struct aero_desc {
   int32_t inlet;
   int32_t outlet;
   int32_t wall;
-  int32_t bouzidi;
+  int32_t bouzidi;     /* 0: mid-link bounce-back; 1: the reference's Bouzidi rule
+                          (boundary.py:303-352); 2: linear Bouzidi-Firdaouss-Lallemand */
   int32_t sponge_thickness;
   int32_t les;
   int32_t thermal;
@@ -147,10 +148,14 @@ int aero_synchronize(aero_ctx* ctx);
 /* Bytes of device memory held by the context. */
 int aero_device_bytes(aero_ctx* ctx, int64_t* bytes);
 /* Kernel timing with HIP events on the context's stream: when enabled,
- * aero_step brackets every step-kernel launch with events; aero_get_timing
- * returns the summed kernel milliseconds and the launch count since enabling. */
+ * aero_step brackets every step kernel launch (one per step) with events
+ * recorded at the kernel's start and end; aero_get_timing returns the summed
+ * kernel milliseconds and the launch count since enabling.  No reference
+ * counterpart (the reference times whole steps from the host). */
 int aero_set_timing(aero_ctx* ctx, int32_t enabled);
 int aero_get_timing(aero_ctx* ctx, double* kernel_ms, int64_t* launches);
+/* Cell counts: total cells and link cells (fluid cells with a surface link). */
+int aero_cell_counts(aero_ctx* ctx, int64_t* total, int64_t* link_cells);
 
 #ifdef __cplusplus
 }

