@@ -30,6 +30,10 @@ FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     # packed FP32 instructions are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # Do not merge stores from the two sides of a branch into one store to a
+    # selected address: on the per-cell population arrays that turns a
+    # constant index into a runtime one and moves the array to scratch memory.
+    "-mllvm", "-simplifycfg-sink-common=false",
     "-Wno-unused-parameter", "-Wno-unused-command-line-argument",
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]

@@ -354,14 +354,13 @@ hipError_t launch_check_f64(int q, int nx, int ny, int nz, int zc, const double*
 
 // ------------------------------------------------------------- reductions
 // Momentum-exchange totals of one step in the same fixed order the step
-// kernel's block 0 uses (reduce_forces_block), for the last step of a call.
-__global__ __launch_bounds__(BLOCK) void reduce_forces_kernel(const double* fpart, int nrec, double fd,
-                                                                      double md, double* row) {
-  reduce_forces_block(fpart, nrec, fd, md, row);
-}
+// kernel's block 0 uses (reduce_forces_block), for the last step of a chunk.
+template <class L>
+__global__ __launch_bounds__(BLOCK) void reduce_forces_kernel(ForceReduce r) { reduce_forces_block<L>(r); }
 
-hipError_t launch_reduce_forces(const double* fpart, int nrec, double fd, double md, double* row, hipStream_t s) {
-  reduce_forces_kernel<<<1, BLOCK, 0, s>>>(fpart, nrec, fd, md, row);
+hipError_t launch_reduce_forces(const ForceReduce& r, hipStream_t s) {
+  if (r.q == 9) reduce_forces_kernel<D2Q9><<<1, BLOCK, 0, s>>>(r);
+  else reduce_forces_kernel<D3Q19><<<1, BLOCK, 0, s>>>(r);
   return hipGetLastError();
 }
 

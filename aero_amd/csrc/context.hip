@@ -9,7 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/aero_hip.h"
+#include "aero_hip.h"
 #include "launch.h"
 #include "step_impl.h"
 
@@ -39,6 +39,9 @@ int guarded(F&& fn) {
     return 1;
   }
 }
+
+// Steps per chunk of aero_step (device history rows and events are reused per chunk).
+constexpr int STEP_CHUNK = 256;
 
 struct DevBuf {
   void* p = nullptr;
@@ -95,12 +98,18 @@ struct aero_ctx {
   int side = 0;
   int prev_side = 0;
   // geometry
-  DevBuf flags, lrec, row_start, lq, fpart[2], sigma, minv_d, minv_f, damp;
+  DevBuf flags, lrec, rec_of, qdir, fpart[2], sigma, minv_d, minv_f, damp;
   int fpart_side = 0;  // buffer holding the last completed step's per-link-cell force sums
   int nrec = 0;
   std::vector<int> rec_cells;
+  std::vector<uint32_t> rec_masks;
   // per-step inputs
-  DevBuf planes, recyc, hist;
+  // per-step inputs and outputs, in chunks of at most STEP_CHUNK steps:
+  // double-buffered device coefficient rows and pinned staging, so the
+  // device never waits on the host between chunks and memory stays bounded
+  DevBuf planes[2], recyc, hist_dev[2], last_plane_buf;
+  double* hist_stage[2] = {nullptr, nullptr};
+  hipEvent_t chunk_ev[2] = {nullptr, nullptr};
   const double* last_plane = nullptr;
   long long last_step_no = 0;
   int last_g_skip = 0;
@@ -119,6 +128,9 @@ struct aero_ctx {
   long long timed_launches = 0;
 
   bool les_split() const { return d.les != AERO_LES_OFF && d.collision != AERO_MRT; }
+  bool need_planes() const { return d.inlet == AERO_IN_FIELD || d.inlet == AERO_IN_SEM; }
+  // inlet planes are 3 x nrow doubles per step: smaller chunks bound their buffers
+  int chunk_steps() const { return need_planes() ? STEP_CHUNK / 4 : STEP_CHUNK; }
   bool convective() const { return d.outlet == AERO_OUT_CONVECTIVE; }
 
   void set_device() const { ck(hipSetDevice(device), "hipSetDevice"); }
@@ -134,8 +146,8 @@ struct aero_ctx {
     p.nblocks = nblocks;
     p.flags = flags.as<uint8_t>();
     p.lrec = lrec.as<LinkCell>();
-    p.row_start = row_start.as<int>();
-    p.lq = lq.as<float>();
+    p.rec_of = rec_of.as<int>();
+    p.qdir = qdir.as<float>();
     p.fpart = fpart[fpart_side].as<double>();
     p.nrec = nrec;
     p.fdenom = force_denom();
@@ -326,6 +338,8 @@ int aero_create(const aero_desc* desc, aero_ctx** out) {
     if (!(d.omega > 0.0)) throw std::invalid_argument("omega must be positive");
     const long long N = (long long)d.nx * d.ny * d.nz;
     if (N * d.q >= (1ll << 31)) throw std::invalid_argument("grid too large for 32-bit cell indexing");
+    // kernels address one direction plane with a 32-bit byte offset
+    if (N * 8 >= (1ll << 32)) throw std::invalid_argument("grid too large: one population plane must stay under 4 GiB");
     c->Q = d.q;
     c->N = N;
     c->nrow = d.nz * d.ny;
@@ -364,6 +378,16 @@ int aero_create(const aero_desc* desc, aero_ctx** out) {
     ck(hipMemcpy(c->minv_d.p, d.mrt_minv, sizeof(double) * 361, hipMemcpyHostToDevice), "copy minv");
     ck(hipMemcpy(c->minv_f.p, mf.data(), sizeof(float) * 361, hipMemcpyHostToDevice), "copy minv");
     c->recyc.alloc(sizeof(double) * (4 * (size_t)c->nrow + 1));
+    for (int b = 0; b < 2; ++b) {
+      c->hist_dev[b].alloc(sizeof(double) * NCOEF * STEP_CHUNK);
+      ck(hipHostMalloc((void**)&c->hist_stage[b], sizeof(double) * NCOEF * STEP_CHUNK, hipHostMallocDefault),
+         "hipHostMalloc");
+      ck(hipEventCreateWithFlags(&c->chunk_ev[b], hipEventDisableTiming), "hipEventCreate");
+    }
+    if (d.inlet == AERO_IN_FIELD || d.inlet == AERO_IN_SEM) {
+      for (int b = 0; b < 2; ++b) c->planes[b].alloc(sizeof(double) * 3 * (size_t)c->nrow * c->chunk_steps());
+      c->last_plane_buf.alloc(sizeof(double) * 3 * (size_t)c->nrow);
+    }
     if (c->les_split()) {
       c->fsplit.alloc(c->csz * d.q * (size_t)N);
       c->usplit.alloc(c->csz * 3 * (size_t)N);
@@ -385,11 +409,15 @@ int aero_destroy(aero_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags,
-                    &c->lrec, &c->row_start, &c->lq, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
-                    &c->damp, &c->planes, &c->recyc, &c->hist, &c->fsplit, &c->usplit, &c->fcache,
+                    &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
+                    &c->damp, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
                     &c->gcache, &c->tmp, &c->checkpart, &c->checkout};
   for (DevBuf* b : bufs) b->release();
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  for (int b = 0; b < 2; ++b) {
+    if (c->hist_stage[b]) (void)hipHostFree(c->hist_stage[b]);
+    if (c->chunk_ev[b]) (void)hipEventDestroy(c->chunk_ev[b]);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -403,11 +431,11 @@ static void set_geometry_impl(aero_ctx* c, const uint8_t* solid, int64_t nlinks,
     c->set_device();
     const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
     c->flags.alloc((size_t)c->N);
-    c->row_start.alloc(sizeof(int) * (c->nrow + 1));
     std::vector<uint8_t> fl((size_t)c->N);
     for (long long k = 0; k < c->N; ++k) fl[k] = solid[k] ? FLAG_SOLID : 0;
-    // group links by cell: mask + q per direction
-    std::map<int, std::pair<uint32_t, std::vector<std::pair<int, float>>>> cells;
+    // group links by cell: direction mask + q per direction
+    const int Q1 = c->Q - 1;
+    std::map<int, std::pair<uint32_t, std::vector<float>>> cells;
     for (int64_t k = 0; k < nlinks; ++k) {
       const int i = links[4 * k], z = links[4 * k + 1], y = links[4 * k + 2], x = links[4 * k + 3];
       if (i < 1 || i >= c->Q || x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz)
@@ -416,37 +444,40 @@ static void set_geometry_impl(aero_ctx* c, const uint8_t* solid, int64_t nlinks,
       if (solid[cell]) throw std::invalid_argument("link origin must be a fluid cell");
       auto& e = cells[cell];
       if (e.first & (1u << i)) throw std::invalid_argument("duplicate link");
+      if (e.second.empty()) e.second.assign(Q1, 0.0f);
       e.first |= 1u << i;
-      e.second.push_back({i, q ? q[k] : 0.5f});
+      e.second[i - 1] = q ? q[k] : 0.5f;
     }
     std::vector<LinkCell> recs;
     std::vector<float> qv;
-    std::vector<int> rs(c->nrow + 1, 0);
+    std::vector<int> rof((size_t)c->N, -1);
     recs.reserve(cells.size());
+    qv.reserve(cells.size() * Q1);
     for (auto& kv : cells) {
-      auto& lst = kv.second.second;
-      std::sort(lst.begin(), lst.end());
-      LinkCell r{kv.first, kv.second.first, (int)qv.size(), 0};
-      for (auto& iq : lst) qv.push_back(iq.second);
-      recs.push_back(r);
+      rof[kv.first] = (int)recs.size();
+      recs.push_back(LinkCell{kv.first, kv.second.first});
+      qv.insert(qv.end(), kv.second.second.begin(), kv.second.second.end());
       fl[kv.first] |= FLAG_LINKS;
-      rs[kv.first / nx + 1] += 1;
     }
-    for (int r = 0; r < c->nrow; ++r) rs[r + 1] += rs[r];
     c->nrec = (int)recs.size();
     c->rec_cells.clear();
-    for (auto& r : recs) c->rec_cells.push_back(r.cell);
+    c->rec_masks.clear();
+    for (auto& r : recs) {
+      c->rec_cells.push_back(r.cell);
+      c->rec_masks.push_back(r.mask);
+    }
     ck(hipMemcpy(c->flags.p, fl.data(), (size_t)c->N, hipMemcpyHostToDevice), "copy flags");
     c->lrec.alloc(sizeof(LinkCell) * std::max<size_t>(1, recs.size()));
-    c->lq.alloc(sizeof(float) * std::max<size_t>(1, qv.size()));
-    for (int k = 0; k < 2; ++k) c->fpart[k].alloc(sizeof(double) * NFORCE * std::max<size_t>(1, recs.size()));
+    c->qdir.alloc(sizeof(float) * std::max<size_t>(1, qv.size()));
+    c->rec_of.alloc(sizeof(int) * (size_t)c->N);
+    for (int k = 0; k < 2; ++k) c->fpart[k].alloc(sizeof(double) * NPART * std::max<size_t>(1, recs.size()));
     c->fpart_side = 0;
     if (!recs.empty()) {
       ck(hipMemcpy(c->lrec.p, recs.data(), sizeof(LinkCell) * recs.size(), hipMemcpyHostToDevice), "copy recs");
-      ck(hipMemcpy(c->lq.p, qv.data(), sizeof(float) * qv.size(), hipMemcpyHostToDevice), "copy q");
+      ck(hipMemcpy(c->qdir.p, qv.data(), sizeof(float) * qv.size(), hipMemcpyHostToDevice), "copy q");
     }
+    ck(hipMemcpy(c->rec_of.p, rof.data(), sizeof(int) * (size_t)c->N, hipMemcpyHostToDevice), "copy record index");
     for (int k = 0; k < 2; ++k) ck(hipMemset(c->fpart[k].p, 0, c->fpart[k].bytes), "memset fpart");
-    ck(hipMemcpy(c->row_start.p, rs.data(), sizeof(int) * (c->nrow + 1), hipMemcpyHostToDevice), "copy rows");
     c->invalidate();
 }
 
@@ -557,87 +588,107 @@ int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, 
     if (nsteps <= 0) return;
     if (!c->has_state) throw std::runtime_error("no state: call aero_set_f first");
     c->set_device();
-    const bool need_planes = c->d.inlet == AERO_IN_FIELD || c->d.inlet == AERO_IN_SEM;
+    const bool need_planes = c->need_planes();
     if (need_planes && !planes) throw std::invalid_argument("this inlet needs per-step planes");
     const size_t plane_cnt = 3 * (size_t)c->nrow;
-    if (need_planes) {
-      c->planes.alloc(sizeof(double) * plane_cnt * nsteps);
-      ck(hipMemcpyAsync(c->planes.p, planes, sizeof(double) * plane_cnt * nsteps, hipMemcpyHostToDevice,
-                        c->stream),
-         "upload planes");
-    }
-    c->hist.alloc(sizeof(double) * NCOEF * (size_t)nsteps);
-    if (c->timing) {
-      while (c->ev_pool.size() < 2 * (size_t)nsteps) {
-        hipEvent_t e;
-        ck(hipEventCreate(&e), "hipEventCreate");
-        c->ev_pool.push_back(e);
-      }
-    }
     const double fd = c->force_denom(), md = c->moment_denom();
-    for (int k = 0; k < nsteps; ++k) {
-      const long long step_no = step0 + k + 1;
-      const int rd = c->side, wr = c->side ^ 1;
-      if (c->d.inlet == AERO_IN_RECYCLING) c->recycling_prepass(rd);
-      StepParams p = c->base_params();
-      p.fin = c->B[rd].p;
-      p.fout = c->B[wr].p;
-      p.gin = c->G[rd].p;
-      p.gout = c->G[wr].p;
-      p.prev_in = c->prev[c->prev_side].as<double>();
-      p.prev_out = c->prev[c->prev_side ^ 1].as<double>();
-      p.plane = need_planes ? c->planes.as<double>() + plane_cnt * k : nullptr;
-      p.phase = 0.17 * (double)step_no;
-      p.g_bc_skip = c->g_skip_pending;
-      p.commit = 1;
-      const int kmode = c->les_split() ? 1 : 0;
-      if (kmode) {
-        p.fsplit = c->fsplit.p;
-        p.usplit = c->usplit.p;
+    const int kmode = c->les_split() ? 1 : 0;
+    const int CH = c->chunk_steps();
+    // rows of chunk b wait in hist_stage[b] until that buffer is reused
+    int pend_off[2] = {-1, -1}, pend_n[2] = {0, 0};
+    auto drain = [&](int b) {
+      if (pend_off[b] < 0) return;
+      ck(hipEventSynchronize(c->chunk_ev[b]), "chunk sync");
+      if (hist) std::memcpy(hist + (size_t)NCOEF * pend_off[b], c->hist_stage[b], sizeof(double) * NCOEF * pend_n[b]);
+      pend_off[b] = -1;
+    };
+    for (int s0 = 0, ci = 0; s0 < nsteps; s0 += CH, ++ci) {
+      const int n = std::min(CH, nsteps - s0);
+      const int b = ci & 1;
+      drain(b);
+      double* rows = c->hist_dev[b].as<double>();
+      if (need_planes)
+        ck(hipMemcpyAsync(c->planes[b].p, planes + plane_cnt * s0, sizeof(double) * plane_cnt * n,
+                          hipMemcpyHostToDevice, c->stream),
+           "upload planes");
+      for (int k = 0; k < n; ++k) {
+        const long long step_no = step0 + s0 + k + 1;
+        const int rd = c->side, wr = c->side ^ 1;
+        if (c->d.inlet == AERO_IN_RECYCLING) c->recycling_prepass(rd);
+        StepParams p = c->base_params();
+        p.fin = c->B[rd].p;
+        p.fout = c->B[wr].p;
+        p.gin = c->G[rd].p;
+        p.gout = c->G[wr].p;
+        p.prev_in = c->prev[c->prev_side].as<double>();
+        p.prev_out = c->prev[c->prev_side ^ 1].as<double>();
+        p.plane = need_planes ? c->planes[b].as<double>() + plane_cnt * k : nullptr;
+        p.phase = 0.17 * (double)step_no;
+        p.g_bc_skip = c->g_skip_pending;
+        p.commit = 1;
+        if (kmode) {
+          p.fsplit = c->fsplit.p;
+          p.usplit = c->usplit.p;
+        }
+        // force sums of this step go to the other buffer; the launch's block 0
+        // reduces the previous step's into that step's coefficient row
+        const int fs = c->fpart_side ^ 1;
+        p.fpart = c->fpart[fs].as<double>();
+        p.fpart_prev = c->fpart[c->fpart_side].as<double>();
+        p.reduce_prev = (k > 0 && c->nrec > 0) ? 1 : 0;
+        p.hist_prev = k > 0 ? rows + (size_t)NCOEF * (k - 1) : nullptr;
+        if (c->timing) {
+          p.ev_start = c->ev_pool[2 * k];
+          p.ev_stop = c->ev_pool[2 * k + 1];
+        }
+        c->launch_step_kernel(kmode, p);
+        if (kmode) c->collide_field(c->fsplit.p, c->usplit.p, wr);
+        c->fpart_side = fs;
+        c->last_plane = p.plane;
+        c->last_step_no = step_no;
+        c->last_g_skip = c->g_skip_pending;
+        c->g_skip_pending = 0;
+        c->side = wr;
+        if (c->convective()) c->prev_side ^= 1;
       }
-      // force sums of this step go to the other buffer; the launch's block 0
-      // reduces the previous step's into that step's coefficient row
-      const int fs = c->fpart_side ^ 1;
-      p.fpart = c->fpart[fs].as<double>();
-      p.fpart_prev = c->fpart[c->fpart_side].as<double>();
-      p.reduce_prev = (k > 0 && c->nrec > 0) ? 1 : 0;
-      p.hist_prev = k > 0 ? c->hist.as<double>() + (size_t)NCOEF * (k - 1) : nullptr;
+      // the chunk's last step is reduced here, so every chunk's rows are
+      // complete when its copy is issued
+      if (c->nrec > 0) {
+        const ForceReduce r{c->Q, c->fpart[c->fpart_side].as<double>(), c->lrec.as<LinkCell>(), c->nrec, c->d.nx, c->d.ny,
+                            c->d.center[0], c->d.center[1], c->d.center[2], fd, md, rows + (size_t)NCOEF * (n - 1)};
+        ck(launch_reduce_forces(r, c->stream), "force reduction");
+      } else {
+        ck(hipMemsetAsync(rows, 0, sizeof(double) * NCOEF * (size_t)n, c->stream), "zero history");
+      }
+      if (hist) {
+        ck(hipMemcpyAsync(c->hist_stage[b], rows, sizeof(double) * NCOEF * (size_t)n, hipMemcpyDeviceToHost,
+                          c->stream),
+           "download history");
+      }
+      ck(hipEventRecord(c->chunk_ev[b], c->stream), "chunk event");
+      pend_off[b] = s0;
+      pend_n[b] = n;
       if (c->timing) {
-        p.ev_start = c->ev_pool[2 * k];
-        p.ev_stop = c->ev_pool[2 * k + 1];
+        ck(hipEventSynchronize(c->ev_pool[2 * n - 1]), "timing sync");
+        for (int k = 0; k < n; ++k) {
+          float ms = 0.f;
+          ck(hipEventElapsedTime(&ms, c->ev_pool[2 * k], c->ev_pool[2 * k + 1]), "event time");
+          c->timed_ms += ms;
+        }
+        c->timed_launches += n;
       }
-      c->launch_step_kernel(kmode, p);
-      if (kmode) c->collide_field(c->fsplit.p, c->usplit.p, wr);
-      c->fpart_side = fs;
-      c->last_plane = p.plane;
-      c->last_step_no = step_no;
-      c->last_g_skip = c->g_skip_pending;
-      c->g_skip_pending = 0;
-      c->side = wr;
-      if (c->convective()) c->prev_side ^= 1;
     }
-    if (c->nrec > 0) {
-      ck(launch_reduce_forces(c->fpart[c->fpart_side].as<double>(), c->nrec, fd, md,
-                              c->hist.as<double>() + (size_t)NCOEF * (nsteps - 1), c->stream),
-         "force reduction");
-    } else {
-      ck(hipMemsetAsync(c->hist.p, 0, sizeof(double) * NCOEF * (size_t)nsteps, c->stream), "zero history");
+    // keep the last step's inlet plane for re-deriving its state later
+    if (need_planes) {
+      ck(hipMemcpyAsync(c->last_plane_buf.p, c->last_plane, sizeof(double) * plane_cnt, hipMemcpyDeviceToDevice,
+                        c->stream),
+         "keep last plane");
+      c->last_plane = c->last_plane_buf.as<double>();
     }
     c->invalidate();
-    if (hist) {
-      ck(hipMemcpyAsync(hist, c->hist.p, sizeof(double) * NCOEF * (size_t)nsteps, hipMemcpyDeviceToHost,
-                        c->stream),
-         "download history");
-    }
+    drain(0);
+    drain(1);
     ck(hipStreamSynchronize(c->stream), "step sync");
-    if (c->timing) {
-      for (int k = 0; k < nsteps; ++k) {
-        float ms = 0.f;
-        ck(hipEventElapsedTime(&ms, c->ev_pool[2 * k], c->ev_pool[2 * k + 1]), "event time");
-        c->timed_ms += ms;
-      }
-      c->timed_launches += nsteps;
-    }
   });
 }
 
@@ -645,6 +696,12 @@ int aero_set_timing(aero_ctx* c, int32_t enabled) {
   return guarded([&] {
     c->set_device();
     c->timing = enabled != 0;
+    // the events are created here, outside any timed stepping
+    while (c->timing && c->ev_pool.size() < 2 * (size_t)STEP_CHUNK) {
+      hipEvent_t e;
+      ck(hipEventCreate(&e), "hipEventCreate");
+      c->ev_pool.push_back(e);
+    }
     c->timed_ms = 0.0;
     c->timed_launches = 0;
   });
@@ -704,7 +761,19 @@ int aero_link_forces(aero_ctx* c, int32_t* cells, double* forces, int32_t* nrec)
     if (cells) std::memcpy(cells, c->rec_cells.data(), sizeof(int) * c->nrec);
     if (forces && c->nrec) {
       ck(hipStreamSynchronize(c->stream), "sync");
-      ck(hipMemcpy(forces, c->fpart[c->fpart_side].p, sizeof(double) * NFORCE * c->nrec, hipMemcpyDeviceToHost), "download");
+      std::vector<double> part((size_t)NPART * c->nrec);
+      ck(hipMemcpy(part.data(), c->fpart[c->fpart_side].p, sizeof(double) * part.size(), hipMemcpyDeviceToHost),
+         "download");
+      for (int k = 0; k < c->nrec; ++k) {
+        double* out = forces + (size_t)NFORCE * k;
+        const double* pk = part.data() + (size_t)NPART * k;
+        if (c->Q == 9)
+          cell_force_terms<D2Q9>(pk, c->rec_cells[k], c->rec_masks[k], c->d.nx, c->d.ny, c->d.center[0],
+                                 c->d.center[1], c->d.center[2], out);
+        else
+          cell_force_terms<D3Q19>(pk, c->rec_cells[k], c->rec_masks[k], c->d.nx, c->d.ny, c->d.center[0],
+                                  c->d.center[1], c->d.center[2], out);
+      }
     }
   });
 }
@@ -719,8 +788,8 @@ int aero_synchronize(aero_ctx* c) {
 int aero_device_bytes(aero_ctx* c, int64_t* bytes) {
   return guarded([&] {
     const DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags,
-                            &c->lrec, &c->row_start, &c->lq, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
-                            &c->damp, &c->planes, &c->recyc, &c->hist, &c->fsplit, &c->usplit, &c->fcache,
+                            &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
+                            &c->damp, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
                             &c->gcache, &c->tmp, &c->checkpart, &c->checkout};
     int64_t t = 0;
     for (const DevBuf* b : bufs) t += (int64_t)b->bytes;

@@ -23,7 +23,7 @@ hipError_t launch_macro_from_compute(int st, int q, long long n, const void* F, 
 hipError_t launch_macro4_f64(int q, long long n, const double* F, double* out, hipStream_t s);
 hipError_t launch_check_f64(int q, int nx, int ny, int nz, int zc, const double* F, int nblocks, double* part,
                             hipStream_t s);
-hipError_t launch_reduce_forces(const double* fpart, int nrec, double fd, double md, double* row, hipStream_t s);
+hipError_t launch_reduce_forces(const ForceReduce& r, hipStream_t s);
 hipError_t launch_reduce_check(const double* part, int nblocks, double* out, hipStream_t s);
 hipError_t launch_reduce_recycling(double* recyc, int nrow, double target_u0, hipStream_t s);
 

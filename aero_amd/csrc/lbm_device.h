@@ -100,46 +100,93 @@ struct D3Q19 {
   static constexpr int MOVE_SGN_TOP[5] = {0, -1, 1, 0, 0};
 };
 
+// acc + e*v for a lattice component e in {-1, 0, 1}: with e a constant of an
+// unrolled loop the zero terms vanish at compile time (0*v cannot be folded
+// in IEEE arithmetic, so writing e*v would keep a multiply-add per zero).
+template <class T>
+__host__ __device__ __forceinline__ T madd(T acc, int e, T v) {
+  return e == 0 ? acc : (e == 1 ? acc + v : (e == -1 ? acc - v : acc + T(e) * v));
+}
+template <class L, class T>
+__host__ __device__ __forceinline__ T edot(int i, T ux, T uy, T uz) {
+  return madd(madd(madd(T(0), L::EX[i], ux), L::EY[i], uy), L::EZ[i], uz);
+}
+
+// Direction components for a runtime index (loops over the set bits of a
+// link mask): two bits per direction, value + 1.
+template <class L>
+struct DirPack {
+  static constexpr unsigned long long pack(const int* e) {
+    unsigned long long v = 0;
+    for (int i = 0; i < L::Q; ++i) v |= (unsigned long long)(e[i] + 1) << (2 * i);
+    return v;
+  }
+  static constexpr unsigned long long PX = pack(L::EX), PY = pack(L::EY), PZ = pack(L::EZ);
+  static __host__ __device__ __forceinline__ int ex(int i) { return (int)((PX >> (2 * i)) & 3ull) - 1; }
+  static __host__ __device__ __forceinline__ int ey(int i) { return (int)((PY >> (2 * i)) & 3ull) - 1; }
+  static __host__ __device__ __forceinline__ int ez(int i) { return (int)((PZ >> (2 * i)) & 3ull) - 1; }
+  // weights by shell: rest, axis (|e| = 1), diagonal
+  static __host__ __device__ __forceinline__ double w(int i) {
+    return i == 0 ? L::W[0] : (i <= 2 * L::DIM ? L::W[1] : L::W[L::Q - 1]);
+  }
+};
+
 // ------------------------------------------------------------------ storage
 // Populations are stored as T and computed in C.  The half forms store the
 // shifted value f - w_i*s (s = 1 for f, T_ref for the scalar g) so the 10/7
 // mantissa bits resolve the deviation from rest equilibrium, not its bulk.
 template <int ST> struct Storage;
+// ld/st address one direction plane with a 32-bit byte offset (the host
+// keeps every plane under 4 GiB), so the compiler forms scalar-base +
+// 32-bit-offset addresses: one VGPR per access instead of a 64-bit pair.
+#define AERO_PLANE_ACCESS                                                                    \
+  static __device__ __forceinline__ C ld(const T* plane, unsigned k, C s) {                 \
+    return load((const T*)((const char*)plane + k * (unsigned)sizeof(T)), 0, s);            \
+  }                                                                                          \
+  static __device__ __forceinline__ void st(T* plane, unsigned k, C v, C s) {               \
+    store((T*)((char*)plane + k * (unsigned)sizeof(T)), 0, v, s);                           \
+  }
 template <> struct Storage<ST_F64> {
   using T = double;
   using C = double;
   static __device__ __forceinline__ C load(const T* p, size_t k, C) { return p[k]; }
   static __device__ __forceinline__ void store(T* p, size_t k, C v, C) { p[k] = v; }
+  AERO_PLANE_ACCESS
 };
 template <> struct Storage<ST_F32> {
   using T = float;
   using C = float;
   static __device__ __forceinline__ C load(const T* p, size_t k, C) { return p[k]; }
   static __device__ __forceinline__ void store(T* p, size_t k, C v, C) { p[k] = v; }
+  AERO_PLANE_ACCESS
 };
 template <> struct Storage<ST_F16S> {
   using T = __half;
   using C = float;
   static __device__ __forceinline__ C load(const T* p, size_t k, C s) { return __half2float(p[k]) + s; }
   static __device__ __forceinline__ void store(T* p, size_t k, C v, C s) { p[k] = __float2half(v - s); }
+  AERO_PLANE_ACCESS
 };
 template <> struct Storage<ST_BF16S> {
   using T = __hip_bfloat16;
   using C = float;
   static __device__ __forceinline__ C load(const T* p, size_t k, C s) { return __bfloat162float(p[k]) + s; }
   static __device__ __forceinline__ void store(T* p, size_t k, C v, C s) { p[k] = __float2bfloat16(v - s); }
+  AERO_PLANE_ACCESS
 };
+#undef AERO_PLANE_ACCESS
 
 // Link-cell record: one per fluid cell that has at least one surface link.
+// Its q values live in a separate (Q-1)-per-record table (direction i at
+// index i-1, zero where i is not a link).
 struct LinkCell {
   int cell;       // linear cell index z*ny*nx + y*nx + x
   uint32_t mask;  // bit i set: direction i is a link (neighbour along e_i solid)
-  int qoff;       // offset of this cell's q values (one per set bit, ascending i)
-  int pad;
 };
 
 constexpr int NCOEF = 20;  // per-step history row: 8 coefficients + 12 raw sums
 constexpr int NFORCE = 12; // Fx Fy Fz Mx My Mz Fpx Fpy Fpz Fvx Fvy Fvz
+constexpr int NPART = 10;  // per link cell: rho, u xyz of f_pre; F_pre xyz; F_new xyz (see link_pre_terms)
 constexpr int NCHECK = 5;  // nan, nonpositive rho, max|u|, sum rho, count
 constexpr int BLOCK = 256;
 
@@ -157,9 +204,9 @@ struct StepParams {
   void* usplit;  // split mode: ux, uy, uz, compute type, 3 x N
   const uint8_t* flags;
   const LinkCell* lrec;
-  const int* row_start;
-  const float* lq;
-  double* fpart;             // NFORCE doubles per link cell, this step
+  const int* rec_of;   // per cell: its link record (read only where FLAG_LINKS is set)
+  const float* qdir;   // Q-1 wall fractions per link record
+  double* fpart;             // NPART doubles per link cell, this step
   const double* fpart_prev;  // the previous step's sums, reduced by the extra block
   double* hist_prev;         // coefficient row the previous step's sums go to
   int reduce_prev;
@@ -187,6 +234,17 @@ struct StepParams {
   void* ev_stop;
   double* fmat;
   double* check_part;
+};
+
+// Inputs of the per-step force reduction (one block, fixed order).
+struct ForceReduce {
+  int q;               // lattice (9 or 19)
+  const double* part;  // NPART doubles per link cell
+  const LinkCell* lrec;
+  int nrec, nx, ny;
+  double cx, cy, cz;  // moment reference centre
+  double fd, md;      // force and moment denominators
+  double* row;        // NCOEF doubles out
 };
 
 }  // namespace aero

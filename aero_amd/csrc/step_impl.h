@@ -52,22 +52,26 @@ __device__ __forceinline__ void gather(const StepParams& p, const typename Stora
     const int ys = L::EY[i] == 0 ? y : (L::EY[i] > 0 ? ym : yp);
     const int zs = L::EZ[i] == 0 ? z : (L::EZ[i] > 0 ? zm : zp);
     const unsigned src = ((unsigned)zs * (unsigned)p.ny + (unsigned)ys) * (unsigned)p.nx + (unsigned)xs;
-    fn[i] = S::load(B + (size_t)i * N, src, C(L::W[i]) * shift_scale);
+    fn[i] = S::ld(B + (size_t)i * N, src, C(L::W[i]) * shift_scale);
   }
 }
 
-// Locate the link-cell record of `cell` (records sorted by cell, indexed by row).
-__device__ __forceinline__ int find_link_record(const StepParams& p, int cell, int row) {
-  int lo = p.row_start[row], hi = p.row_start[row + 1] - 1;
-  while (lo <= hi) {
-    const int mid = (lo + hi) >> 1;
-    const int c = p.lrec[mid].cell;
-    if (c == cell) return mid;
-    if (c < cell) lo = mid + 1;
-    else hi = mid - 1;
-  }
-  return -1;
+// v if bit is 1, else +0.0, by masking the bits: no per-lane condition (a
+// 64-bit scalar register pair each) is kept live across the unrolled loops.
+__device__ __forceinline__ double keep_if(double v, unsigned bit) {
+  return __longlong_as_double(__double_as_longlong(v) & -(long long)bit);
 }
+__device__ __forceinline__ float keep_if(float v, unsigned bit) {
+  return __int_as_float(__float_as_int(v) & -(int)bit);
+}
+
+// Link-cell record of a cell flagged FLAG_LINKS (a per-cell index only link
+// cells read, so finding it costs one load).
+__device__ __forceinline__ int link_record(const StepParams& p, int cell) { return p.rec_of[cell]; }
+
+#ifndef LINK_GROUP
+#define LINK_GROUP 9
+#endif
 
 // Bounce-back (boundary.py:283-300, boundary3d.py:333-350) or Bouzidi
 // (boundary.py:303-352, boundary3d.py:353-404) on the links of record `rec`.
@@ -75,103 +79,149 @@ __device__ __forceinline__ int find_link_record(const StepParams& p, int cell, i
 // f_pre[i] (plain bounce-back), or interpolated with q: bouzidi 1 follows the
 // reference (q < 1/2 uses f_pre[OPP[i]] at the cell, q >= 1/2 f_pre[i] at the
 // clipped x - e_i), bouzidi 2 is the linear rule (q < 1/2 takes f_pre[i] at
-// x - e_i, q >= 1/2 f_pre[OPP[i]] at the cell).  The cell's f_pre is not held
-// in registers: each link reloads the two values it needs (cache hits, the
-// lines were just pulled), so the common path keeps its small register budget.
+// x - e_i, q >= 1/2 f_pre[OPP[i]] at the cell).  Every direction's operands
+// are loaded whether or not it is a link (addresses stay in the domain) and
+// the result is selected by the mask, so all loads issue together: one
+// memory round trip instead of one per link.
 template <class L, int ST>
 __device__ __forceinline__ void apply_links_light(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
                                                   int rec, int cell, int x, int y, int z,
                                                   typename Storage<ST>::C fn[L::Q]) {
   using S = Storage<ST>;
   using C = typename S::C;
+  constexpr int Q = L::Q;
   const size_t N = (size_t)p.ncell;
-  const LinkCell r = p.lrec[rec];
+  const uint32_t mask = p.lrec[rec].mask;
+  if (!p.bouzidi) {
 #pragma unroll
-  for (int i = 1; i < L::Q; ++i) {
-    if ((r.mask >> i) & 1u) {
-      const int o = L::OPP[i];
-      const C fi = S::load(B + (size_t)i * N, (size_t)cell, C(L::W[i]));
-      if (!p.bouzidi) {
-        fn[o] = fi;
-      } else {
-        const int k = __popc(r.mask & ((1u << i) - 1u));
-        const C q = (C)p.lq[r.qoff + k];
-        const bool upstream = p.bouzidi == 1 ? !(q < C(0.5)) : (q < C(0.5));
-        C partner;
-        if (upstream) {
-          const int xn = clampi(x + L::EX[o], 0, p.nx - 1);
-          const int yn = clampi(y + L::EY[o], 0, p.ny - 1);
-          const int zn = clampi(z + L::EZ[o], 0, p.nz - 1);
-          partner = S::load(B + (size_t)i * N, ((size_t)zn * p.ny + yn) * p.nx + xn, C(L::W[i]));
-        } else {
-          partner = S::load(B + (size_t)o * N, (size_t)cell, C(L::W[o]));
-        }
-        if (q < C(0.5)) {
-          fn[o] = C(2) * q * fi + (C(1) - C(2) * q) * partner;
-        } else {
-          const C inv2q = C(1) / (C(2) * q);
-          fn[o] = inv2q * fi + (C(1) - inv2q) * partner;
-        }
-      }
+    for (int i = 1; i < Q; ++i) {
+      const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, C(L::W[i]));
+      fn[L::OPP[i]] = ((mask >> i) & 1u) ? fi : fn[L::OPP[i]];
     }
+    return;
+  }
+  const float* qr = p.qdir + (size_t)rec * (Q - 1);
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    // directions go in groups of LINK_GROUP: each group's loads issue
+    // together, and the barrier keeps the scheduler from hoisting every
+    // group's loads at once (which would need ~2Q extra registers)
+    if (i > 1 && (i - 1) % LINK_GROUP == 0) __builtin_amdgcn_sched_barrier(0);
+    const int o = L::OPP[i];
+    const C q = (C)qr[i - 1];
+    const bool lo = q < C(0.5);
+    const bool upstream = p.bouzidi == 1 ? !lo : lo;
+    const int xn = clampi(x + L::EX[o], 0, p.nx - 1);
+    const int yn = clampi(y + L::EY[o], 0, p.ny - 1);
+    const int zn = clampi(z + L::EZ[o], 0, p.nz - 1);
+    const unsigned nb = ((unsigned)zn * p.ny + yn) * p.nx + xn;
+    const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, C(L::W[i]));
+    const C f_up = S::ld(B + (size_t)i * N, nb, C(L::W[i]));          // f_pre[i](x - e_i)
+    const C f_op = S::ld(B + (size_t)o * N, (unsigned)cell, C(L::W[o]));  // f_pre[OPP[i]](x)
+    const C partner = upstream ? f_up : f_op;
+    const C a = lo ? C(2) * q : C(1) / (C(2) * q);
+    const C v = a * fi + (C(1) - a) * partner;
+    fn[o] = ((mask >> i) & 1u) ? v : fn[o];
   }
 }
 
-// Momentum exchange of one link cell (forces.py:15-75 and 106-139,
-// forces3d.py:8-66 and 398-425) streaming f_pre from memory, FP64 sums
-// written to the cell's record slot.
+// Momentum exchange at one link cell (forces.py:15-75 and 106-139,
+// forces3d.py:8-66 and 398-425): per link i, f_pre[i](x) + f_new[OPP[i]](x)
+// times e_i; pressure part from feq of the cell's f_pre (u not zeroed), the
+// viscous part the remainder; moments about the reference centre.  The sums
+// split into terms of f_pre alone (link_pre_terms, taken before the cell's
+// populations are pulled) and
+//   S = sum over links of e_i f_new[OPP[i]]
+// (link_post_sum, just before the collision), each written to the cell's
+// NPART-double record slot as soon as it is known, so no force term is held
+// in registers across the step.  The reduction combines them per cell.
 template <class L, int ST>
-__device__ __forceinline__ void link_forces_light(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
-                                                  int rec, int cell, int x, int y, int z,
-                                                  const typename Storage<ST>::C fn[L::Q]) {
-  using S = Storage<ST>;
-  using C = typename S::C;
+__device__ __forceinline__ void link_pre_terms(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
+                                               int rec, int cell) {
+  using St = Storage<ST>;
+  using C = typename St::C;
+  constexpr int Q = L::Q;
   const size_t N = (size_t)p.ncell;
-  double rho = 0, mx = 0, my = 0, mz = 0;
+  const uint32_t mask = p.lrec[rec].mask;
+  C fp[Q];
 #pragma unroll
-  for (int i = 0; i < L::Q; ++i) {
-    const double v = (double)S::load(B + (size_t)i * N, (size_t)cell, C(L::W[i]));
+  for (int i = 0; i < Q; ++i) fp[i] = St::ld(B + (size_t)i * N, (unsigned)cell, C(L::W[i]));
+  double rho = 0, mx = 0, my = 0, mz = 0, fx = 0, fy = 0, fz = 0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const double v = (double)fp[i];
     rho += v;
-    mx += L::EX[i] * v;
-    my += L::EY[i] * v;
-    mz += L::EZ[i] * v;
+    mx = madd(mx, L::EX[i], v);
+    my = madd(my, L::EY[i], v);
+    mz = madd(mz, L::EZ[i], v);
+    const double lv = keep_if(v, (mask >> i) & 1u);
+    fx = madd(fx, L::EX[i], lv);
+    fy = madd(fy, L::EY[i], lv);
+    fz = madd(fz, L::EZ[i], lv);
   }
   const double inv = rho > 0.0 ? 1.0 / rho : 0.0;
-  const double ux = inv * mx, uy = inv * my, uz = inv * mz;
-  const double usq = ux * ux + uy * uy + uz * uz;
-  const double rx = (double)x - p.center[0], ry = (double)y - p.center[1], rz = (double)z - p.center[2];
+  double* out = p.fpart + (size_t)rec * NPART;
+  out[0] = rho;
+  out[1] = inv * mx;
+  out[2] = inv * my;
+  out[3] = inv * mz;
+  out[4] = fx;
+  out[5] = fy;
+  out[6] = fz;
+}
+
+template <class L, class C>
+__device__ __forceinline__ void link_post_sum(const StepParams& p, int rec, const C fn[L::Q]) {
   const uint32_t mask = p.lrec[rec].mask;
-  double out[NFORCE];
-#pragma unroll
-  for (int k = 0; k < NFORCE; ++k) out[k] = 0.0;
+  double sx = 0, sy = 0, sz = 0;
 #pragma unroll
   for (int i = 1; i < L::Q; ++i) {
-    if ((mask >> i) & 1u) {
-      const int o = L::OPP[i];
-      const double fo = (double)S::load(B + (size_t)i * N, (size_t)cell, C(L::W[i]));
-      const double fi = (double)fn[o];
-      const double mom = fo + fi;
-      const double dfx = L::EX[i] * mom, dfy = L::EY[i] * mom, dfz = L::EZ[i] * mom;
-      out[0] += dfx;
-      out[1] += dfy;
-      out[2] += dfz;
-      out[3] += ry * dfz - rz * dfy;
-      out[4] += rz * dfx - rx * dfz;
-      out[5] += rx * dfy - ry * dfx;
-      const double feo = feq_i<double>(L::W[i], rho, L::EX[i] * ux + L::EY[i] * uy + L::EZ[i] * uz, usq);
-      const double fei = feq_i<double>(L::W[o], rho, L::EX[o] * ux + L::EY[o] * uy + L::EZ[o] * uz, usq);
-      const double mp = feo + fei;
-      const double mv = (fo - feo) + (fi - fei);
-      out[6] += L::EX[i] * mp;
-      out[7] += L::EY[i] * mp;
-      out[8] += L::EZ[i] * mp;
-      out[9] += L::EX[i] * mv;
-      out[10] += L::EY[i] * mv;
-      out[11] += L::EZ[i] * mv;
-    }
+    const double v = keep_if((double)fn[L::OPP[i]], (mask >> i) & 1u);
+    sx = madd(sx, L::EX[i], v);
+    sy = madd(sy, L::EY[i], v);
+    sz = madd(sz, L::EZ[i], v);
   }
-#pragma unroll
-  for (int k = 0; k < NFORCE; ++k) p.fpart[(size_t)rec * NFORCE + k] = out[k];
+  double* out = p.fpart + (size_t)rec * NPART;
+  out[7] = sx;
+  out[8] = sy;
+  out[9] = sz;
+}
+
+// The twelve per-cell sums (Fx Fy Fz, M = r x F, Fp xyz, Fv = F - Fp) of one
+// link cell from its NPART record slot; r = cell - centre.  The pressure part
+// uses feq_i + feq_OPP[i] = 2 w_i rho (1 + 4.5 (e_i.u)^2 - 1.5 u.u) (the
+// linear terms cancel, w_OPP[i] = w_i), summed over the set bits in order.
+template <class L>
+__device__ __host__ __forceinline__ void cell_force_terms(const double* part, int cell, uint32_t mask, int nx, int ny,
+                                                          double cx, double cy, double cz, double out[NFORCE]) {
+  using D = DirPack<L>;
+  const double rho = part[0], ux = part[1], uy = part[2], uz = part[3];
+  const double base = 1.0 - 1.5 * (ux * ux + uy * uy + uz * uz);
+  double px = 0, py = 0, pz = 0;
+  for (uint32_t m = mask & ~1u; m; m &= m - 1) {
+    const int i = __builtin_ctz(m);
+    const int ex = D::ex(i), ey = D::ey(i), ez = D::ez(i);
+    const double eu = ex * ux + ey * uy + ez * uz;
+    const double mp = 2.0 * D::w(i) * rho * (base + 4.5 * eu * eu);
+    px += ex * mp;
+    py += ey * mp;
+    pz += ez * mp;
+  }
+  const int x = cell % nx, yz = cell / nx, y = yz % ny, z = yz / ny;
+  const double Fx = part[4] + part[7], Fy = part[5] + part[8], Fz = part[6] + part[9];
+  const double rx = (double)x - cx, ry = (double)y - cy, rz = (double)z - cz;
+  out[0] = Fx;
+  out[1] = Fy;
+  out[2] = Fz;
+  out[3] = ry * Fz - rz * Fy;
+  out[4] = rz * Fx - rx * Fz;
+  out[5] = rx * Fy - ry * Fx;
+  out[6] = px;
+  out[7] = py;
+  out[8] = pz;
+  out[9] = Fx - px;
+  out[10] = Fy - py;
+  out[11] = Fz - pz;
 }
 
 // Uniform Guo body force on a fluid cell (ibm_guo.py:264-301).
@@ -319,30 +369,36 @@ __device__ __forceinline__ void sponge_and_walls(const StepParams& p, int x, int
     }
   }
   if (p.wall_mode != WALL_NONE && (y == 0 || y == p.ny - 1)) {  // boundary.py:162-226
-    const bool moving = p.wall_mode == WALL_MOVING;
-    const bool slip = p.wall_mode == WALL_SLIP;
-    C rsum = 0;
-    if (moving) {
+    // Every branch indexes fn with compile-time constants only, so fn stays
+    // in registers (a runtime-selected index would move it to scratch).
+    if (p.wall_mode == WALL_SLIP) {
+      if (y == 0) {
 #pragma unroll
-      for (int i = 0; i < Q; ++i) rsum += fn[i];
-      rsum = rsum > C(1e-12) ? rsum : C(1e-12);
-    }
-    if (y == 0) {
-      const C add = moving ? C(6.0 * (1.0 / 36.0)) * rsum * (C)p.u_bot : C(0);
-#pragma unroll
-      for (int k = 0; k < L::NWALL; ++k) {
-        const int d = slip ? L::SLIP_BOT[k][0] : L::NOSLIP_BOT[k][0];
-        const int s = slip ? L::SLIP_BOT[k][1] : L::NOSLIP_BOT[k][1];
-        fn[d] = L::MOVE_SGN_BOT[k] == 0 || !moving ? fn[s] : fn[s] + C(L::MOVE_SGN_BOT[k]) * add;
+        for (int k = 0; k < L::NWALL; ++k) fn[L::SLIP_BOT[k][0]] = fn[L::SLIP_BOT[k][1]];
       }
-    }
-    if (y == p.ny - 1) {
-      const C add = moving ? C(6.0 * (1.0 / 36.0)) * rsum * (C)p.u_top : C(0);
+      if (y == p.ny - 1) {
 #pragma unroll
-      for (int k = 0; k < L::NWALL; ++k) {
-        const int d = slip ? L::SLIP_TOP[k][0] : L::NOSLIP_TOP[k][0];
-        const int s = slip ? L::SLIP_TOP[k][1] : L::NOSLIP_TOP[k][1];
-        fn[d] = L::MOVE_SGN_TOP[k] == 0 || !moving ? fn[s] : fn[s] + C(L::MOVE_SGN_TOP[k]) * add;
+        for (int k = 0; k < L::NWALL; ++k) fn[L::SLIP_TOP[k][0]] = fn[L::SLIP_TOP[k][1]];
+      }
+    } else {
+      const bool moving = p.wall_mode == WALL_MOVING;
+      C rsum = 0;
+      if (moving) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) rsum += fn[i];
+        rsum = rsum > C(1e-12) ? rsum : C(1e-12);
+      }
+      if (y == 0) {
+        const C add = moving ? C(6.0 * (1.0 / 36.0)) * rsum * (C)p.u_bot : C(0);
+#pragma unroll
+        for (int k = 0; k < L::NWALL; ++k)
+          fn[L::NOSLIP_BOT[k][0]] = fn[L::NOSLIP_BOT[k][1]] + C(L::MOVE_SGN_BOT[k]) * add;
+      }
+      if (y == p.ny - 1) {
+        const C add = moving ? C(6.0 * (1.0 / 36.0)) * rsum * (C)p.u_top : C(0);
+#pragma unroll
+        for (int k = 0; k < L::NWALL; ++k)
+          fn[L::NOSLIP_TOP[k][0]] = fn[L::NOSLIP_TOP[k][1]] + C(L::MOVE_SGN_TOP[k]) * add;
       }
     }
   }
@@ -359,8 +415,8 @@ __device__ __forceinline__ int pre_outlet(const StepParams& p, const typename St
   if (L::DIM == 3 && p.body_force_on && !(flag & FLAG_SOLID)) body_force<L, C>(p, fn);
   int rec = -1;
   if (flag & FLAG_LINKS) {
-    rec = find_link_record(p, cell, z * p.ny + y);
-    if (rec >= 0) apply_links_light<L, ST>(p, B, rec, cell, x, y, z, fn);
+    rec = link_record(p, cell);
+    apply_links_light<L, ST>(p, B, rec, cell, x, y, z, fn);
   }
   if (x == 0) apply_inlet<L, C>(p, y, z, fn);
   return rec;
@@ -396,7 +452,7 @@ __device__ __forceinline__ int post_bc(const StepParams& p, const typename Stora
         if (commit) p.prev_out[k] = (double)v;
       }
     }  // zero gradient (boundary.py:68-75): fn already is the x = nx-2 state
-    rec = (flag & FLAG_LINKS) ? find_link_record(p, cell, row) : -1;
+    rec = (flag & FLAG_LINKS) ? link_record(p, cell) : -1;
   }
   if constexpr (L::DIM == 2) {
     if (x == p.nx - 1 && p.outlet_mode == OUT_PRESSURE) {  // boundary.py:139-159
@@ -460,7 +516,7 @@ __device__ __forceinline__ void scalar_step(const StepParams& p, int cell, int x
       const C eu = C(L::EX[i]) * ux + C(L::EY[i]) * uy + C(L::EZ[i]) * uz;
       const C geq = C(L::W[i]) * T * (C(1) + C(3) * eu);
       const C gp = solid ? C(L::W[i]) * T : g[i] - om * (g[i] - geq);
-      S::store(gout + (size_t)i * N, (size_t)cell, gp, C(L::W[i]) * (C)p.gshift);
+      S::st(gout + (size_t)i * N, (unsigned)cell, gp, C(L::W[i]) * (C)p.gshift);
     }
   }
 }
@@ -475,9 +531,9 @@ __device__ __forceinline__ void collide(const double* mrt_s, const double* minv_
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     rho += fn[i];
-    mx += C(L::EX[i]) * fn[i];
-    my += C(L::EY[i]) * fn[i];
-    mz += C(L::EZ[i]) * fn[i];
+    mx = madd(mx, L::EX[i], fn[i]);
+    my = madd(my, L::EY[i], fn[i]);
+    mz = madd(mz, L::EZ[i], fn[i]);
   }
   const C inv = rho > C(0) ? C(1) / rho : C(0);
   C ux = inv * mx, uy = inv * my, uz = inv * mz;
@@ -490,8 +546,7 @@ __device__ __forceinline__ void collide(const double* mrt_s, const double* minv_
   C fe[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    const C eu = C(L::EX[i]) * ux + C(L::EY[i]) * uy + C(L::EZ[i]) * uz;
-    fe[i] = feq_i(C(L::W[i]), rho, eu, usq);
+    fe[i] = feq_i(C(L::W[i]), rho, edot<L>(i, ux, uy, uz), usq);
   }
   if constexpr (COLL == COLL_BGK) {
 #pragma unroll
@@ -550,7 +605,7 @@ __device__ __forceinline__ void finish_cell(const StepParams& p, int cell, bool 
     collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, (C)p.omega, (C)p.trt_sm, fn, out);
     T* Bo = (T*)p.fout;
 #pragma unroll
-    for (int i = 0; i < Q; ++i) S::store(Bo + (size_t)i * N, (size_t)cell, out[i], C(L::W[i]));
+    for (int i = 0; i < Q; ++i) S::st(Bo + (size_t)i * N, (unsigned)cell, out[i], C(L::W[i]));
   } else {
     C* F = (C*)p.fsplit;
     C* U = (C*)p.usplit;
@@ -587,26 +642,36 @@ __device__ __forceinline__ void write_coefficients(const double F[NFORCE], doubl
 
 // Fixed-order FP64 reduction of the per-link-cell momentum-exchange sums by
 // one block of BLOCK threads: thread t sums link cells t, t+256, ...
-// sequentially, then a fixed tree combines the partials.  The order never
-// depends on what else runs on the device, so a case reproduces its own bits
-// alone or in a batch, and whichever launch performs it.
-__device__ __forceinline__ void reduce_forces_block(const double* fpart, int nrec, double fd, double md,
-                                                    double* row) {
-  __shared__ double red[NFORCE][BLOCK];
+// sequentially; the four waves then add their partials into a 64-lane LDS
+// row one wave after another, and a fixed tree reduces that row.  The order
+// never depends on timing or on what else runs, so a case reproduces its own
+// bits alone or in a batch, whichever launch performs the reduction.  6 KB of
+// LDS, so the step kernel that hosts it keeps its occupancy.
+template <class L>
+__device__ __forceinline__ void reduce_forces_block(const ForceReduce& r) {
+  __shared__ double red[NFORCE][64];
   double acc[NFORCE];
 #pragma unroll
   for (int k = 0; k < NFORCE; ++k) acc[k] = 0.0;
-  for (int r = threadIdx.x; r < nrec; r += BLOCK) {
+  for (int k = threadIdx.x; k < r.nrec; k += BLOCK) {
+    double t[NFORCE];
+    const LinkCell lc = r.lrec[k];
+    cell_force_terms<L>(r.part + (size_t)k * NPART, lc.cell, lc.mask, r.nx, r.ny, r.cx, r.cy, r.cz, t);
 #pragma unroll
-    for (int k = 0; k < NFORCE; ++k) acc[k] += fpart[(size_t)r * NFORCE + k];
+    for (int j = 0; j < NFORCE; ++j) acc[j] += t[j];
   }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int w = 0; w < BLOCK / 64; ++w) {
+    if (wave == w) {
 #pragma unroll
-  for (int k = 0; k < NFORCE; ++k) red[k][threadIdx.x] = acc[k];
-  __syncthreads();
-  for (int w = BLOCK / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
+      for (int k = 0; k < NFORCE; ++k) red[k][lane] = (w == 0 ? 0.0 : red[k][lane]) + acc[k];
+    }
+    __syncthreads();
+  }
+  for (int h = 32; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
 #pragma unroll
-      for (int k = 0; k < NFORCE; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+      for (int k = 0; k < NFORCE; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
     }
     __syncthreads();
   }
@@ -614,7 +679,7 @@ __device__ __forceinline__ void reduce_forces_block(const double* fpart, int nre
     double F[NFORCE];
 #pragma unroll
     for (int k = 0; k < NFORCE; ++k) F[k] = red[k][0];
-    write_coefficients(F, fd, md, row);
+    write_coefficients(F, r.fd, r.md, r.row);
   }
 }
 
@@ -630,14 +695,29 @@ __device__ __forceinline__ void reduce_forces_block(const double* fpart, int nre
 // block (index 0, dispatched first so it overlaps the cell blocks) reduces the
 // previous step's force sums, written by the previous launch, into that
 // step's coefficient row.
+// Occupancy target of the step kernel: the 4-byte and 2-byte BGK/TRT forms
+// ask the register allocator for 7 waves per SIMD (at most 72 VGPRs; the
+// common pull + collide path needs ~56, the rare link / outlet / reduction
+// paths would otherwise raise the whole kernel to ~80).  FP64 and MRT keep
+// the compiler's choice.
+template <int COLL, int ST>
+struct StepWaves {
+  static constexpr int value = (ST != ST_F64 && COLL != COLL_MRT) ? 7 : 1;
+};
+
 template <class L, int COLL, int ST, bool THERMAL, int KMODE>
-__global__ __launch_bounds__(BLOCK) void step_kernel(StepParams p) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(StepWaves<COLL, ST>::value)))
+void step_kernel(StepParams p) {
   using S = Storage<ST>;
   using C = typename S::C;
   using T = typename S::T;
   constexpr int Q = L::Q;
   if (blockIdx.x == 0) {
-    if (p.reduce_prev) reduce_forces_block(p.fpart_prev, p.nrec, p.fdenom, p.mdenom, p.hist_prev);
+    if (p.reduce_prev) {
+      const ForceReduce r{L::Q, p.fpart_prev, p.lrec, p.nrec, p.nx, p.ny, p.center[0], p.center[1], p.center[2],
+                          p.fdenom, p.mdenom, p.hist_prev};
+      reduce_forces_block<L>(r);
+    }
     return;
   }
   const long long c64 = (long long)(blockIdx.x - 1) * BLOCK + threadIdx.x;
@@ -650,10 +730,12 @@ __global__ __launch_bounds__(BLOCK) void step_kernel(StepParams p) {
   const uint8_t flag = p.flags[cell];
   const bool solid = flag & FLAG_SOLID;
   const T* B = (const T*)p.fin;
+  if (flag & FLAG_LINKS) link_pre_terms<L, ST>(p, B, link_record(p, cell), cell);
+  __builtin_amdgcn_sched_barrier(0);
   C fn[Q];
   const int rec = post_bc<L, ST>(p, B, cell, x, y, z, flag, true, fn);
   if constexpr (THERMAL) scalar_step<L, ST>(p, cell, x, y, z, solid, fn, true);
-  if (rec >= 0) link_forces_light<L, ST>(p, B, rec, cell, x, y, z, fn);
+  if (rec >= 0) link_post_sum<L, C>(p, rec, fn);
   finish_cell<L, COLL, ST, KMODE>(p, cell, solid, fn);
 }
 
@@ -885,7 +967,7 @@ __global__ __launch_bounds__(BLOCK) void collide_kernel(CollideParams p) {
   collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, om, sm, fn, out);
   T* Bo = (T*)p.fout;
 #pragma unroll
-  for (int i = 0; i < Q; ++i) S::store(Bo + (size_t)i * N, (size_t)cell, out[i], C(L::W[i]));
+  for (int i = 0; i < Q; ++i) S::st(Bo + (size_t)i * N, (unsigned)cell, out[i], C(L::W[i]));
 }
 
 }  // namespace aero

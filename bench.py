@@ -56,14 +56,18 @@ def make_case(nx, ny, nz, radius, cx, storage, steps_hint=0, seed=0):
 
 
 def timed(solver, steps, warmup):
+    """Wall time of exactly `steps` steps after `warmup` untimed ones, then the
+    step kernel's average duration from HIP events recorded at the kernel's
+    start and end on its own stream, over a further (untimed) run."""
     import ctypes
-    from aero_amd import _abi
     solver._advance(warmup)
-    solver._ctx.call("aero_set_timing", ctypes.c_int32(1))
     solver._ctx.call("aero_synchronize")
     t0 = time.perf_counter()
     rows = solver._advance(steps)
+    solver._ctx.call("aero_synchronize")
     t1 = time.perf_counter()
+    solver._ctx.call("aero_set_timing", ctypes.c_int32(1))
+    solver._advance(min(steps, 512))
     ms = ctypes.c_double(0.0)
     n = ctypes.c_int64(0)
     solver._ctx.call("aero_get_timing", ctypes.byref(ms), ctypes.byref(n))
