@@ -127,6 +127,20 @@ def traffic_from_profiles(name):
     return None
 
 
+def copy_calibration(mib=1024, reps=4):
+    """A plain float32 device-to-device copy of `mib` MiB, `reps` times: the
+    kernel whose FETCH_SIZE / WRITE_SIZE per dispatch (known to be `mib` MiB
+    each way) scales the counters read for the step kernel."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.rand(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,7 +148,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the extra storage forms")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of known size (calibrates the HBM byte counters)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs one resident domain per device and "
               "has no multi-GPU path (see DESIGN.md for how it would shard).", file=sys.stderr)
@@ -169,7 +188,7 @@ def main():
                    "trt_lambda": 0.1875, "storage": "f32", "links": int(len(s.surface_links))},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles("step_d3q19_trt_f32_128x64x64"),
+                     "traffic": traffic_from_profiles("step_kernel_d3q19_trt_f32_128x64x64"),
                      "kernel": "aero::step_kernel<D3Q19, TRT, F32>", "kernel_ms": round(kern_ms, 5),
                      "algorithmic_bytes_per_launch": bytes_per_launch, "cells": ncell, "link_cells": nlink,
                      "bytes_per_cell": 2 * 19 * 4},
