@@ -6,15 +6,31 @@
 
 namespace aero {
 
-hipError_t launch_step_d3q19_bgk(int st, bool thermal, int kmode, const StepParams& p, hipStream_t s);
-hipError_t launch_step_d3q19_trt(int st, bool thermal, int kmode, const StepParams& p, hipStream_t s);
-hipError_t launch_step_d3q19_mrt(int st, bool thermal, int kmode, const StepParams& p, hipStream_t s);
+hipError_t launch_batch_step(int q, int coll, int st, bool thermal, const BatchCase* cases, const int2* tiles,
+                             int ntiles, int k, hipStream_t s, void* e0, void* e1) {
+  if (q == 9) {
+    switch (coll) {
+      case COLL_BGK: return launch_batch_d2q9_bgk(st, thermal, cases, tiles, ntiles, k, s, e0, e1);
+      case COLL_TRT: return launch_batch_d2q9_trt(st, thermal, cases, tiles, ntiles, k, s, e0, e1);
+      case COLL_MRT: return launch_batch_d2q9_mrt(st, thermal, cases, tiles, ntiles, k, s, e0, e1);
+    }
+  } else {
+    switch (coll) {
+      case COLL_BGK: return launch_batch_d3q19_bgk(st, thermal, cases, tiles, ntiles, k, s, e0, e1);
+      case COLL_TRT: return launch_batch_d3q19_trt(st, thermal, cases, tiles, ntiles, k, s, e0, e1);
+      case COLL_MRT: return launch_batch_d3q19_mrt(st, thermal, cases, tiles, ntiles, k, s, e0, e1);
+    }
+  }
+  return hipErrorInvalidValue;
+}
 
-hipError_t launch_step_d3q19(int coll, int st, bool thermal, int kmode, const StepParams& p, hipStream_t s) {
-  switch (coll) {
-    case COLL_BGK: return launch_step_d3q19_bgk(st, thermal, kmode, p, s);
-    case COLL_TRT: return launch_step_d3q19_trt(st, thermal, kmode, p, s);
-    case COLL_MRT: return launch_step_d3q19_mrt(st, thermal, kmode, p, s);
+hipError_t launch_split_step(int q, int coll, int st, bool thermal, const StepParams& p, hipStream_t s) {
+  if (q == 9) {
+    if (coll == COLL_BGK) return launch_split_d2q9_bgk(st, thermal, p, s);
+    if (coll == COLL_TRT) return launch_split_d2q9_trt(st, thermal, p, s);
+  } else {
+    if (coll == COLL_BGK) return launch_split_d3q19_bgk(st, thermal, p, s);
+    if (coll == COLL_TRT) return launch_split_d3q19_trt(st, thermal, p, s);
   }
   return hipErrorInvalidValue;
 }
@@ -83,7 +99,7 @@ __global__ __launch_bounds__(BLOCK) void recycling_plane_kernel(StepParams p, do
   const int x = p.nx - 2;
   const int cell = row * p.nx + x;
   C fn[19];
-  pre_outlet<D3Q19, ST>(p, (const typename S::T*)p.fin, cell, x, y, z, p.flags[cell], fn);
+  pre_outlet<D3Q19, ST>(p, dyn_of(p), (const typename S::T*)p.fin, cell, x, y, z, p.flags[cell], fn);
   double rho = 0, mx = 0, my = 0, mz = 0;
 #pragma unroll
   for (int i = 0; i < 19; ++i) {
@@ -357,6 +373,18 @@ hipError_t launch_check_f64(int q, int nx, int ny, int nz, int zc, const double*
 // kernel's block 0 uses (reduce_forces_block), for the last step of a chunk.
 template <class L>
 __global__ __launch_bounds__(BLOCK) void reduce_forces_kernel(ForceReduce r) { reduce_forces_block<L>(r); }
+
+template <class L>
+__global__ __launch_bounds__(BLOCK) void reduce_forces_many_kernel(const ForceReduce* rs) {
+  reduce_forces_block<L>(rs[blockIdx.x]);
+}
+
+hipError_t launch_reduce_forces_many(int q, const ForceReduce* rs, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (q == 9) reduce_forces_many_kernel<D2Q9><<<n, BLOCK, 0, s>>>(rs);
+  else reduce_forces_many_kernel<D3Q19><<<n, BLOCK, 0, s>>>(rs);
+  return hipGetLastError();
+}
 
 hipError_t launch_reduce_forces(const ForceReduce& r, hipStream_t s) {
   if (r.q == 9) reduce_forces_kernel<D2Q9><<<1, BLOCK, 0, s>>>(r);

@@ -83,6 +83,66 @@ size_t storage_bytes(int st) {
 
 }  // namespace
 
+// Buffers for stepping a set of cases in chunks of at most STEP_CHUNK steps:
+// the batched kernel's case table (pinned host copy + device copy, double
+// buffered) and packed tile list, double-buffered coefficient rows with
+// pinned staging, and the last-step force reductions.  They only grow, so
+// stepping a set of cases no larger than before allocates nothing.
+struct StepRunner {
+  DevBuf cases[2], tiles, rows[2], reds[2];
+  BatchCase* cases_host[2] = {nullptr, nullptr};
+  ForceReduce* reds_host[2] = {nullptr, nullptr};
+  double* stage[2] = {nullptr, nullptr};
+  size_t ncase_cap = 0, rows_cap = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::vector<std::pair<const void*, int>> tile_key;  // case list the tile table was built for
+
+  template <class T>
+  static void host_grow(T*& p, size_t n) {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    ck(hipHostMalloc((void**)&p, sizeof(T) * std::max<size_t>(1, n), hipHostMallocDefault), "hipHostMalloc");
+  }
+  void reserve(int ncase, int ch) {
+    const size_t nrows = (size_t)ncase * ch * NCOEF;
+    for (int b = 0; b < 2; ++b) {
+      if (!ev[b]) ck(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming), "hipEventCreate");
+      cases[b].alloc(sizeof(BatchCase) * ncase);
+      reds[b].alloc(sizeof(ForceReduce) * ncase);
+      rows[b].alloc(sizeof(double) * nrows);
+    }
+    if ((size_t)ncase > ncase_cap) {
+      for (int b = 0; b < 2; ++b) {
+        host_grow(cases_host[b], ncase);
+        host_grow(reds_host[b], ncase);
+      }
+      ncase_cap = ncase;
+    }
+    if (nrows > rows_cap) {
+      for (int b = 0; b < 2; ++b) host_grow(stage[b], nrows);
+      rows_cap = nrows;
+    }
+  }
+  void release() {
+    for (int b = 0; b < 2; ++b) {
+      cases[b].release();
+      rows[b].release();
+      reds[b].release();
+      if (cases_host[b]) (void)hipHostFree(cases_host[b]);
+      if (reds_host[b]) (void)hipHostFree(reds_host[b]);
+      if (stage[b]) (void)hipHostFree(stage[b]);
+      if (ev[b]) (void)hipEventDestroy(ev[b]);
+      cases_host[b] = nullptr;
+      reds_host[b] = nullptr;
+      stage[b] = nullptr;
+      ev[b] = nullptr;
+    }
+    tiles.release();
+    ncase_cap = rows_cap = 0;
+    tile_key.clear();
+  }
+};
+
 struct aero_ctx {
   aero_desc d;
   int device = 0;
@@ -123,6 +183,7 @@ struct aero_ctx {
   DevBuf checkpart, checkout;
   // optional event timing of the step kernel
   bool timing = false;
+  StepRunner runner;  // single-case stepping (a batch of one)
   std::vector<hipEvent_t> ev_pool;
   double timed_ms = 0.0;
   long long timed_launches = 0;
@@ -144,17 +205,17 @@ struct aero_ctx {
     p.nrow = nrow;
     p.ncell = N;
     p.nblocks = nblocks;
-    p.flags = flags.as<uint8_t>();
-    p.lrec = lrec.as<LinkCell>();
-    p.rec_of = rec_of.as<int>();
-    p.qdir = qdir.as<float>();
-    p.fpart = fpart[fpart_side].as<double>();
+    p.flags = (decltype(p.flags))(flags.as<uint8_t>());
+    p.lrec = (decltype(p.lrec))(lrec.as<LinkCell>());
+    p.rec_of = (decltype(p.rec_of))(rec_of.as<int>());
+    p.qdir = (decltype(p.qdir))(qdir.as<float>());
+    p.fpart = (decltype(p.fpart))(fpart[fpart_side].as<double>());
     p.nrec = nrec;
     p.fdenom = force_denom();
     p.mdenom = moment_denom();
-    p.sigma = d.sponge_thickness > 0 ? sigma.as<double>() : nullptr;
-    p.minv_d = minv_d.as<double>();
-    p.minv_f = minv_f.as<float>();
+    p.sigma = (decltype(p.sigma))(d.sponge_thickness > 0 ? sigma.as<double>() : nullptr);
+    p.minv_d = (decltype(p.minv_d))(minv_d.as<double>());
+    p.minv_f = (decltype(p.minv_f))(minv_f.as<float>());
     for (int a = 0; a < 19; ++a) p.mrt_s[a] = d.mrt_s[a];
     p.omega = d.omega;
     p.trt_lambda = d.trt_lambda;
@@ -186,7 +247,7 @@ struct aero_ctx {
     p.buoyancy = d.buoyancy;
     p.bouzidi = d.bouzidi;
     p.check_z = -1;
-    p.recyc = recyc.as<double>();
+    p.recyc = (decltype(p.recyc))(recyc.as<double>());
     return p;
   }
 
@@ -197,10 +258,49 @@ struct aero_ctx {
     return Q == 19 ? 0.5 * d.rho0 * d.u0 * d.u0 * d.D * d.D * d.D : 0.5 * d.rho0 * d.u0 * d.u0 * d.D * d.D;
   }
 
-  void launch_step_kernel(int kmode, const StepParams& p) {
-    hipError_t e = Q == 9 ? launch_step_d2q9(d.collision, d.storage, d.thermal != 0, kmode, p, stream)
-                          : launch_step_d3q19(d.collision, d.storage, d.thermal != 0, kmode, p, stream);
-    ck(e, "step kernel launch");
+  void launch_split_kernel(const StepParams& p) {
+    ck(launch_split_step(Q, d.collision, d.storage, d.thermal != 0, p, stream), "split step launch");
+  }
+
+  // Parameters of the batched kernel for the next steps from the current
+  // state: p[0] for even steps of the chunk, p[1] for odd ones.
+  BatchCase make_case(long long step0, const double* planes_dev, double* hist_rows) const {
+    BatchCase bc;
+    std::memset(&bc, 0, sizeof(bc));
+    for (int par = 0; par < 2; ++par) {
+      StepParams p = base_params();
+      const int rd = (side + par) & 1;
+      p.fin = (decltype(p.fin))(B[rd].p);
+      p.fout = (decltype(p.fout))(B[rd ^ 1].p);
+      p.gin = (decltype(p.gin))(G[rd].p);
+      p.gout = (decltype(p.gout))(G[rd ^ 1].p);
+      const int ps = convective() ? ((prev_side + par) & 1) : prev_side;
+      p.prev_in = (decltype(p.prev_in))(prev[ps].as<double>());
+      p.prev_out = (decltype(p.prev_out))(prev[ps ^ 1].as<double>());
+      // force sums of a step go to the other buffer; block 0 of the launch
+      // reduces the previous step's
+      p.fpart = (decltype(p.fpart))(fpart[(fpart_side + par + 1) & 1].as<double>());
+      p.fpart_prev = (decltype(p.fpart_prev))(fpart[(fpart_side + par) & 1].as<double>());
+      p.commit = 1;
+      bc.p[par] = p;
+    }
+    bc.step0 = step0;
+    bc.planes = (decltype(bc.planes))planes_dev;
+    bc.hist = (decltype(bc.hist))hist_rows;
+    bc.first_g_skip = g_skip_pending;
+    return bc;
+  }
+
+  // State bookkeeping after n steps ending at step number `last`.
+  void advance_state(int n, long long last, const double* last_plane_dev) {
+    const int flip = n & 1;
+    side ^= flip;
+    if (convective()) prev_side ^= flip;
+    fpart_side ^= flip;
+    last_step_no = last;
+    last_g_skip = n == 1 ? g_skip_pending : 0;
+    g_skip_pending = 0;
+    last_plane = last_plane_dev;
   }
 
   // B[dst] = collide(F) with the optional LES finite-difference omega.
@@ -232,11 +332,12 @@ struct aero_ctx {
     ck(launch_collide(Q, d.collision, d.storage, cp, stream), "collide launch");
   }
 
-  void recycling_prepass(int read_side) {
+  void recycling_prepass(int read_side, hipStream_t s = nullptr) {
+    if (!s) s = stream;
     StepParams p = base_params();
-    p.fin = B[read_side].p;
-    ck(launch_recycling_plane(d.storage, p, recyc.as<double>(), stream), "recycling plane");
-    ck(launch_reduce_recycling(recyc.as<double>(), nrow, d.u0, stream), "recycling reduce");
+    p.fin = (decltype(p.fin))(B[read_side].p);
+    ck(launch_recycling_plane(d.storage, p, recyc.as<double>(), s), "recycling plane");
+    ck(launch_reduce_recycling(recyc.as<double>(), nrow, d.u0, s), "recycling reduce");
   }
 
   // f64 post-boundary state of the last completed step into fcache.
@@ -247,15 +348,15 @@ struct aero_ctx {
     const int old = side ^ 1;
     if (d.inlet == AERO_IN_RECYCLING) recycling_prepass(old);
     StepParams p = base_params();
-    p.fin = B[old].p;
-    p.gin = G[old].p;
-    p.prev_in = prev[convective() ? (prev_side ^ 1) : prev_side].as<double>();
-    p.plane = last_plane;
+    p.fin = (decltype(p.fin))(B[old].p);
+    p.gin = (decltype(p.gin))(G[old].p);
+    p.prev_in = (decltype(p.prev_in))(prev[convective() ? (prev_side ^ 1) : prev_side].as<double>());
+    p.plane = (decltype(p.plane))(last_plane);
     p.phase = 0.17 * (double)last_step_no;
     p.g_bc_skip = last_g_skip;
     p.commit = 0;
     p.out_mode = 0;
-    p.fmat = fcache.as<double>();
+    p.fmat = (decltype(p.fmat))(fcache.as<double>());
     ck(launch_materialize(Q, d.storage, d.thermal != 0, p, stream), "materialize launch");
     fcache_valid = true;
   }
@@ -264,7 +365,7 @@ struct aero_ctx {
     if (gcache_valid) return;
     gcache.alloc(sizeof(double) * Q * N);
     StepParams p = base_params();
-    p.gin = G[side].p;
+    p.gin = (decltype(p.gin))(G[side].p);
     p.g_bc_skip = 0;
     ck(launch_materialize_g(Q, d.storage, p, gcache.as<double>(), stream), "materialize g");
     gcache_valid = true;
@@ -305,6 +406,152 @@ void inverse_stream(const aero_ctx* c, const double* g, std::vector<double>& out
 
 static void set_geometry_impl(aero_ctx* c, const uint8_t* solid, int64_t nlinks, const int32_t* links,
                               const float* q);
+
+// Advance every case in `cs` by nsteps steps on `stream`: per step one
+// batched-kernel launch for each group of cases that share lattice,
+// collision, storage and scalar transport (plus the recycling prepass of
+// cases that have one).  Coefficient rows go to hists[i] (or nowhere when
+// null); per-step inlet planes come from planes[i].
+static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long long* step0s,
+                      const double* const* planes, double* const* hists, hipStream_t stream, StepRunner& R) {
+  const int n = (int)cs.size();
+  if (n == 0 || nsteps <= 0) return;
+  int CH = STEP_CHUNK;
+  for (aero_ctx* c : cs) CH = std::min(CH, c->chunk_steps());
+  R.reserve(n, CH);
+  // groups and the packed tile list (rebuilt only when the case list changes)
+  struct Group {
+    int q, coll, st, th, toff, ntiles;
+  };
+  std::vector<Group> groups;
+  std::vector<std::vector<int>> members;
+  for (int i = 0; i < n; ++i) {
+    const aero_desc& d = cs[i]->d;
+    int g = 0;
+    for (; g < (int)groups.size(); ++g)
+      if (groups[g].q == d.q && groups[g].coll == d.collision && groups[g].st == d.storage &&
+          groups[g].th == (d.thermal != 0))
+        break;
+    if (g == (int)groups.size()) {
+      groups.push_back(Group{d.q, d.collision, d.storage, d.thermal != 0, 0, 0});
+      members.emplace_back();
+    }
+    members[g].push_back(i);
+  }
+  std::vector<std::pair<const void*, int>> key;
+  for (aero_ctx* c : cs) key.emplace_back((const void*)c, c->nblocks);
+  {
+    std::vector<int2> tiles;
+    for (size_t g = 0; g < groups.size(); ++g) {
+      groups[g].toff = (int)tiles.size();
+      for (int i : members[g])
+        for (int b = 0; b <= cs[i]->nblocks; ++b) tiles.push_back(make_int2(i, b));
+      groups[g].ntiles = (int)tiles.size() - groups[g].toff;
+    }
+    if (key != R.tile_key) {
+      R.tiles.alloc(sizeof(int2) * tiles.size());
+      ck(hipMemcpyAsync(R.tiles.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, stream),
+         "upload tiles");
+      ck(hipStreamSynchronize(stream), "tiles sync");
+      R.tile_key = key;
+    }
+  }
+  const bool timed = n == 1 && cs[0]->timing;
+  int pend_off[2] = {-1, -1}, pend_n[2] = {0, 0};
+  auto drain = [&](int b) {
+    if (pend_off[b] < 0) return;
+    ck(hipEventSynchronize(R.ev[b]), "chunk sync");
+    for (int i = 0; i < n; ++i)
+      if (hists[i])
+        std::memcpy(hists[i] + (size_t)NCOEF * pend_off[b], R.stage[b] + (size_t)i * CH * NCOEF,
+                    sizeof(double) * NCOEF * pend_n[b]);
+    pend_off[b] = -1;
+  };
+  bool any_hist = false;
+  for (int i = 0; i < n; ++i) any_hist |= hists[i] != nullptr;
+  for (int s0 = 0, ci = 0; s0 < nsteps; s0 += CH, ++ci) {
+    const int nk = std::min(CH, nsteps - s0);
+    const int b = ci & 1;
+    drain(b);  // also frees this buffer's pinned case table and reductions
+    double* rows = R.rows[b].as<double>();
+    ck(hipMemsetAsync(rows, 0, sizeof(double) * NCOEF * CH * (size_t)n, stream), "zero rows");
+    std::vector<const double*> last_planes(n, nullptr);
+    for (int i = 0; i < n; ++i) {
+      aero_ctx* c = cs[i];
+      const double* pdev = nullptr;
+      if (c->need_planes()) {
+        const size_t plane_cnt = 3 * (size_t)c->nrow;
+        ck(hipMemcpyAsync(c->planes[b].p, planes[i] + plane_cnt * s0, sizeof(double) * plane_cnt * nk,
+                          hipMemcpyHostToDevice, stream),
+           "upload planes");
+        pdev = c->planes[b].as<double>();
+        last_planes[i] = pdev + plane_cnt * (nk - 1);
+      }
+      R.cases_host[b][i] = c->make_case(step0s[i] + s0, pdev, rows + (size_t)i * CH * NCOEF);
+    }
+    ck(hipMemcpyAsync(R.cases[b].p, R.cases_host[b], sizeof(BatchCase) * n, hipMemcpyHostToDevice, stream),
+       "upload cases");
+    const BatchCase* cases = R.cases[b].as<BatchCase>();
+    for (int k = 0; k < nk; ++k) {
+      for (aero_ctx* c : cs)
+        if (c->d.inlet == AERO_IN_RECYCLING) c->recycling_prepass((c->side + k) & 1, stream);
+      for (const Group& g : groups)
+        ck(launch_batch_step(g.q, g.coll, g.st, g.th, cases, R.tiles.as<int2>() + g.toff, g.ntiles, k, stream,
+                             timed ? cs[0]->ev_pool[2 * k] : nullptr, timed ? cs[0]->ev_pool[2 * k + 1] : nullptr),
+           "step launch");
+    }
+    // the chunk's last step is reduced here, so every chunk's rows are
+    // complete when they are copied out
+    int m = 0;
+    for (int q : {9, 19}) {
+      const int m0 = m;
+      for (int i = 0; i < n; ++i) {
+        aero_ctx* c = cs[i];
+        if (c->Q != q || c->nrec == 0) continue;
+        R.reds_host[b][m++] = make_force_reduce(c->Q, c->fpart[(c->fpart_side + nk) & 1].as<double>(),
+                                          c->lrec.as<LinkCell>(), c->nrec, c->d.nx, c->d.ny, c->d.center[0],
+                                          c->d.center[1], c->d.center[2], c->force_denom(), c->moment_denom(),
+                                          rows + (size_t)i * CH * NCOEF + (size_t)NCOEF * (nk - 1));
+      }
+      if (m > m0) {
+        ck(hipMemcpyAsync(R.reds[b].as<ForceReduce>() + m0, R.reds_host[b] + m0, sizeof(ForceReduce) * (m - m0),
+                          hipMemcpyHostToDevice, stream),
+           "upload reductions");
+        ck(launch_reduce_forces_many(q, R.reds[b].as<ForceReduce>() + m0, m - m0, stream), "force reduction");
+      }
+    }
+    if (any_hist)
+      ck(hipMemcpyAsync(R.stage[b], rows, sizeof(double) * NCOEF * CH * (size_t)n, hipMemcpyDeviceToHost, stream),
+         "download rows");
+    ck(hipEventRecord(R.ev[b], stream), "chunk event");
+    pend_off[b] = s0;
+    pend_n[b] = nk;
+    for (int i = 0; i < n; ++i) cs[i]->advance_state(nk, step0s[i] + s0 + nk, last_planes[i]);
+    if (timed) {
+      aero_ctx* c = cs[0];
+      ck(hipEventSynchronize(c->ev_pool[2 * nk - 1]), "timing sync");
+      for (int k = 0; k < nk; ++k) {
+        float ms = 0.f;
+        ck(hipEventElapsedTime(&ms, c->ev_pool[2 * k], c->ev_pool[2 * k + 1]), "event time");
+        c->timed_ms += ms;
+      }
+      c->timed_launches += nk;
+    }
+  }
+  for (aero_ctx* c : cs) {
+    // keep the last step's inlet plane for re-deriving its state later
+    if (c->need_planes()) {
+      ck(hipMemcpyAsync(c->last_plane_buf.p, c->last_plane, sizeof(double) * 3 * (size_t)c->nrow,
+                        hipMemcpyDeviceToDevice, stream),
+         "keep last plane");
+      c->last_plane = c->last_plane_buf.as<double>();
+    }
+    c->invalidate();
+  }
+  drain(0);
+  drain(1);
+  ck(hipStreamSynchronize(stream), "step sync");
+}
 
 extern "C" {
 
@@ -378,11 +625,14 @@ int aero_create(const aero_desc* desc, aero_ctx** out) {
     ck(hipMemcpy(c->minv_d.p, d.mrt_minv, sizeof(double) * 361, hipMemcpyHostToDevice), "copy minv");
     ck(hipMemcpy(c->minv_f.p, mf.data(), sizeof(float) * 361, hipMemcpyHostToDevice), "copy minv");
     c->recyc.alloc(sizeof(double) * (4 * (size_t)c->nrow + 1));
-    for (int b = 0; b < 2; ++b) {
-      c->hist_dev[b].alloc(sizeof(double) * NCOEF * STEP_CHUNK);
-      ck(hipHostMalloc((void**)&c->hist_stage[b], sizeof(double) * NCOEF * STEP_CHUNK, hipHostMallocDefault),
-         "hipHostMalloc");
-      ck(hipEventCreateWithFlags(&c->chunk_ev[b], hipEventDisableTiming), "hipEventCreate");
+    c->runner.reserve(1, c->chunk_steps());
+    if (c->les_split()) {
+      for (int b = 0; b < 2; ++b) {
+        c->hist_dev[b].alloc(sizeof(double) * NCOEF * STEP_CHUNK);
+        ck(hipHostMalloc((void**)&c->hist_stage[b], sizeof(double) * NCOEF * STEP_CHUNK, hipHostMallocDefault),
+           "hipHostMalloc");
+        ck(hipEventCreateWithFlags(&c->chunk_ev[b], hipEventDisableTiming), "hipEventCreate");
+      }
     }
     if (d.inlet == AERO_IN_FIELD || d.inlet == AERO_IN_SEM) {
       for (int b = 0; b < 2; ++b) c->planes[b].alloc(sizeof(double) * 3 * (size_t)c->nrow * c->chunk_steps());
@@ -413,6 +663,7 @@ int aero_destroy(aero_ctx* c) {
                     &c->damp, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
                     &c->gcache, &c->tmp, &c->checkpart, &c->checkout};
   for (DevBuf* b : bufs) b->release();
+  c->runner.release();
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   for (int b = 0; b < 2; ++b) {
     if (c->hist_stage[b]) (void)hipHostFree(c->hist_stage[b]);
@@ -421,6 +672,65 @@ int aero_destroy(aero_ctx* c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
+}
+
+// ------------------------------------------------------------------ batch
+struct aero_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  StepRunner runner;
+};
+
+int aero_batch_create(aero_batch** out) {
+  *out = nullptr;
+  aero_batch* b = new aero_batch();
+  const int rc = guarded([&] {
+    ck(hipGetDevice(&b->device), "hipGetDevice");
+    ck(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+  });
+  if (rc) {
+    aero_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+int aero_batch_destroy(aero_batch* b) {
+  if (!b) return 0;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  b->runner.release();
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+int aero_batch_step(aero_batch* b, int32_t n, aero_ctx* const* ctxs, int32_t nsteps, const int64_t* step0s,
+                    const double* const* planes, double* const* hists) {
+  return guarded([&] {
+    if (n <= 0 || nsteps <= 0) return;
+    ck(hipSetDevice(b->device), "hipSetDevice");
+    std::vector<aero_ctx*> cs(ctxs, ctxs + n);
+    std::vector<long long> s0(n);
+    std::vector<const double*> pl(n, nullptr);
+    std::vector<double*> hs(n, nullptr);
+    for (int i = 0; i < n; ++i) {
+      aero_ctx* c = cs[i];
+      if (!c) throw std::invalid_argument("null context in batch");
+      if (c->device != b->device) throw std::invalid_argument("every context of a batch must be on its device");
+      if (!c->has_state) throw std::runtime_error("no state: call aero_set_f first");
+      if (c->les_split()) throw std::invalid_argument("LES contexts step on their own (aero_step)");
+      for (int j = 0; j < i; ++j)
+        if (cs[j] == c) throw std::invalid_argument("a context appears twice in a batch");
+      if (c->need_planes() && !(planes && planes[i])) throw std::invalid_argument("this inlet needs per-step planes");
+      s0[i] = step0s ? step0s[i] : 0;
+      pl[i] = planes ? planes[i] : nullptr;
+      hs[i] = hists ? hists[i] : nullptr;
+      ck(hipStreamSynchronize(c->stream), "context sync");  // work queued on the context's own stream
+    }
+    run_cases(cs, nsteps, s0.data(), pl.data(), hs.data(), b->stream, b->runner);
+  });
 }
 
 }  // extern "C"
@@ -590,9 +900,17 @@ int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, 
     c->set_device();
     const bool need_planes = c->need_planes();
     if (need_planes && !planes) throw std::invalid_argument("this inlet needs per-step planes");
+    if (!c->les_split()) {
+      // the batched kernel with a batch of one: the same code path as in a batch
+      const std::vector<aero_ctx*> cs{c};
+      const long long s0 = step0;
+      run_cases(cs, nsteps, &s0, &planes, &hist, c->stream, c->runner);
+      return;
+    }
+    // LES: split step (post-boundary state + velocity), finite-difference
+    // relaxation field, then the collision, every step
     const size_t plane_cnt = 3 * (size_t)c->nrow;
     const double fd = c->force_denom(), md = c->moment_denom();
-    const int kmode = c->les_split() ? 1 : 0;
     const int CH = c->chunk_steps();
     // rows of chunk b wait in hist_stage[b] until that buffer is reused
     int pend_off[2] = {-1, -1}, pend_n[2] = {0, 0};
@@ -616,33 +934,27 @@ int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, 
         const int rd = c->side, wr = c->side ^ 1;
         if (c->d.inlet == AERO_IN_RECYCLING) c->recycling_prepass(rd);
         StepParams p = c->base_params();
-        p.fin = c->B[rd].p;
-        p.fout = c->B[wr].p;
-        p.gin = c->G[rd].p;
-        p.gout = c->G[wr].p;
-        p.prev_in = c->prev[c->prev_side].as<double>();
-        p.prev_out = c->prev[c->prev_side ^ 1].as<double>();
-        p.plane = need_planes ? c->planes[b].as<double>() + plane_cnt * k : nullptr;
+        p.fin = (decltype(p.fin))(c->B[rd].p);
+        p.fout = (decltype(p.fout))(c->B[wr].p);
+        p.gin = (decltype(p.gin))(c->G[rd].p);
+        p.gout = (decltype(p.gout))(c->G[wr].p);
+        p.prev_in = (decltype(p.prev_in))(c->prev[c->prev_side].as<double>());
+        p.prev_out = (decltype(p.prev_out))(c->prev[c->prev_side ^ 1].as<double>());
+        p.plane = (decltype(p.plane))(need_planes ? c->planes[b].as<double>() + plane_cnt * k : nullptr);
         p.phase = 0.17 * (double)step_no;
         p.g_bc_skip = c->g_skip_pending;
         p.commit = 1;
-        if (kmode) {
-          p.fsplit = c->fsplit.p;
-          p.usplit = c->usplit.p;
-        }
+        p.fsplit = (decltype(p.fsplit))(c->fsplit.p);
+        p.usplit = (decltype(p.usplit))(c->usplit.p);
         // force sums of this step go to the other buffer; the launch's block 0
         // reduces the previous step's into that step's coefficient row
         const int fs = c->fpart_side ^ 1;
-        p.fpart = c->fpart[fs].as<double>();
-        p.fpart_prev = c->fpart[c->fpart_side].as<double>();
+        p.fpart = (decltype(p.fpart))(c->fpart[fs].as<double>());
+        p.fpart_prev = (decltype(p.fpart_prev))(c->fpart[c->fpart_side].as<double>());
         p.reduce_prev = (k > 0 && c->nrec > 0) ? 1 : 0;
-        p.hist_prev = k > 0 ? rows + (size_t)NCOEF * (k - 1) : nullptr;
-        if (c->timing) {
-          p.ev_start = c->ev_pool[2 * k];
-          p.ev_stop = c->ev_pool[2 * k + 1];
-        }
-        c->launch_step_kernel(kmode, p);
-        if (kmode) c->collide_field(c->fsplit.p, c->usplit.p, wr);
+        p.hist_prev = (decltype(p.hist_prev))(k > 0 ? rows + (size_t)NCOEF * (k - 1) : nullptr);
+        c->launch_split_kernel(p);
+        c->collide_field(c->fsplit.p, c->usplit.p, wr);
         c->fpart_side = fs;
         c->last_plane = p.plane;
         c->last_step_no = step_no;
@@ -654,8 +966,9 @@ int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, 
       // the chunk's last step is reduced here, so every chunk's rows are
       // complete when its copy is issued
       if (c->nrec > 0) {
-        const ForceReduce r{c->Q, c->fpart[c->fpart_side].as<double>(), c->lrec.as<LinkCell>(), c->nrec, c->d.nx, c->d.ny,
-                            c->d.center[0], c->d.center[1], c->d.center[2], fd, md, rows + (size_t)NCOEF * (n - 1)};
+        const ForceReduce r = make_force_reduce(c->Q, c->fpart[c->fpart_side].as<double>(), c->lrec.as<LinkCell>(),
+                                                c->nrec, c->d.nx, c->d.ny, c->d.center[0], c->d.center[1],
+                                                c->d.center[2], fd, md, rows + (size_t)NCOEF * (n - 1));
         ck(launch_reduce_forces(r, c->stream), "force reduction");
       } else {
         ck(hipMemsetAsync(rows, 0, sizeof(double) * NCOEF * (size_t)n, c->stream), "zero history");
@@ -668,15 +981,6 @@ int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, 
       ck(hipEventRecord(c->chunk_ev[b], c->stream), "chunk event");
       pend_off[b] = s0;
       pend_n[b] = n;
-      if (c->timing) {
-        ck(hipEventSynchronize(c->ev_pool[2 * n - 1]), "timing sync");
-        for (int k = 0; k < n; ++k) {
-          float ms = 0.f;
-          ck(hipEventElapsedTime(&ms, c->ev_pool[2 * k], c->ev_pool[2 * k + 1]), "event time");
-          c->timed_ms += ms;
-        }
-        c->timed_launches += n;
-      }
     }
     // keep the last step's inlet plane for re-deriving its state later
     if (need_planes) {

@@ -9,9 +9,34 @@ namespace aero {
 
 struct CollideParams;
 
-// kmode 0: fused step, 1: split step (LES finite-difference pass follows)
-hipError_t launch_step_d2q9(int coll, int st, bool thermal, int kmode, const StepParams& p, hipStream_t s);
-hipError_t launch_step_d3q19(int coll, int st, bool thermal, int kmode, const StepParams& p, hipStream_t s);
+// Per (lattice, collision) instantiations (step_<lattice>_<collision>.hip).
+#define AERO_DECL_BATCH(NAME)                                                                   \
+  hipError_t launch_batch_##NAME(int st, bool thermal, const BatchCase* cases, const int2* tiles, \
+                                 int ntiles, int k, hipStream_t s, void* e0, void* e1);
+#define AERO_DECL_SPLIT(NAME) \
+  hipError_t launch_split_##NAME(int st, bool thermal, const StepParams& p, hipStream_t s);
+AERO_DECL_BATCH(d2q9_bgk)
+AERO_DECL_BATCH(d2q9_trt)
+AERO_DECL_BATCH(d2q9_mrt)
+AERO_DECL_BATCH(d3q19_bgk)
+AERO_DECL_BATCH(d3q19_trt)
+AERO_DECL_BATCH(d3q19_mrt)
+AERO_DECL_SPLIT(d2q9_bgk)
+AERO_DECL_SPLIT(d2q9_trt)
+AERO_DECL_SPLIT(d3q19_bgk)
+AERO_DECL_SPLIT(d3q19_trt)
+#undef AERO_DECL_BATCH
+#undef AERO_DECL_SPLIT
+
+// Step k of a chunk for a group of cases sharing lattice, collision, storage
+// and scalar transport: one launch over `ntiles` packed blocks.  Events, when
+// given, are recorded at the kernel's start and end.
+hipError_t launch_batch_step(int q, int coll, int st, bool thermal, const BatchCase* cases, const int2* tiles,
+                             int ntiles, int k, hipStream_t s, void* ev_start, void* ev_stop);
+// Split step of one case (LES finite-difference pass follows; BGK / TRT).
+hipError_t launch_split_step(int q, int coll, int st, bool thermal, const StepParams& p, hipStream_t s);
+// Last-step force reduction of many cases of one lattice: block b reduces rs[b].
+hipError_t launch_reduce_forces_many(int q, const ForceReduce* rs, int n, hipStream_t s);
 hipError_t launch_materialize(int q, int st, bool thermal, const StepParams& p, hipStream_t s);
 hipError_t launch_collide(int q, int coll, int st, const CollideParams& p, hipStream_t s);
 hipError_t launch_recycling_plane(int st, const StepParams& p, double* out, hipStream_t s);

@@ -190,35 +190,45 @@ constexpr int NPART = 10;  // per link cell: rho, u xyz of f_pre; F_pre xyz; F_n
 constexpr int NCHECK = 5;  // nan, nonpositive rho, max|u|, sum rho, count
 constexpr int BLOCK = 256;
 
+// Pointers the step kernel streams through are declared in the global
+// address space on the device side (plain pointers in host code; same
+// layout), so accesses compile to global rather than flat instructions also
+// when the parameters are read from a table in memory.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define AERO_GLOBAL __attribute__((address_space(1)))
+#else
+#define AERO_GLOBAL
+#endif
+
 // Everything one step launch needs; passed by value as the kernel argument.
 struct StepParams {
   int nx, ny, nz;
   int nrow;  // nz * ny
   long long ncell;
   int nblocks;  // blocks covering ncell (the step launch adds one reduction block)
-  const void* fin;
-  void* fout;
-  const void* gin;
-  void* gout;
-  void* fsplit;  // split mode: post-BC populations, compute type, Q x N
-  void* usplit;  // split mode: ux, uy, uz, compute type, 3 x N
-  const uint8_t* flags;
-  const LinkCell* lrec;
-  const int* rec_of;   // per cell: its link record (read only where FLAG_LINKS is set)
-  const float* qdir;   // Q-1 wall fractions per link record
-  double* fpart;             // NPART doubles per link cell, this step
-  const double* fpart_prev;  // the previous step's sums, reduced by the extra block
-  double* hist_prev;         // coefficient row the previous step's sums go to
+  const AERO_GLOBAL void* fin;
+  AERO_GLOBAL void* fout;
+  const AERO_GLOBAL void* gin;
+  AERO_GLOBAL void* gout;
+  AERO_GLOBAL void* fsplit;  // split mode: post-BC populations, compute type, Q x N
+  AERO_GLOBAL void* usplit;  // split mode: ux, uy, uz, compute type, 3 x N
+  const AERO_GLOBAL uint8_t* flags;
+  const AERO_GLOBAL LinkCell* lrec;
+  const AERO_GLOBAL int* rec_of;   // per cell: its link record (read only where FLAG_LINKS is set)
+  const AERO_GLOBAL float* qdir;   // Q-1 wall fractions per link record
+  AERO_GLOBAL double* fpart;             // NPART doubles per link cell, this step
+  const AERO_GLOBAL double* fpart_prev;  // the previous step's sums, reduced by the extra block
+  AERO_GLOBAL double* hist_prev;         // coefficient row the previous step's sums go to
   int reduce_prev;
   int nrec;  // link cells
   double fdenom, mdenom;
-  const double* prev_in;
-  double* prev_out;
-  const double* sigma;  // sponge sigma(x), nx entries, or null
-  const double* plane;  // inlet planes, 3 x nrow
-  const double* recyc;  // recycling: rho, ux, uy, uz per row-plane cell (4 x nrow), then scale
-  const double* minv_d;
-  const float* minv_f;
+  const AERO_GLOBAL double* prev_in;
+  AERO_GLOBAL double* prev_out;
+  const AERO_GLOBAL double* sigma;  // sponge sigma(x), nx entries, or null
+  const AERO_GLOBAL double* plane;  // inlet planes, 3 x nrow
+  const AERO_GLOBAL double* recyc;  // recycling: rho, ux, uy, uz per row-plane cell (4 x nrow), then scale
+  const AERO_GLOBAL double* minv_d;
+  const AERO_GLOBAL float* minv_f;
   double mrt_s[19];
   double omega, trt_sm, trt_lambda, u0, u_conv;
   double inlet_amp, phase;
@@ -232,19 +242,50 @@ struct StepParams {
   int check_z;   // materialize/check: restrict to this z plane (-1: all)
   void* ev_start;  // host side only: events around the step launch (timing)
   void* ev_stop;
-  double* fmat;
-  double* check_part;
+  AERO_GLOBAL double* fmat;
+  AERO_GLOBAL double* check_part;
+};
+
+// One case of a batched step launch: its step parameters for even and odd
+// steps of the chunk (fin/fout, g, outlet state and force buffers swapped),
+// plus what changes every step.
+struct BatchCase {
+  StepParams p[2];
+  long long step0;        // step count before the chunk
+  const AERO_GLOBAL double* planes;   // per-step inlet planes (3 x nrow doubles each) or null
+  AERO_GLOBAL double* hist;           // NCOEF doubles per step of the chunk
+  int first_g_skip;       // skip the scalar wall values on the chunk's first step
+  int pad;
 };
 
 // Inputs of the per-step force reduction (one block, fixed order).
 struct ForceReduce {
   int q;               // lattice (9 or 19)
-  const double* part;  // NPART doubles per link cell
-  const LinkCell* lrec;
+  const AERO_GLOBAL double* part;  // NPART doubles per link cell
+  const AERO_GLOBAL LinkCell* lrec;
   int nrec, nx, ny;
   double cx, cy, cz;  // moment reference centre
   double fd, md;      // force and moment denominators
-  double* row;        // NCOEF doubles out
+  AERO_GLOBAL double* row;        // NCOEF doubles out
 };
+
+// Host-side construction (casts plain device pointers to the struct's types).
+inline ForceReduce make_force_reduce(int q, const double* part, const LinkCell* lrec, int nrec, int nx, int ny,
+                                     double cx, double cy, double cz, double fd, double md, double* row) {
+  ForceReduce r;
+  r.q = q;
+  r.part = (decltype(r.part))part;
+  r.lrec = (decltype(r.lrec))lrec;
+  r.nrec = nrec;
+  r.nx = nx;
+  r.ny = ny;
+  r.cx = cx;
+  r.cy = cy;
+  r.cz = cz;
+  r.fd = fd;
+  r.md = md;
+  r.row = (decltype(r.row))row;
+  return r;
+}
 
 }  // namespace aero

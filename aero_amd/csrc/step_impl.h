@@ -24,6 +24,29 @@
 
 namespace aero {
 
+// Tell the compiler a pointer addresses global memory (not LDS, not
+// scratch).  Pointers read from a parameter table in memory are otherwise
+// generic to it and their accesses compile to flat instructions.
+template <class T>
+__device__ __forceinline__ void assume_global(const T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_assume(!__builtin_amdgcn_is_shared((const void*)p) && !__builtin_amdgcn_is_private((const void*)p));
+#endif
+}
+
+// The parts of a step's parameters that change every step (kept apart so
+// the batched kernel can read the rest straight from the case table).
+struct StepDyn {
+  double phase;                     // travelling inlet sine phase, 0.17 * step number
+  const AERO_GLOBAL double* plane;  // this step's inlet planes or null
+  AERO_GLOBAL double* hist_prev;    // coefficient row of the previous step
+  int g_bc_skip;         // skip the scalar wall values (state just set from a raw field)
+  int reduce_prev;       // block 0 reduces the previous step's force sums
+};
+__device__ __forceinline__ StepDyn dyn_of(const StepParams& p) {
+  return StepDyn{p.phase, p.plane, p.hist_prev, p.g_bc_skip, p.reduce_prev};
+}
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 template <class C>
@@ -252,7 +275,7 @@ __device__ __forceinline__ void body_force(const StepParams& p, C fn[L::Q]) {
 
 // ---------------------------------------------------------------- inlets
 template <class C>
-__device__ __forceinline__ void inlet_2d(const StepParams& p, int y, C f[9]) {
+__device__ __forceinline__ void inlet_2d(const StepParams& p, const StepDyn& dyn, int y, C f[9]) {
   if (p.inlet_mode == IN_ZOUHE) {  // boundary.py:15-45
     const C u0 = (C)p.u0;
     const C rho = ((f[0] + f[2] + f[4]) + C(2) * (f[3] + f[6] + f[7])) / (C(1) - u0);
@@ -261,14 +284,14 @@ __device__ __forceinline__ void inlet_2d(const StepParams& p, int y, C f[9]) {
     f[8] = f[6] + C(0.5) * (f[2] - f[4]) + C(1.0 / 6.0) * rho * u0;
     if (p.inlet_amp > 0.0) {
       const double ny = p.ny > 1 ? p.ny : 1;
-      const C uy = (C)(p.inlet_amp * p.u0 * sin(2.0 * M_PI * (double)y / ny + p.phase));
+      const C uy = (C)(p.inlet_amp * p.u0 * sin(2.0 * M_PI * (double)y / ny + dyn.phase));
       f[2] += uy * C(0.25);
       f[4] -= uy * C(0.25);
       f[5] += uy * C(0.15);
       f[8] -= uy * C(0.15);
     }
   } else if (p.inlet_mode == IN_FIELD) {  // boundary.py:48-65
-    const C ux_t = (C)p.plane[y], uy_t = (C)p.plane[p.nrow + y];
+    const C ux_t = (C)dyn.plane[y], uy_t = (C)dyn.plane[p.nrow + y];
     C den = C(1) - ux_t;
     if (den < C(1e-8)) den = C(1e-8);
     const C rho = ((f[0] + f[2] + f[4]) + C(2) * (f[3] + f[6] + f[7])) / den;
@@ -286,7 +309,7 @@ __device__ __forceinline__ void inlet_2d(const StepParams& p, int y, C f[9]) {
 }
 
 template <class C>
-__device__ __forceinline__ void inlet_3d(const StepParams& p, int y, int z, C f[19]) {
+__device__ __forceinline__ void inlet_3d(const StepParams& p, const StepDyn& dyn, int y, int z, C f[19]) {
   const int row = z * p.ny + y;
   if (p.inlet_mode == IN_RECYCLING) {  // boundary3d.py:172-198
     const double scale = p.recyc[4 * p.nrow];
@@ -314,23 +337,23 @@ __device__ __forceinline__ void inlet_3d(const StepParams& p, int y, int z, C f[
     cz = czb;
     if (p.inlet_amp > 0.0) {
       const double nz = p.nz > 1 ? p.nz : 1;
-      const C uz = (C)(p.inlet_amp * p.u0 * sin(2.0 * M_PI * (double)z / nz + p.phase));
+      const C uz = (C)(p.inlet_amp * p.u0 * sin(2.0 * M_PI * (double)z / nz + dyn.phase));
       cz = cz + rho * uz;
     }
   } else if (p.inlet_mode == IN_FIELD) {  // boundary3d.py:71-93
-    ux = (C)p.plane[row];
+    ux = (C)dyn.plane[row];
     C den = C(1) - ux;
     if (den < C(1e-8)) den = C(1e-8);
     rho = (C(2) * Fm + Fn) / den;
-    cy = rho * (C)p.plane[p.nrow + row];
-    cz = rho * (C)p.plane[2 * p.nrow + row];
+    cy = rho * (C)dyn.plane[p.nrow + row];
+    cz = rho * (C)dyn.plane[2 * p.nrow + row];
   } else if (p.inlet_mode == IN_SEM) {  // boundary3d.py:96-137
-    ux = (C)p.u0 + (C)p.plane[row];
+    ux = (C)p.u0 + (C)dyn.plane[row];
     C den = C(1) - ux;
     if (den < C(1e-10)) den = C(1e-10);
     rho = (C(2) * Fm + Fn) / den;
-    cy = cyb - rho * (C)p.plane[p.nrow + row];
-    cz = czb - rho * (C)p.plane[2 * p.nrow + row];
+    cy = cyb - rho * (C)dyn.plane[p.nrow + row];
+    cz = czb - rho * (C)dyn.plane[2 * p.nrow + row];
   } else {
     return;
   }
@@ -342,9 +365,9 @@ __device__ __forceinline__ void inlet_3d(const StepParams& p, int y, int z, C f[
 }
 
 template <class L, class C>
-__device__ __forceinline__ void apply_inlet(const StepParams& p, int y, int z, C fn[L::Q]) {
-  if constexpr (L::DIM == 2) inlet_2d<C>(p, y, fn);
-  else inlet_3d<C>(p, y, z, fn);
+__device__ __forceinline__ void apply_inlet(const StepParams& p, const StepDyn& dyn, int y, int z, C fn[L::Q]) {
+  if constexpr (L::DIM == 2) inlet_2d<C>(p, dyn, y, fn);
+  else inlet_3d<C>(p, dyn, y, z, fn);
 }
 
 // Sponge relaxation (sponge.py:24-70) and the y-wall conditions
@@ -408,7 +431,8 @@ __device__ __forceinline__ void sponge_and_walls(const StepParams& p, int x, int
 // outlet and the recycling inlet read at x = nx-2 (boundary.py:68-113,
 // boundary3d.py:185-190).  Returns the cell's link record or -1.
 template <class L, int ST>
-__device__ __forceinline__ int pre_outlet(const StepParams& p, const typename Storage<ST>::T* __restrict__ B, int cell,
+__device__ __forceinline__ int pre_outlet(const StepParams& p, const StepDyn& dyn,
+                                          const typename Storage<ST>::T* __restrict__ B, int cell,
                                           int x, int y, int z, uint8_t flag, typename Storage<ST>::C fn[L::Q]) {
   using C = typename Storage<ST>::C;
   gather<L, ST>(p, B, x, y, z, C(1), fn);
@@ -418,7 +442,7 @@ __device__ __forceinline__ int pre_outlet(const StepParams& p, const typename St
     rec = link_record(p, cell);
     apply_links_light<L, ST>(p, B, rec, cell, x, y, z, fn);
   }
-  if (x == 0) apply_inlet<L, C>(p, y, z, fn);
+  if (x == 0) apply_inlet<L, C>(p, dyn, y, z, fn);
   return rec;
 }
 
@@ -429,7 +453,8 @@ __device__ __forceinline__ int pre_outlet(const StepParams& p, const typename St
 // instead of its own.  With commit set the convective outlet's state is
 // advanced.  Returns the cell's own link record or -1.
 template <class L, int ST>
-__device__ __forceinline__ int post_bc(const StepParams& p, const typename Storage<ST>::T* __restrict__ B, int cell,
+__device__ __forceinline__ int post_bc(const StepParams& p, const StepDyn& dyn,
+                                       const typename Storage<ST>::T* __restrict__ B, int cell,
                                        int x, int y, int z, uint8_t flag, bool commit,
                                        typename Storage<ST>::C fn[L::Q]) {
   using C = typename Storage<ST>::C;
@@ -439,7 +464,7 @@ __device__ __forceinline__ int post_bc(const StepParams& p, const typename Stora
   const int xg = redirect ? p.nx - 2 : x;
   const int cg = redirect ? cell - 1 : cell;
   const uint8_t fg = redirect ? p.flags[cg] : flag;
-  int rec = pre_outlet<L, ST>(p, B, cg, xg, y, z, fg, fn);
+  int rec = pre_outlet<L, ST>(p, dyn, B, cg, xg, y, z, fg, fn);
   if (redirect) {
     if (p.outlet_mode == OUT_CONVECTIVE) {  // boundary.py:78-113
       const double u = p.u_conv < 0.0 ? 0.0 : (p.u_conv > 1.0 ? 1.0 : p.u_conv);
@@ -472,7 +497,8 @@ __device__ __forceinline__ int post_bc(const StepParams& p, const typename Stora
 // Dirichlet values (unless the state was just set from a raw field), adds the
 // buoyancy force to fn and, with commit, writes the new post-collision g.
 template <class L, int ST>
-__device__ __forceinline__ void scalar_step(const StepParams& p, int cell, int x, int y, int z, bool solid,
+__device__ __forceinline__ void scalar_step(const StepParams& p, const StepDyn& dyn, int cell, int x, int y, int z,
+                                            bool solid,
                                             typename Storage<ST>::C fn[L::Q], bool write_g) {
   using S = Storage<ST>;
   using C = typename S::C;
@@ -480,7 +506,7 @@ __device__ __forceinline__ void scalar_step(const StepParams& p, int cell, int x
   const size_t N = (size_t)p.ncell;
   C g[Q];
   gather<L, ST>(p, (const typename S::T*)p.gin, x, y, z, (C)p.gshift, g);
-  if (!p.g_bc_skip) {
+  if (!dyn.g_bc_skip) {
     if (y == 0) {
 #pragma unroll
       for (int i = 0; i < Q; ++i) g[i] = C(L::W[i]) * (C)p.T_hot;
@@ -695,34 +721,35 @@ __device__ __forceinline__ void reduce_forces_block(const ForceReduce& r) {
 // block (index 0, dispatched first so it overlaps the cell blocks) reduces the
 // previous step's force sums, written by the previous launch, into that
 // step's coefficient row.
-// Occupancy target of the step kernel: the 4-byte and 2-byte BGK/TRT forms
-// ask the register allocator for 7 waves per SIMD (at most 72 VGPRs; the
-// common pull + collide path needs ~56, the rare link / outlet / reduction
-// paths would otherwise raise the whole kernel to ~80).  FP64 and MRT keep
-// the compiler's choice.
+// Occupancy target of the step kernel: the FP32 BGK/TRT forms ask the
+// register allocator for 7 waves per SIMD (at most 72 VGPRs; the common pull
+// + collide path needs ~56, the rare link / outlet / reduction paths would
+// otherwise raise the whole kernel to ~80), the 2-byte forms for 6 (their
+// conversions need a few more registers; at 7 they spill heavily).  FP64 and
+// MRT keep the compiler's choice.
 template <int COLL, int ST>
 struct StepWaves {
-  static constexpr int value = (ST != ST_F64 && COLL != COLL_MRT) ? 7 : 1;
+  static constexpr int value = (ST != ST_F64 && COLL != COLL_MRT) ? (ST == ST_F32 ? 7 : 6) : 1;
 };
 
 template <class L, int COLL, int ST, bool THERMAL, int KMODE>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(StepWaves<COLL, ST>::value)))
-void step_kernel(StepParams p) {
+__device__ __forceinline__ void step_body(const StepParams& p, const StepDyn& dyn, int blk) {
   using S = Storage<ST>;
   using C = typename S::C;
   using T = typename S::T;
   constexpr int Q = L::Q;
-  if (blockIdx.x == 0) {
-    if (p.reduce_prev) {
+  if (blk == 0) {
+    if (dyn.reduce_prev) {
       const ForceReduce r{L::Q, p.fpart_prev, p.lrec, p.nrec, p.nx, p.ny, p.center[0], p.center[1], p.center[2],
-                          p.fdenom, p.mdenom, p.hist_prev};
+                          p.fdenom, p.mdenom, dyn.hist_prev};
       reduce_forces_block<L>(r);
     }
     return;
   }
-  const long long c64 = (long long)(blockIdx.x - 1) * BLOCK + threadIdx.x;
-  if (c64 >= p.ncell) return;
-  const int cell = (int)c64;
+  // cell indices fit in 31 bits (checked when the context is created)
+  const unsigned cu = (unsigned)(blk - 1) * (unsigned)BLOCK + threadIdx.x;
+  if (cu >= (unsigned)p.ncell) return;
+  const int cell = (int)cu;
   const int x = cell % p.nx;
   const int yz = cell / p.nx;
   const int y = yz % p.ny;
@@ -733,10 +760,47 @@ void step_kernel(StepParams p) {
   if (flag & FLAG_LINKS) link_pre_terms<L, ST>(p, B, link_record(p, cell), cell);
   __builtin_amdgcn_sched_barrier(0);
   C fn[Q];
-  const int rec = post_bc<L, ST>(p, B, cell, x, y, z, flag, true, fn);
-  if constexpr (THERMAL) scalar_step<L, ST>(p, cell, x, y, z, solid, fn, true);
+  const int rec = post_bc<L, ST>(p, dyn, B, cell, x, y, z, flag, true, fn);
+  if constexpr (THERMAL) scalar_step<L, ST>(p, dyn, cell, x, y, z, solid, fn, true);
   if (rec >= 0) link_post_sum<L, C>(p, rec, fn);
   finish_cell<L, COLL, ST, KMODE>(p, cell, solid, fn);
+}
+
+// One case, parameters by value (split LES step, recycling inlet).
+template <class L, int COLL, int ST, bool THERMAL, int KMODE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(StepWaves<COLL, ST>::value)))
+void step_kernel(StepParams p) {
+  step_body<L, COLL, ST, THERMAL, KMODE>(p, dyn_of(p), blockIdx.x);
+}
+
+// Step k of a chunk for every case of a packed tile list: tile b is block
+// `y` of case `x` (block 0 of a case reduces its previous step's forces).
+// Each case's parameters sit in device memory for both parities of k (the
+// ping-pong buffers swap every step); the per-step fields are derived here.
+// A single case runs through this same kernel as a batch of one, so a case
+// computes the same bits alone or among others.
+template <class L, int COLL, int ST, bool THERMAL>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(StepWaves<COLL, ST>::value)))
+void batch_step_kernel(const BatchCase* __restrict__ cases, const int2* __restrict__ tiles, int k) {
+  const int2 t = tiles[blockIdx.x];
+  const BatchCase& bc = cases[t.x];
+  const StepParams& p = bc.p[k & 1];
+  const StepDyn dyn{0.17 * (double)(bc.step0 + k + 1), bc.planes ? bc.planes + (size_t)3 * p.nrow * k : nullptr,
+                    bc.hist + (size_t)NCOEF * (k > 0 ? k - 1 : 0), k == 0 ? bc.first_g_skip : 0,
+                    (k > 0 && p.nrec > 0) ? 1 : 0};
+  step_body<L, COLL, ST, THERMAL, 0>(p, dyn, t.y);
+}
+
+template <class L, int COLL, int ST, bool THERMAL>
+hipError_t launch_batch(const BatchCase* cases, const int2* tiles, int ntiles, int k, hipStream_t s,
+                        void* ev_start, void* ev_stop) {
+  const dim3 grid((unsigned)ntiles);
+  if (ev_start)
+    hipExtLaunchKernelGGL(batch_step_kernel<L, COLL, ST, THERMAL>, grid, dim3(BLOCK), 0, s, (hipEvent_t)ev_start,
+                          (hipEvent_t)ev_stop, 0, cases, tiles, k);
+  else
+    batch_step_kernel<L, COLL, ST, THERMAL><<<grid, BLOCK, 0, s>>>(cases, tiles, k);
+  return hipGetLastError();
 }
 
 // One step = one launch of nblocks + 1 blocks; with timing events set they
@@ -775,8 +839,9 @@ __global__ __launch_bounds__(BLOCK) void materialize_kernel(StepParams p) {
     if (in_plane) {
       const uint8_t flag = p.flags[cell];
       C fn[Q];
-      post_bc<L, ST>(p, (const T*)p.fin, cell, x, y, z, flag, false, fn);
-      if constexpr (THERMAL) scalar_step<L, ST>(p, cell, x, y, z, flag & FLAG_SOLID, fn, false);
+      const StepDyn dyn = dyn_of(p);
+      post_bc<L, ST>(p, dyn, (const T*)p.fin, cell, x, y, z, flag, false, fn);
+      if constexpr (THERMAL) scalar_step<L, ST>(p, dyn, cell, x, y, z, flag & FLAG_SOLID, fn, false);
       const size_t N = (size_t)p.ncell;
       if (p.out_mode == 0) {
 #pragma unroll

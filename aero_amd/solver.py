@@ -185,6 +185,15 @@ class Solver(DeviceSolver):
         self._last_cd_v = float(last[7])
         self._last_cm = float(last[5])
 
+    def _record(self, rows: np.ndarray):
+        """Append the coefficient rows of a run of steps to the histories."""
+        self.Cd_history.extend(rows[:, 0].tolist())
+        self.Cl_history.extend(rows[:, 1].tolist())
+        self.Cm_history.extend(rows[:, 5].tolist())
+        self.Cd_p_history.extend(rows[:, 6].tolist())
+        self.Cd_v_history.extend(rows[:, 7].tolist())
+        self._consume(rows)
+
     @property
     def _last_force_profile(self):
         cells, forces = self._link_forces()
@@ -217,13 +226,7 @@ class Solver(DeviceSolver):
         warn_hdf5(hdf5_path)
         avg_window = max(1, steps // 5)
 
-        def on_chunk(rows):
-            self.Cd_history.extend(rows[:, 0].tolist())
-            self.Cl_history.extend(rows[:, 1].tolist())
-            self.Cm_history.extend(rows[:, 5].tolist())
-            self.Cd_p_history.extend(rows[:, 6].tolist())
-            self.Cd_v_history.extend(rows[:, 7].tolist())
-            self._consume(rows)
+        on_chunk = self._record
 
         def on_check(step, total, elapsed):
             self._device_check(step)

@@ -199,6 +199,15 @@ class Solver3D(DeviceSolver):
                 np.add.at(prof[k], z, forces[:, col])
         return {"z": list(range(self.Nz)), **{k: v.tolist() for k, v in prof.items()}}
 
+    _HISTS = ("Cd_history", "Cly_history", "Clz_history", "Cmx_history", "Cmy_history", "Cmz_history",
+              "Cd_p_history", "Cd_v_history")
+
+    def _record(self, rows: np.ndarray):
+        """Append the coefficient rows of a run of steps to the histories."""
+        for c, name in enumerate(self._HISTS):
+            getattr(self, name).extend(rows[:, c].tolist())
+        self._consume(rows)
+
     def _step(self) -> tuple:
         """One reference step (solver3d.py:325-541); returns (Cd, Cl_y, Cl_z)."""
         rows = self._advance(1)
@@ -218,14 +227,7 @@ class Solver3D(DeviceSolver):
             print(f"  Collision : {self.collision}")
         warn_hdf5(hdf5_path)
         avg_window = max(1, steps // 5)
-        hists = ("Cd_history", "Cly_history", "Clz_history", "Cmx_history", "Cmy_history", "Cmz_history",
-                 "Cd_p_history", "Cd_v_history")
-        cols = (0, 1, 2, 3, 4, 5, 6, 7)
-
-        def on_chunk(rows):
-            for name, c in zip(hists, cols):
-                getattr(self, name).extend(rows[:, c].tolist())
-            self._consume(rows)
+        on_chunk = self._record
 
         def on_check(step, total, elapsed):
             self._device_check(step)  # z-midplane, as solver3d.py:608-615

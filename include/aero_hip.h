@@ -157,6 +157,22 @@ int aero_get_timing(aero_ctx* ctx, double* kernel_ms, int64_t* launches);
 /* Cell counts: total cells and link cells (fluid cells with a surface link). */
 int aero_cell_counts(aero_ctx* ctx, int64_t* total, int64_t* link_cells);
 
+/* Batched stepping.  aero_batch_step advances n contexts of one device by
+ * nsteps steps each with one kernel launch per step for every group of
+ * contexts sharing lattice, collision, storage and scalar transport (a packed
+ * tile list over all of them), on the batch's own stream.  A context computes
+ * the same bits in a batch as through aero_step, which runs the same kernel
+ * with a batch of one.  step0s[i], planes[i] and hists[i] mean what they do
+ * for aero_step (planes / hists may be NULL, or hold NULL entries).  LES
+ * contexts (finite-difference pass every step) are rejected: step them with
+ * aero_step.  No counterpart in the reference, which runs one case per
+ * process; this is the device side of the batched / serving path. */
+typedef struct aero_batch aero_batch;
+int aero_batch_create(aero_batch** out);
+int aero_batch_destroy(aero_batch* batch);
+int aero_batch_step(aero_batch* batch, int32_t n, aero_ctx* const* ctxs, int32_t nsteps, const int64_t* step0s,
+                    const double* const* planes, double* const* hists);
+
 #ifdef __cplusplus
 }
 #endif
