@@ -674,6 +674,15 @@ int aero_destroy(aero_ctx* c) {
   return 0;
 }
 
+int aero_mem_info(int64_t* free_bytes, int64_t* total_bytes) {
+  return guarded([&] {
+    size_t f = 0, t = 0;
+    ck(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
+  });
+}
+
 // ------------------------------------------------------------------ batch
 struct aero_batch {
   int device = 0;

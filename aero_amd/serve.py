@@ -1,0 +1,350 @@
+"""Continuous-batching serving loop for wind-tunnel cases.
+
+Requests arrive over time (each one a case: lattice, grid, geometry,
+Reynolds number, collision model, storage, optional passive scalar).  The
+server keeps every admitted case's populations resident on the device and
+advances all live cases together, ``chunk`` steps at a time, through
+``Batch.advance``: one kernel launch per step for each group of cases that
+share a kernel.  Between chunks it
+
+* retires every case that met the reference's auto-stop criterion
+  (``detect_statistical_stationarity``, aero/diagnostics.py:197-228, checked
+  every ``chunk`` = check_every steps as in the reference run loop,
+  aero/lbm/solver.py:517-570), reached its step limit, or diverged (a
+  non-finite drag coefficient);
+* admits waiting requests whose arrival time has passed while the live set
+  stays under ``max_live`` cases and the device keeps ``mem_reserve`` bytes
+  free after the new case's estimated footprint.
+
+A case's trajectory does not depend on what else is live (the batched
+kernel computes each case's bits exactly as alone) and its stop decisions
+look only at its own history, so every result is the same alone and in the
+mix.  Timestamps give time-to-result (completion minus arrival) per request.
+The reference has no counterpart: it runs one case per process.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+from .diagnostics import detect_statistical_stationarity
+
+
+# ------------------------------------------------------------------ cases
+@dataclass
+class CaseSpec:
+    """One wind-tunnel case, built into a solver on admission."""
+
+    dim: int = 2
+    nx: int = 256
+    ny: int = 128
+    nz: int = 1
+    shape: str = "cylinder"          # cylinder | rectangle | polygon | sphere | box
+    size: tuple = (16.0,)            # radius / (w, h) / circumradius / (w, h, d)
+    vertices: Optional[list] = None  # polygon vertices, fractional coordinates
+    Re: float = 100.0
+    u0: float = 0.05
+    collision: str = "bgk"
+    storage: str = "f32"
+    thermal: bool = False
+    max_steps: int = 30000
+    seed: int = 0
+
+    @property
+    def cells(self) -> int:
+        return self.nx * self.ny * (self.nz if self.dim == 3 else 1)
+
+    def geometry(self):
+        from . import geometry as G
+        if self.dim == 2:
+            if self.shape == "cylinder":
+                return G.Cylinder(radius=self.size[0], cx_frac=0.25, cy_frac=0.5)
+            if self.shape == "rectangle":
+                return G.Rectangle(width=self.size[0], height=self.size[1], cx_frac=0.25, cy_frac=0.5)
+            if self.shape == "polygon":
+                return G.Polygon(self.vertices)
+            raise ValueError(f"unknown 2D shape {self.shape!r}")
+        if self.shape == "sphere":
+            return G.Sphere(radius=self.size[0], cx_frac=0.25)
+        if self.shape == "box":
+            return G.Box(width=self.size[0], height=self.size[1], depth=self.size[2], cx_frac=0.25)
+        raise ValueError(f"unknown 3D shape {self.shape!r}")
+
+    def reference_length(self, solid: np.ndarray) -> float:
+        if self.shape == "polygon":
+            # frontal height in cells (the reference leaves D for polygons to the caller)
+            ys = np.nonzero(solid.any(axis=1))[0]
+            return float(ys[-1] - ys[0] + 1) if ys.size else 1.0
+        return float(self.geometry().reference_length())
+
+    def device_bytes(self) -> int:
+        """Footprint estimate used for admission: two population copies, the
+        scalar's two copies, flags, link index and small per-row state."""
+        q = 9 if self.dim == 2 else 19
+        s = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2}[self.storage]
+        per_cell = 2 * q * s * (2 if self.thermal else 1) + 1 + 4
+        return self.cells * per_cell + (1 << 20)
+
+    def build(self):
+        """The solver for this case, at equilibrium (rho = 1, u = u0) plus
+        seeded 1e-4 transverse velocity noise, as the benchmark configs state."""
+        import aero_amd
+        from .lattice import compute_feq, compute_feq_3d
+        g = self.geometry()
+        rng = np.random.default_rng(self.seed)
+        if self.dim == 2:
+            solid = g.mark_solid(self.ny, self.nx)
+            D = self.reference_length(solid)
+            nu = self.u0 * D / self.Re
+            omega = 1.0 / (3.0 * nu + 0.5)
+            s = aero_amd.Solver(Ny=self.ny, Nx=self.nx, solid=solid, omega=omega, u0=self.u0, D=D,
+                                collision=self.collision, trt_lambda=0.1875, trt_variant="lambda",
+                                outlet_bc="zerogradient", sponge_thickness=max(8, self.nx // 20),
+                                storage=self.storage, thermal=self.thermal, alpha_T=0.01)
+            sh = (self.ny, self.nx)
+            s.f = compute_feq(np.ones(sh), np.full(sh, self.u0), rng.uniform(-1e-4, 1e-4, sh))
+        else:
+            solid = g.mark_solid(self.nz, self.ny, self.nx)
+            D = self.reference_length(solid)
+            nu = self.u0 * D / self.Re
+            omega = 1.0 / (3.0 * nu + 0.5)
+            s = aero_amd.Solver3D(Nz=self.nz, Ny=self.ny, Nx=self.nx, solid=solid, omega=omega, u0=self.u0, D=D,
+                                  collision=self.collision, trt_lambda=0.1875, trt_variant="lambda",
+                                  outlet_bc="zerogradient", sponge_thickness=max(8, self.nx // 20),
+                                  storage=self.storage, thermal=self.thermal, alpha_T=0.01)
+            sh = (self.nz, self.ny, self.nx)
+            s.f = compute_feq_3d(np.ones(sh), np.full(sh, self.u0), rng.uniform(-1e-4, 1e-4, sh),
+                                 rng.uniform(-1e-4, 1e-4, sh))
+        s._f_outlet_prev = s.f[..., -1].copy()
+        return s
+
+
+def c4_cases(n: int = 256, seed: int = 0, max_steps: int = 30000, storage: str = "f32") -> List[CaseSpec]:
+    """The batched 2D workload: nx uniform in [128, 1024] step 32, ny = nx/2;
+    cylinder r in [8, 32], rectangle w, h in [8, 48] or convex polygon with
+    5-9 vertices and circumradius in [10, 30]; Re log-uniform in [20, 200];
+    collision uniform over BGK / TRT / MRT; u0 = 0.05."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        nx = int(rng.choice(np.arange(128, 1025, 32)))
+        ny = nx // 2
+        kind = ("cylinder", "rectangle", "polygon")[int(rng.integers(3))]
+        verts = None
+        if kind == "cylinder":
+            size = (float(rng.uniform(8, 32)),)
+        elif kind == "rectangle":
+            size = (float(rng.uniform(8, 48)), float(rng.uniform(8, 48)))
+        else:
+            nv = int(rng.integers(5, 10))
+            R = float(rng.uniform(10, 30))
+            ang = np.sort(rng.uniform(0, 2 * math.pi, nv))
+            cx, cy = 0.25 * nx, 0.5 * ny
+            verts = [((cx + R * math.cos(a)) / nx, (cy + R * math.sin(a)) / ny) for a in ang]
+            size = (R,)
+        Re = float(math.exp(rng.uniform(math.log(20.0), math.log(200.0))))
+        coll = ("bgk", "trt", "mrt")[int(rng.integers(3))]
+        out.append(CaseSpec(dim=2, nx=nx, ny=ny, shape=kind, size=size, vertices=verts, Re=Re, collision=coll,
+                            storage=storage, max_steps=max_steps, seed=seed * 100003 + k))
+    return out
+
+
+def c5_requests(n: int, rate: float, seed: int = 0, max_steps: int = 30000,
+                storage: str = "f32") -> List[tuple]:
+    """Poisson arrival stream (rate requests/s) of mixed cases: 80% 2D as in
+    the batched workload, 20% 3D spheres / boxes on grids from 64^3 to
+    256x128x128 with Re uniform in [50, 300]; a passive scalar on 25%."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    two_d = iter(c4_cases(n, seed=seed + 1, max_steps=max_steps, storage=storage))
+    grids = [(64, 64, 64), (128, 64, 64), (192, 96, 96), (256, 128, 128)]
+    out = []
+    for k in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        if rng.random() < 0.8:
+            spec = next(two_d)
+        else:
+            nx, ny, nz = grids[int(rng.integers(len(grids)))]
+            if rng.random() < 0.5:
+                spec = CaseSpec(dim=3, nx=nx, ny=ny, nz=nz, shape="sphere", size=(ny / 8.0,))
+            else:
+                h = ny / 6.0
+                spec = CaseSpec(dim=3, nx=nx, ny=ny, nz=nz, shape="box", size=(h, h, h))
+            spec.Re = float(rng.uniform(50, 300))
+            spec.collision = ("bgk", "trt", "mrt")[int(rng.integers(3))]
+            spec.storage = storage
+            spec.max_steps = max_steps
+            spec.seed = seed * 100003 + 50000 + k
+        spec.thermal = bool(rng.random() < 0.25)
+        out.append((t, spec))
+    return out
+
+
+# ---------------------------------------------------------------- serving
+@dataclass
+class Result:
+    rid: int
+    spec: CaseSpec
+    arrival: float
+    admitted: float
+    finished: float
+    steps: int
+    stop_reason: str
+    Cd_mean: float
+    Cl_std: float
+    strouhal: Optional[float]
+    Cd_history: list = field(repr=False, default_factory=list)
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+@dataclass
+class _Live:
+    rid: int
+    spec: CaseSpec
+    solver: object
+    arrival: float
+    admitted: float
+
+
+class Server:
+    """Continuous batching of wind-tunnel cases on one device."""
+
+    def __init__(self, chunk: int = 500, max_live: int = 512, mem_reserve: int = 8 << 30,
+                 convergence_window: int = 5000, convergence_tol: float = 0.01, strouhal_window: int = 4096,
+                 strouhal_tol: float = 0.05, batch=None, free_memory: Optional[Callable[[], int]] = None,
+                 keep_history: bool = False):
+        self.chunk = int(chunk)
+        self.max_live = int(max_live)
+        self.mem_reserve = int(mem_reserve)
+        self.stop_kw = dict(convergence_window=convergence_window, convergence_tol=convergence_tol,
+                            strouhal_window=strouhal_window, strouhal_tol=strouhal_tol)
+        self._batch = batch
+        self._free_memory = free_memory
+        self.keep_history = keep_history
+        self.steps_done = 0      # lattice updates, for aggregate MLUPS
+        self.busy_seconds = 0.0  # time inside batched stepping
+
+    def _batch_obj(self):
+        if self._batch is None:
+            from .batch import Batch
+            self._batch = Batch()
+        return self._batch
+
+    def _free(self) -> int:
+        if self._free_memory is not None:
+            return int(self._free_memory())
+        from . import _abi
+        free = _abi.ctypes.c_int64()
+        total = _abi.ctypes.c_int64()
+        _abi.check(_abi.lib().aero_mem_info(_abi.ctypes.byref(free), _abi.ctypes.byref(total)))
+        return int(free.value)
+
+    def _done(self, lv: _Live) -> Optional[str]:
+        s = lv.solver
+        if not np.isfinite(s.Cd_history[-1]):
+            return "diverged"
+        _, _, stop = detect_statistical_stationarity(s.Cd_history, s.Cl_history if hasattr(s, "Cl_history")
+                                                     else s.Cly_history, s.D, s.u0, **self.stop_kw)
+        if stop:
+            return "converged"
+        if len(s.Cd_history) >= lv.spec.max_steps:
+            return "max_steps"
+        return None
+
+    def _result(self, lv: _Live, reason: str, now: float, upto: int) -> Result:
+        s = lv.solver
+        cd = np.asarray(s.Cd_history[:upto])
+        cl = np.asarray((s.Cl_history if hasattr(s, "Cl_history") else s.Cly_history)[:upto])
+        w = max(1, len(cd) // 5)
+        from .diagnostics import compute_strouhal
+        st = compute_strouhal(cl.tolist(), s.D, s.u0) if reason != "diverged" else None
+        return Result(lv.rid, lv.spec, lv.arrival, lv.admitted, now, upto, reason,
+                      float(np.mean(cd[-w:])), float(np.std(cl[-w:])), st,
+                      cd.tolist() if self.keep_history else [])
+
+    def serve(self, requests: Sequence[tuple], clock: Callable[[], float] = time.perf_counter,
+              sleep: Callable[[float], None] = time.sleep) -> List[Result]:
+        """Serve (arrival_seconds, CaseSpec) requests; returns one Result per
+        request in completion order.
+
+        The host's stop checks of chunk j run while the device advances chunk
+        j+1 (a worker thread holds the device call).  A case found done at
+        chunk j is retired with its history cut at the end of chunk j, exactly
+        what it would have been alone; its speculative chunk is discarded."""
+        from concurrent.futures import ThreadPoolExecutor
+        order = sorted(range(len(requests)), key=lambda i: requests[i][0])
+        waiting = [(requests[i][0], i, requests[i][1]) for i in order]
+        live: List[_Live] = []
+        done: List[Result] = []
+        t0 = clock()
+        pool = ThreadPoolExecutor(max_workers=1)
+        try:
+            while waiting or live:
+                now = clock() - t0
+                free = None
+                while waiting and waiting[0][0] <= now and len(live) < self.max_live:
+                    arr, rid, spec = waiting[0]
+                    need = spec.device_bytes()
+                    if free is None:
+                        free = self._free()
+                    if live and free - need < self.mem_reserve:
+                        break  # wait for a live case to retire
+                    waiting.pop(0)
+                    solver = spec.build()
+                    live.append(_Live(rid, spec, solver, arr, clock() - t0))
+                    free -= need
+                if not live:
+                    sleep(max(0.0, waiting[0][0] - (clock() - t0)))
+                    continue
+                n = self.chunk
+                batch = self._batch_obj()
+                stepping = list(live)
+                tb = clock()
+                fut = pool.submit(batch.advance, [lv.solver for lv in stepping], n, False)
+                # stop checks on the rows recorded so far, while the device steps
+                retire = []
+                for lv in stepping:
+                    if lv.solver.Cd_history:
+                        reason = self._done(lv)
+                        if reason is not None:
+                            retire.append((lv, reason, len(lv.solver.Cd_history)))
+                rows = fut.result()
+                self.busy_seconds += clock() - tb
+                now = clock() - t0
+                gone = set()
+                for lv, reason, upto in retire:
+                    done.append(self._result(lv, reason, now, upto))
+                    gone.add(id(lv))
+                for lv, r in zip(stepping, rows):
+                    if id(lv) in gone:
+                        lv.solver.close()
+                    else:
+                        lv.solver._record(r)
+                        self.steps_done += n * lv.spec.cells
+                live = [lv for lv in live if id(lv) not in gone]
+        finally:
+            pool.shutdown(wait=True)
+        return done
+
+
+def summarize(results: Sequence[Result], wall: float, updates: int, stepping_seconds: Optional[float] = None) -> dict:
+    """p50 / p99 time-to-result, solves per hour and aggregate MLUPS (over
+    the whole wall time, and over the time spent inside batched stepping)."""
+    lat = np.array([r.latency for r in results]) if results else np.zeros(1)
+    reasons = {}
+    for r in results:
+        reasons[r.stop_reason] = reasons.get(r.stop_reason, 0) + 1
+    return {"requests": len(results), "wall_s": round(wall, 3),
+            "solves_per_hour": round(len(results) / wall * 3600.0, 1) if wall > 0 else None,
+            "p50_s": round(float(np.percentile(lat, 50)), 3), "p99_s": round(float(np.percentile(lat, 99)), 3),
+            "MLUPS": round(updates / wall / 1e6, 1) if wall > 0 else None,
+            "stepping_MLUPS": round(updates / stepping_seconds / 1e6, 1) if stepping_seconds else None,
+            "stop_reasons": reasons}

@@ -141,6 +141,67 @@ def copy_calibration(mib=1024, reps=4):
     print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "reps": reps}), flush=True)
 
 
+def _sequential(cases, chunk=500):
+    """Each case alone through the plain single-case API, one after another:
+    Solver.run with the reference's auto-stop checked every `chunk` steps."""
+    out = []
+    t0 = time.perf_counter()
+    updates = 0
+    for c in cases:
+        s = c.build()
+        try:
+            r = s.run(steps=c.max_steps, check_every=chunk, verbose=False, auto_stop=True)
+            reason, cd = r["stop_reason"], np.asarray(r["Cd_history"])
+        except RuntimeError:
+            reason, cd = "diverged", np.asarray(s.Cd_history)
+        updates += s.step_count * c.cells
+        w = max(1, len(cd) // 5)
+        out.append((reason, len(cd), float(np.mean(cd[-w:])) if len(cd) else float("nan")))
+        s.close()
+    return out, time.perf_counter() - t0, updates
+
+
+def serving(full=False):
+    """Batched 2D solves (the 256-case workload; a bounded sample unless
+    `full`), the same cases run one at a time through the plain single-case
+    API, and Poisson arrival streams of mixed 2D/3D requests; returns
+    solves/hour, p50/p99 time-to-result and aggregate MLUPS for each."""
+    from aero_amd import serve
+    out = {}
+    n_cases, max_steps = (256, 30000) if full else (64, 6000)
+    cases = serve.c4_cases(n_cases, seed=0, max_steps=max_steps)
+    srv = serve.Server(chunk=500)
+    t0 = time.perf_counter()
+    res = srv.serve([(0.0, c) for c in cases])
+    wall = time.perf_counter() - t0
+    out["batched_2d"] = dict(serve.summarize(res, wall, srv.steps_done, srv.busy_seconds), cases=n_cases,
+                             max_steps=max_steps)
+    seq, wall_s, upd_s = _sequential(cases)
+    reasons = {}
+    for r in seq:
+        reasons[r[0]] = reasons.get(r[0], 0) + 1
+    out["sequential_2d"] = {"requests": len(seq), "wall_s": round(wall_s, 3),
+                            "solves_per_hour": round(len(seq) / wall_s * 3600.0, 1),
+                            "MLUPS": round(upd_s / wall_s / 1e6, 1), "stop_reasons": reasons}
+    # batched and sequential runs stop at the same step with the same drag
+    by = {r.rid: r for r in res}
+    same = [k for k, r in enumerate(seq) if r[0] != "diverged" and by[k].stop_reason != "diverged"]
+    out["sequential_2d"]["same_stop_step"] = all(seq[k][1] == by[k].steps for k in same)
+    out["sequential_2d"]["max_abs_cd_mean_diff"] = float(max((abs(seq[k][2] - by[k].Cd_mean) for k in same),
+                                                              default=0.0))
+    out["batched_over_sequential_solves_per_hour"] = round(
+        out["batched_2d"]["solves_per_hour"] / out["sequential_2d"]["solves_per_hour"], 2)
+    for rate in ((2.0, 10.0) if not full else (1.0, 5.0, 10.0, 20.0, 50.0)):
+        n = 48 if not full else 200
+        reqs = serve.c5_requests(n, rate=rate, seed=int(rate), max_steps=4000 if not full else 30000)
+        srv = serve.Server(chunk=500)
+        t0 = time.perf_counter()
+        r5 = srv.serve(reqs)
+        out[f"poisson_{rate:g}_per_s"] = dict(serve.summarize(r5, time.perf_counter() - t0, srv.steps_done,
+                                                              srv.busy_seconds), requests_rate_per_s=rate)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +209,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the extra storage forms")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving / batched-solves leg")
+    ap.add_argument("--serve-full", action="store_true",
+                    help="serving leg on the full 256-case workload and the full arrival-rate sweep")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (calibrates the HBM byte counters)")
     a = ap.parse_args()
@@ -216,6 +280,8 @@ def main():
                                           "roofline_frac": round(bpl / (k3 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
             s3.close()
         out["forms"] = extra
+    if not a.no_serve:
+        out["serving"] = serving(full=a.serve_full)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, nx, ny, nz)
     print(json.dumps(out), flush=True)

@@ -168,6 +168,8 @@ int aero_cell_counts(aero_ctx* ctx, int64_t* total, int64_t* link_cells);
  * aero_step.  No counterpart in the reference, which runs one case per
  * process; this is the device side of the batched / serving path. */
 typedef struct aero_batch aero_batch;
+/* Free and total device memory of the current device (admission control). */
+int aero_mem_info(int64_t* free_bytes, int64_t* total_bytes);
 int aero_batch_create(aero_batch** out);
 int aero_batch_destroy(aero_batch* batch);
 int aero_batch_step(aero_batch* batch, int32_t n, aero_ctx* const* ctxs, int32_t nsteps, const int64_t* step0s,
