@@ -163,7 +163,7 @@ __global__ __launch_bounds__(BLOCK) void materialize_g_kernel(StepParams p, doub
   const int cell = (int)c64;
   const int x = cell % p.nx, yz = cell / p.nx, y = yz % p.ny, z = yz / p.ny;
   C g[L::Q];
-  gather<L, ST>(p, (const typename S::T*)p.gin, x, y, z, (C)p.gshift, g);
+  gather<L, ST, 1>(p, (const typename S::T*)p.gin, x, y, z, (C)p.gshift, g);
   const size_t N = (size_t)p.ncell;
 #pragma unroll
   for (int i = 0; i < L::Q; ++i) {

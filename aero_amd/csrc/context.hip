@@ -328,6 +328,7 @@ struct aero_ctx {
     cp.trt_variant = d.trt_variant;
     cp.base_nu = (1.0 / d.omega - 0.5) / 3.0;  // physics.py:27-29
     cp.les_cs = d.les_cs;
+    cp.u0 = d.u0;
     cp.les = U ? d.les : AERO_LES_OFF;
     ck(launch_collide(Q, d.collision, d.storage, cp, stream), "collide launch");
   }

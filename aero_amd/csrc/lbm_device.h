@@ -131,6 +131,17 @@ struct DirPack {
   }
 };
 
+// Storage shift of the half-precision forms for population i of f: the
+// free-stream equilibrium feq_i(rho = 1, u = (u0, 0, 0)), so the stored
+// 10- or 7-bit-mantissa value is the (small) deviation from the free stream
+// wherever the flow is near it.  The scalar g shifts by w_i * T_ref.
+template <class L, class C>
+__host__ __device__ __forceinline__ C fshift(int i, C u0) {
+  const C a = C(1) - C(1.5) * u0 * u0;
+  const int ex = L::EX[i];
+  return C(L::W[i]) * (ex == 0 ? a : a + C(4.5) * u0 * u0 + (ex > 0 ? C(3) * u0 : C(-3) * u0));
+}
+
 // ------------------------------------------------------------------ storage
 // Populations are stored as T and computed in C.  The half forms store the
 // shifted value f - w_i*s (s = 1 for f, T_ref for the scalar g) so the 10/7

@@ -54,10 +54,12 @@ __device__ __forceinline__ C feq_i(C w, C rho, C eu, C usq) {
   return w * rho * (C(1) + C(3) * eu + C(4.5) * eu * eu - C(1.5) * usq);
 }
 
-// Pull gather of all Q populations of cell (x, y, z) from buffer B.
-template <class L, int ST>
+// Pull gather of all Q populations of cell (x, y, z) from buffer B: the
+// populations f (KIND 0, shifted by the free-stream equilibrium in the half
+// forms) or the scalar's g (KIND 1, shifted by w_i * T_ref).
+template <class L, int ST, int KIND = 0>
 __device__ __forceinline__ void gather(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
-                                       int x, int y, int z, typename Storage<ST>::C shift_scale,
+                                       int x, int y, int z, typename Storage<ST>::C shift_scale /* u0 or T_ref */,
                                        typename Storage<ST>::C fn[L::Q]) {
   using S = Storage<ST>;
   using C = typename S::C;
@@ -75,7 +77,7 @@ __device__ __forceinline__ void gather(const StepParams& p, const typename Stora
     const int ys = L::EY[i] == 0 ? y : (L::EY[i] > 0 ? ym : yp);
     const int zs = L::EZ[i] == 0 ? z : (L::EZ[i] > 0 ? zm : zp);
     const unsigned src = ((unsigned)zs * (unsigned)p.ny + (unsigned)ys) * (unsigned)p.nx + (unsigned)xs;
-    fn[i] = S::ld(B + (size_t)i * N, src, C(L::W[i]) * shift_scale);
+    fn[i] = S::ld(B + (size_t)i * N, src, KIND == 0 ? fshift<L, C>(i, shift_scale) : C(L::W[i]) * shift_scale);
   }
 }
 
@@ -118,7 +120,7 @@ __device__ __forceinline__ void apply_links_light(const StepParams& p, const typ
   if (!p.bouzidi) {
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
-      const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, C(L::W[i]));
+      const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, fshift<L, C>(i, (C)p.u0));
       fn[L::OPP[i]] = ((mask >> i) & 1u) ? fi : fn[L::OPP[i]];
     }
     return;
@@ -138,9 +140,9 @@ __device__ __forceinline__ void apply_links_light(const StepParams& p, const typ
     const int yn = clampi(y + L::EY[o], 0, p.ny - 1);
     const int zn = clampi(z + L::EZ[o], 0, p.nz - 1);
     const unsigned nb = ((unsigned)zn * p.ny + yn) * p.nx + xn;
-    const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, C(L::W[i]));
-    const C f_up = S::ld(B + (size_t)i * N, nb, C(L::W[i]));          // f_pre[i](x - e_i)
-    const C f_op = S::ld(B + (size_t)o * N, (unsigned)cell, C(L::W[o]));  // f_pre[OPP[i]](x)
+    const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, fshift<L, C>(i, (C)p.u0));
+    const C f_up = S::ld(B + (size_t)i * N, nb, fshift<L, C>(i, (C)p.u0));          // f_pre[i](x - e_i)
+    const C f_op = S::ld(B + (size_t)o * N, (unsigned)cell, fshift<L, C>(o, (C)p.u0));  // f_pre[OPP[i]](x)
     const C partner = upstream ? f_up : f_op;
     const C a = lo ? C(2) * q : C(1) / (C(2) * q);
     const C v = a * fi + (C(1) - a) * partner;
@@ -168,7 +170,7 @@ __device__ __forceinline__ void link_pre_terms(const StepParams& p, const typena
   const uint32_t mask = p.lrec[rec].mask;
   C fp[Q];
 #pragma unroll
-  for (int i = 0; i < Q; ++i) fp[i] = St::ld(B + (size_t)i * N, (unsigned)cell, C(L::W[i]));
+  for (int i = 0; i < Q; ++i) fp[i] = St::ld(B + (size_t)i * N, (unsigned)cell, fshift<L, C>(i, (C)p.u0));
   double rho = 0, mx = 0, my = 0, mz = 0, fx = 0, fy = 0, fz = 0;
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
@@ -435,7 +437,7 @@ __device__ __forceinline__ int pre_outlet(const StepParams& p, const StepDyn& dy
                                           const typename Storage<ST>::T* __restrict__ B, int cell,
                                           int x, int y, int z, uint8_t flag, typename Storage<ST>::C fn[L::Q]) {
   using C = typename Storage<ST>::C;
-  gather<L, ST>(p, B, x, y, z, C(1), fn);
+  gather<L, ST>(p, B, x, y, z, (C)p.u0, fn);
   if (L::DIM == 3 && p.body_force_on && !(flag & FLAG_SOLID)) body_force<L, C>(p, fn);
   int rec = -1;
   if (flag & FLAG_LINKS) {
@@ -505,7 +507,7 @@ __device__ __forceinline__ void scalar_step(const StepParams& p, const StepDyn& 
   constexpr int Q = L::Q;
   const size_t N = (size_t)p.ncell;
   C g[Q];
-  gather<L, ST>(p, (const typename S::T*)p.gin, x, y, z, (C)p.gshift, g);
+  gather<L, ST, 1>(p, (const typename S::T*)p.gin, x, y, z, (C)p.gshift, g);
   if (!dyn.g_bc_skip) {
     if (y == 0) {
 #pragma unroll
@@ -631,7 +633,7 @@ __device__ __forceinline__ void finish_cell(const StepParams& p, int cell, bool 
     collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, (C)p.omega, (C)p.trt_sm, fn, out);
     T* Bo = (T*)p.fout;
 #pragma unroll
-    for (int i = 0; i < Q; ++i) S::st(Bo + (size_t)i * N, (unsigned)cell, out[i], C(L::W[i]));
+    for (int i = 0; i < Q; ++i) S::st(Bo + (size_t)i * N, (unsigned)cell, out[i], fshift<L, C>(i, (C)p.u0));
   } else {
     C* F = (C*)p.fsplit;
     C* U = (C*)p.usplit;
@@ -923,7 +925,7 @@ struct CollideParams {
   const double* minv_d;
   const float* minv_f;
   double mrt_s[19];
-  double omega, trt_sm, trt_lambda, base_nu, les_cs;
+  double omega, trt_sm, trt_lambda, base_nu, les_cs, u0;
   int les;
   int trt_variant;
 };
@@ -1032,7 +1034,7 @@ __global__ __launch_bounds__(BLOCK) void collide_kernel(CollideParams p) {
   collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, om, sm, fn, out);
   T* Bo = (T*)p.fout;
 #pragma unroll
-  for (int i = 0; i < Q; ++i) S::st(Bo + (size_t)i * N, (unsigned)cell, out[i], C(L::W[i]));
+  for (int i = 0; i < Q; ++i) S::st(Bo + (size_t)i * N, (unsigned)cell, out[i], fshift<L, C>(i, (C)p.u0));
 }
 
 }  // namespace aero

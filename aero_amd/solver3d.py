@@ -273,7 +273,7 @@ class Solver3D(DeviceSolver):
             out[f"{key}_std"] = float(np.std(arr))
         out["Cd_p_mean"] = float(np.mean(a(self.Cd_p_history)))
         out["Cd_v_mean"] = float(np.mean(a(self.Cd_v_history)))
-        for name in hists:
+        for name in self._HISTS:
             out[name] = getattr(self, name)
         out["observables"] = {
             "cd_spectrum": coefficient_spectrum(self.Cd_history),
