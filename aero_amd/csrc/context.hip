@@ -217,6 +217,8 @@ struct aero_ctx {
     p.minv_d = (decltype(p.minv_d))(minv_d.as<double>());
     p.minv_f = (decltype(p.minv_f))(minv_f.as<float>());
     for (int a = 0; a < 19; ++a) p.mrt_s[a] = d.mrt_s[a];
+    for (int a = 0; a < Q; ++a)
+      p.fsh[a] = (float)(Q == 19 ? fshift<D3Q19, double>(a, d.u0) : fshift<D2Q9, double>(a, d.u0));
     p.omega = d.omega;
     p.trt_lambda = d.trt_lambda;
     p.trt_sm = trt_s_minus(d.omega, d.trt_lambda, d.trt_variant);
@@ -329,6 +331,7 @@ struct aero_ctx {
     cp.base_nu = (1.0 / d.omega - 0.5) / 3.0;  // physics.py:27-29
     cp.les_cs = d.les_cs;
     cp.u0 = d.u0;
+    for (int a = 0; a < 19; ++a) cp.fsh[a] = bp.fsh[a];
     cp.les = U ? d.les : AERO_LES_OFF;
     ck(launch_collide(Q, d.collision, d.storage, cp, stream), "collide launch");
   }

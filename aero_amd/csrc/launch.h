@@ -15,6 +15,7 @@ struct CollideParams;
                                  int ntiles, int k, hipStream_t s, void* e0, void* e1);
 #define AERO_DECL_SPLIT(NAME) \
   hipError_t launch_split_##NAME(int st, bool thermal, const StepParams& p, hipStream_t s);
+
 AERO_DECL_BATCH(d2q9_bgk)
 AERO_DECL_BATCH(d2q9_trt)
 AERO_DECL_BATCH(d2q9_mrt)

@@ -241,6 +241,7 @@ struct StepParams {
   const AERO_GLOBAL double* minv_d;
   const AERO_GLOBAL float* minv_f;
   double mrt_s[19];
+  float fsh[19];  // storage shift of f per direction (half forms): feq_i(1, (u0, 0, 0))
   double omega, trt_sm, trt_lambda, u0, u_conv;
   double inlet_amp, phase;
   double rho_in, rho_out, u_top, u_bot;

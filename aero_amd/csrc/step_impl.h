@@ -77,7 +77,7 @@ __device__ __forceinline__ void gather(const StepParams& p, const typename Stora
     const int ys = L::EY[i] == 0 ? y : (L::EY[i] > 0 ? ym : yp);
     const int zs = L::EZ[i] == 0 ? z : (L::EZ[i] > 0 ? zm : zp);
     const unsigned src = ((unsigned)zs * (unsigned)p.ny + (unsigned)ys) * (unsigned)p.nx + (unsigned)xs;
-    fn[i] = S::ld(B + (size_t)i * N, src, KIND == 0 ? fshift<L, C>(i, shift_scale) : C(L::W[i]) * shift_scale);
+    fn[i] = S::ld(B + (size_t)i * N, src, KIND == 0 ? (C)p.fsh[i] : C(L::W[i]) * shift_scale);
   }
 }
 
@@ -120,7 +120,7 @@ __device__ __forceinline__ void apply_links_light(const StepParams& p, const typ
   if (!p.bouzidi) {
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
-      const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, fshift<L, C>(i, (C)p.u0));
+      const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, (C)p.fsh[i]);
       fn[L::OPP[i]] = ((mask >> i) & 1u) ? fi : fn[L::OPP[i]];
     }
     return;
@@ -140,9 +140,9 @@ __device__ __forceinline__ void apply_links_light(const StepParams& p, const typ
     const int yn = clampi(y + L::EY[o], 0, p.ny - 1);
     const int zn = clampi(z + L::EZ[o], 0, p.nz - 1);
     const unsigned nb = ((unsigned)zn * p.ny + yn) * p.nx + xn;
-    const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, fshift<L, C>(i, (C)p.u0));
-    const C f_up = S::ld(B + (size_t)i * N, nb, fshift<L, C>(i, (C)p.u0));          // f_pre[i](x - e_i)
-    const C f_op = S::ld(B + (size_t)o * N, (unsigned)cell, fshift<L, C>(o, (C)p.u0));  // f_pre[OPP[i]](x)
+    const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, (C)p.fsh[i]);
+    const C f_up = S::ld(B + (size_t)i * N, nb, (C)p.fsh[i]);          // f_pre[i](x - e_i)
+    const C f_op = S::ld(B + (size_t)o * N, (unsigned)cell, (C)p.fsh[o]);  // f_pre[OPP[i]](x)
     const C partner = upstream ? f_up : f_op;
     const C a = lo ? C(2) * q : C(1) / (C(2) * q);
     const C v = a * fi + (C(1) - a) * partner;
@@ -170,7 +170,7 @@ __device__ __forceinline__ void link_pre_terms(const StepParams& p, const typena
   const uint32_t mask = p.lrec[rec].mask;
   C fp[Q];
 #pragma unroll
-  for (int i = 0; i < Q; ++i) fp[i] = St::ld(B + (size_t)i * N, (unsigned)cell, fshift<L, C>(i, (C)p.u0));
+  for (int i = 0; i < Q; ++i) fp[i] = St::ld(B + (size_t)i * N, (unsigned)cell, (C)p.fsh[i]);
   double rho = 0, mx = 0, my = 0, mz = 0, fx = 0, fy = 0, fz = 0;
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
@@ -571,25 +571,47 @@ __device__ __forceinline__ void collide(const double* mrt_s, const double* minv_
     uz = C(0);
   }
   const C usq = ux * ux + uy * uy + uz * uz;
-  C fe[Q];
+  // Equilibrium by opposite pairs (i, OPP[i]): with eu = e_i.u the even part
+  // is w_i rho (1 + 4.5 eu^2 - 1.5 u.u) and the odd part w_i rho 3 eu, so
+  // feq_i = even + odd and feq_OPP[i] = even - odd (w_OPP[i] = w_i).
+  const C base = C(1) - C(1.5) * usq;
+  if constexpr (COLL == COLL_BGK || COLL == COLL_TRT) {
+    const C c1 = C(1) - om;  // relaxation of the even part
+    const C c2 = C(1) - sm;  // of the odd part (TRT)
+    out[0] = c1 * fn[0] + om * (C(L::W[0]) * rho * base);
 #pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    fe[i] = feq_i(C(L::W[i]), rho, edot<L>(i, ux, uy, uz), usq);
-  }
-  if constexpr (COLL == COLL_BGK) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) out[i] = (C(1) - om) * fn[i] + om * fe[i];
-  } else if constexpr (COLL == COLL_TRT) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
+    for (int i = 1; i < Q; ++i) {
       const int j = L::OPP[i];
-      const C fp = C(0.5) * (fn[i] + fn[j]);
-      const C fm = C(0.5) * (fn[i] - fn[j]);
-      const C ep = C(0.5) * (fe[i] + fe[j]);
-      const C em = C(0.5) * (fe[i] - fe[j]);
-      out[i] = fe[i] + (C(1) - om) * (fp - ep) + (C(1) - sm) * (fm - em);
+      if (i > j) continue;
+      const C eu = edot<L>(i, ux, uy, uz);
+      const C wr = C(L::W[i]) * rho;
+      const C ev = wr * (base + C(4.5) * eu * eu);
+      const C od = wr * (C(3) * eu);
+      if constexpr (COLL == COLL_BGK) {
+        out[i] = c1 * fn[i] + om * (ev + od);
+        out[j] = c1 * fn[j] + om * (ev - od);
+      } else {
+        // f+- = (f_i +- f_j) / 2; out = feq + (1 - w+)(f+ - even) +- (1 - w-)(f- - odd)
+        const C a = ev + c1 * (C(0.5) * (fn[i] + fn[j]) - ev);
+        const C b = od + c2 * (C(0.5) * (fn[i] - fn[j]) - od);
+        out[i] = a + b;
+        out[j] = a - b;
+      }
     }
   } else {
+    C fe[Q];
+    fe[0] = C(L::W[0]) * rho * base;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const int j = L::OPP[i];
+      if (i > j) continue;
+      const C eu = edot<L>(i, ux, uy, uz);
+      const C wr = C(L::W[i]) * rho;
+      const C ev = wr * (base + C(4.5) * eu * eu);
+      const C od = wr * (C(3) * eu);
+      fe[i] = ev + od;
+      fe[j] = ev - od;
+    }
     C mp[Q];
 #pragma unroll
     for (int a = 0; a < Q; ++a) {
@@ -633,7 +655,7 @@ __device__ __forceinline__ void finish_cell(const StepParams& p, int cell, bool 
     collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, (C)p.omega, (C)p.trt_sm, fn, out);
     T* Bo = (T*)p.fout;
 #pragma unroll
-    for (int i = 0; i < Q; ++i) S::st(Bo + (size_t)i * N, (unsigned)cell, out[i], fshift<L, C>(i, (C)p.u0));
+    for (int i = 0; i < Q; ++i) S::st(Bo + (size_t)i * N, (unsigned)cell, out[i], (C)p.fsh[i]);
   } else {
     C* F = (C*)p.fsplit;
     C* U = (C*)p.usplit;
@@ -723,56 +745,62 @@ __device__ __forceinline__ void reduce_forces_block(const ForceReduce& r) {
 // block (index 0, dispatched first so it overlaps the cell blocks) reduces the
 // previous step's force sums, written by the previous launch, into that
 // step's coefficient row.
-// Occupancy target of the step kernel: the FP32 BGK/TRT forms ask the
-// register allocator for 7 waves per SIMD (at most 72 VGPRs; the common pull
-// + collide path needs ~56, the rare link / outlet / reduction paths would
-// otherwise raise the whole kernel to ~80), the 2-byte forms for 6 (their
-// conversions need a few more registers; at 7 they spill heavily).  FP64 and
-// MRT keep the compiler's choice.
+// Occupancy target of the step kernel: the 4- and 2-byte BGK/TRT forms ask
+// the register allocator for 7 waves per SIMD (at most 72 VGPRs; the common
+// pull + collide path needs ~50, the rare link / outlet / reduction paths
+// would otherwise raise the whole kernel to ~85).  FP64 and MRT keep the
+// compiler's choice.
 template <int COLL, int ST>
 struct StepWaves {
-  static constexpr int value = (ST != ST_F64 && COLL != COLL_MRT) ? (ST == ST_F32 ? 7 : 6) : 1;
+  static constexpr int value = (ST != ST_F64 && COLL != COLL_MRT) ? 7 : 1;
 };
 
-template <class L, int COLL, int ST, bool THERMAL, int KMODE>
-__device__ __forceinline__ void step_body(const StepParams& p, const StepDyn& dyn, int blk) {
-  using S = Storage<ST>;
-  using C = typename S::C;
-  using T = typename S::T;
-  constexpr int Q = L::Q;
-  if (blk == 0) {
-    if (dyn.reduce_prev) {
-      const ForceReduce r{L::Q, p.fpart_prev, p.lrec, p.nrec, p.nx, p.ny, p.center[0], p.center[1], p.center[2],
-                          p.fdenom, p.mdenom, dyn.hist_prev};
-      reduce_forces_block<L>(r);
-    }
-    return;
-  }
-  // cell indices fit in 31 bits (checked when the context is created)
-  const unsigned cu = (unsigned)(blk - 1) * (unsigned)BLOCK + threadIdx.x;
-  if (cu >= (unsigned)p.ncell) return;
-  const int cell = (int)cu;
-  const int x = cell % p.nx;
-  const int yz = cell / p.nx;
-  const int y = yz % p.ny;
-  const int z = yz / p.ny;
-  const uint8_t flag = p.flags[cell];
-  const bool solid = flag & FLAG_SOLID;
-  const T* B = (const T*)p.fin;
-  if (flag & FLAG_LINKS) link_pre_terms<L, ST>(p, B, link_record(p, cell), cell);
-  __builtin_amdgcn_sched_barrier(0);
-  C fn[Q];
-  const int rec = post_bc<L, ST>(p, dyn, B, cell, x, y, z, flag, true, fn);
-  if constexpr (THERMAL) scalar_step<L, ST>(p, dyn, cell, x, y, z, solid, fn, true);
-  if (rec >= 0) link_post_sum<L, C>(p, rec, fn);
-  finish_cell<L, COLL, ST, KMODE>(p, cell, solid, fn);
-}
+// The step of one cell (or, in block 0, the previous step's force
+// reduction), expanded textually into each step kernel with `p` (the case's
+// StepParams), `dyn` (this step's StepDyn) and `blk` (the case-local block
+// index) in scope.  A macro rather than an inlined function: with the body in
+// a callee the register allocator's outcome for the 2-byte storage forms
+// degrades sharply (heavy spilling at the same occupancy target).
+#define AERO_STEP_BODY(L, COLL, ST, THERMAL, KMODE)                                                        \
+  do {                                                                                                     \
+    using S_ = Storage<ST>;                                                                                \
+    using C_ = typename S_::C;                                                                             \
+    using T_ = typename S_::T;                                                                             \
+    if (blk == 0) {                                                                                        \
+      if (dyn.reduce_prev) {                                                                               \
+        const ForceReduce r_{L::Q, p.fpart_prev, p.lrec, p.nrec, p.nx, p.ny, p.center[0], p.center[1],     \
+                             p.center[2], p.fdenom, p.mdenom, dyn.hist_prev};                              \
+        reduce_forces_block<L>(r_);                                                                        \
+      }                                                                                                    \
+      return;                                                                                              \
+    }                                                                                                      \
+    /* cell indices fit in 31 bits (checked when the context is created) */                                \
+    const unsigned cu_ = (unsigned)(blk - 1) * (unsigned)BLOCK + threadIdx.x;                              \
+    if (cu_ >= (unsigned)p.ncell) return;                                                                  \
+    const int cell = (int)cu_;                                                                             \
+    const int x = cell % p.nx;                                                                             \
+    const int yz = cell / p.nx;                                                                            \
+    const int y = yz % p.ny;                                                                               \
+    const int z = yz / p.ny;                                                                               \
+    const uint8_t flag = p.flags[cell];                                                                    \
+    const bool solid = flag & FLAG_SOLID;                                                                  \
+    const T_* B = (const T_*)p.fin;                                                                        \
+    if (flag & FLAG_LINKS) link_pre_terms<L, ST>(p, B, link_record(p, cell), cell);                       \
+    __builtin_amdgcn_sched_barrier(0);                                                                     \
+    C_ fn[L::Q];                                                                                           \
+    const int rec = post_bc<L, ST>(p, dyn, B, cell, x, y, z, flag, true, fn);                              \
+    if constexpr (THERMAL) scalar_step<L, ST>(p, dyn, cell, x, y, z, solid, fn, true);                     \
+    if (rec >= 0) link_post_sum<L, C_>(p, rec, fn);                                                        \
+    finish_cell<L, COLL, ST, KMODE>(p, cell, solid, fn);                                                   \
+  } while (0)
 
 // One case, parameters by value (split LES step, recycling inlet).
 template <class L, int COLL, int ST, bool THERMAL, int KMODE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(StepWaves<COLL, ST>::value)))
 void step_kernel(StepParams p) {
-  step_body<L, COLL, ST, THERMAL, KMODE>(p, dyn_of(p), blockIdx.x);
+  const StepDyn dyn = dyn_of(p);
+  const int blk = blockIdx.x;
+  AERO_STEP_BODY(L, COLL, ST, THERMAL, KMODE);
 }
 
 // Step k of a chunk for every case of a packed tile list: tile b is block
@@ -790,7 +818,8 @@ void batch_step_kernel(const BatchCase* __restrict__ cases, const int2* __restri
   const StepDyn dyn{0.17 * (double)(bc.step0 + k + 1), bc.planes ? bc.planes + (size_t)3 * p.nrow * k : nullptr,
                     bc.hist + (size_t)NCOEF * (k > 0 ? k - 1 : 0), k == 0 ? bc.first_g_skip : 0,
                     (k > 0 && p.nrec > 0) ? 1 : 0};
-  step_body<L, COLL, ST, THERMAL, 0>(p, dyn, t.y);
+  const int blk = t.y;
+  AERO_STEP_BODY(L, COLL, ST, THERMAL, 0);
 }
 
 template <class L, int COLL, int ST, bool THERMAL>
@@ -926,6 +955,7 @@ struct CollideParams {
   const float* minv_f;
   double mrt_s[19];
   double omega, trt_sm, trt_lambda, base_nu, les_cs, u0;
+  float fsh[19];  // storage shift of f per direction (half forms), as in StepParams
   int les;
   int trt_variant;
 };
@@ -1034,7 +1064,7 @@ __global__ __launch_bounds__(BLOCK) void collide_kernel(CollideParams p) {
   collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, om, sm, fn, out);
   T* Bo = (T*)p.fout;
 #pragma unroll
-  for (int i = 0; i < Q; ++i) S::st(Bo + (size_t)i * N, (unsigned)cell, out[i], fshift<L, C>(i, (C)p.u0));
+  for (int i = 0; i < Q; ++i) S::st(Bo + (size_t)i * N, (unsigned)cell, out[i], (C)p.fsh[i]);
 }
 
 }  // namespace aero

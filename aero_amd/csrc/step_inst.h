@@ -40,3 +40,4 @@
     }                                                                                          \
     return hipErrorInvalidValue;                                                               \
   }
+

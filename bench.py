@@ -253,7 +253,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles("step_kernel_d3q19_trt_f32_128x64x64"),
-                     "kernel": "aero::step_kernel<D3Q19, TRT, F32>", "kernel_ms": round(kern_ms, 5),
+                     "kernel": "aero::batch_step_kernel<D3Q19, TRT, F32>", "kernel_ms": round(kern_ms, 5),
                      "algorithmic_bytes_per_launch": bytes_per_launch, "cells": ncell, "link_cells": nlink,
                      "bytes_per_cell": 2 * 19 * 4},
         "cd_last": float(rows[-1, 0]),
