@@ -107,9 +107,52 @@ template <class T>
 __host__ __device__ __forceinline__ T madd(T acc, int e, T v) {
   return e == 0 ? acc : (e == 1 ? acc + v : (e == -1 ? acc - v : acc + T(e) * v));
 }
+// e*v for a lattice component: the first term of a sum, so no 0 + x add is
+// left for the compiler (it may not drop one in IEEE arithmetic).
+template <class T>
+__host__ __device__ __forceinline__ T mfirst(int e, T v) {
+  return e == 1 ? v : (e == -1 ? -v : T(e) * v);
+}
 template <class L, class T>
 __host__ __device__ __forceinline__ T edot(int i, T ux, T uy, T uz) {
-  return madd(madd(madd(T(0), L::EX[i], ux), L::EY[i], uy), L::EZ[i], uz);
+  if (L::EX[i] != 0) return madd(madd(mfirst(L::EX[i], ux), L::EY[i], uy), L::EZ[i], uz);
+  if (L::EY[i] != 0) return madd(mfirst(L::EY[i], uy), L::EZ[i], uz);
+  return L::EZ[i] != 0 ? mfirst(L::EZ[i], uz) : T(0);
+}
+// Density and momentum of one cell's populations; every sum starts from its
+// first nonzero term.
+template <class L, class T>
+__host__ __device__ __forceinline__ void moments(const T f[L::Q], T& rho, T& mx, T& my, T& mz) {
+  rho = f[0];
+  bool sx = false, sy = false, sz = false;
+  mx = my = mz = T(0);
+#pragma unroll
+  for (int i = 1; i < L::Q; ++i) {
+    rho += f[i];
+    if (L::EX[i] != 0) { mx = sx ? madd(mx, L::EX[i], f[i]) : mfirst(L::EX[i], f[i]); sx = true; }
+    if (L::EY[i] != 0) { my = sy ? madd(my, L::EY[i], f[i]) : mfirst(L::EY[i], f[i]); sy = true; }
+    if (L::EZ[i] != 0) { mz = sz ? madd(mz, L::EZ[i], f[i]) : mfirst(L::EZ[i], f[i]); sz = true; }
+  }
+}
+
+// Division of a cell index (below 2^31) by a grid extent d >= 1 as a
+// multiply-high and a shift (Granlund & Montgomery, "Division by invariant
+// integers using multiplication", 1994): with l = ceil(log2 d) and
+// m = ceil(2^(31+l) / d) < 2^32, floor(n/d) = floor(n m / 2^(31+l)) for all
+// n < 2^31.  d = 1 is flagged by m = 0.
+struct DivMagic {
+  unsigned m;
+  unsigned s;  // l - 1
+};
+inline DivMagic make_div_magic(unsigned d) {
+  if (d <= 1) return DivMagic{0u, 0u};
+  unsigned l = 0;
+  while ((1ull << l) < d) ++l;
+  const unsigned long long m = ((1ull << (31 + l)) + d - 1) / d;
+  return DivMagic{(unsigned)m, l - 1};
+}
+__device__ __forceinline__ unsigned udiv(unsigned n, DivMagic dm) {
+  return dm.m ? (__umulhi(n, dm.m) >> dm.s) : n;
 }
 
 // Direction components for a runtime index (loops over the set bits of a
@@ -135,6 +178,13 @@ struct DirPack {
 // free-stream equilibrium feq_i(rho = 1, u = (u0, 0, 0)), so the stored
 // 10- or 7-bit-mantissa value is the (small) deviation from the free stream
 // wherever the flow is near it.  The scalar g shifts by w_i * T_ref.
+// Its value depends only on the weight shell (rest, axis, diagonal) and e_x
+// of the direction, so the kernels keep one value per class: 9 classes.
+constexpr int NSHIFT = 9;
+template <class L>
+__host__ __device__ constexpr int fsh_class(int i) {
+  return (i == 0 ? 0 : (L::W[i] == L::W[1] ? 1 : 2)) * 3 + L::EX[i] + 1;
+}
 template <class L, class C>
 __host__ __device__ __forceinline__ C fshift(int i, C u0) {
   const C a = C(1) - C(1.5) * u0 * u0;
@@ -156,12 +206,16 @@ template <int ST> struct Storage;
   }                                                                                          \
   static __device__ __forceinline__ void st(T* plane, unsigned k, C v, C s) {               \
     store((T*)((char*)plane + k * (unsigned)sizeof(T)), 0, v, s);                           \
+  }                                                                                          \
+  static __device__ __forceinline__ void st_nt(T* plane, unsigned k, C v, C s) {            \
+    store_nt((T*)((char*)plane + k * (unsigned)sizeof(T)), v, s);                           \
   }
 template <> struct Storage<ST_F64> {
   using T = double;
   using C = double;
   static __device__ __forceinline__ C load(const T* p, size_t k, C) { return p[k]; }
   static __device__ __forceinline__ void store(T* p, size_t k, C v, C) { p[k] = v; }
+  static __device__ __forceinline__ void store_nt(T* q, C v, C) { __builtin_nontemporal_store(v, q); }
   AERO_PLANE_ACCESS
 };
 template <> struct Storage<ST_F32> {
@@ -169,13 +223,31 @@ template <> struct Storage<ST_F32> {
   using C = float;
   static __device__ __forceinline__ C load(const T* p, size_t k, C) { return p[k]; }
   static __device__ __forceinline__ void store(T* p, size_t k, C v, C) { p[k] = v; }
+  static __device__ __forceinline__ void store_nt(T* q, C v, C) { __builtin_nontemporal_store(v, q); }
   AERO_PLANE_ACCESS
 };
+// FP16: the conversion and the shift are one mixed-precision FMA each way
+// (v_fma_mix_f32 on load, v_fma_mixlo_f16 on store).  The unit multiplier
+// is held opaque in a scalar register: with a literal 1.0 the compiler folds
+// the FMA back into a separate add and convert.  The load is exact-then-
+// one-rounding as before; the store rounds v - s once, straight to FP16.
 template <> struct Storage<ST_F16S> {
   using T = __half;
   using C = float;
-  static __device__ __forceinline__ C load(const T* p, size_t k, C s) { return __half2float(p[k]) + s; }
-  static __device__ __forceinline__ void store(T* p, size_t k, C v, C s) { p[k] = __float2half(v - s); }
+  static __device__ __forceinline__ float one() {
+    float o = 1.0f;
+    asm("" : "+s"(o));
+    return o;
+  }
+  static __device__ __forceinline__ C load(const T* p, size_t k, C s) {
+    return __builtin_fmaf(__half2float(p[k]), one(), s);
+  }
+  static __device__ __forceinline__ void store(T* p, size_t k, C v, C s) {
+    p[k] = __float2half(__builtin_fmaf(v, one(), -s));
+  }
+  static __device__ __forceinline__ void store_nt(T* q, C v, C s) {
+    __builtin_nontemporal_store(__half_as_ushort(__float2half(__builtin_fmaf(v, one(), -s))), (unsigned short*)q);
+  }
   AERO_PLANE_ACCESS
 };
 template <> struct Storage<ST_BF16S> {
@@ -183,6 +255,9 @@ template <> struct Storage<ST_BF16S> {
   using C = float;
   static __device__ __forceinline__ C load(const T* p, size_t k, C s) { return __bfloat162float(p[k]) + s; }
   static __device__ __forceinline__ void store(T* p, size_t k, C v, C s) { p[k] = __float2bfloat16(v - s); }
+  static __device__ __forceinline__ void store_nt(T* q, C v, C s) {
+    __builtin_nontemporal_store(__bfloat16_as_ushort(__float2bfloat16(v - s)), (unsigned short*)q);
+  }
   AERO_PLANE_ACCESS
 };
 #undef AERO_PLANE_ACCESS
@@ -215,8 +290,10 @@ constexpr int BLOCK = 256;
 struct StepParams {
   int nx, ny, nz;
   int nrow;  // nz * ny
+  DivMagic div_nx, div_ny;  // cell -> (x, y, z) without integer division
   long long ncell;
   int nblocks;  // blocks covering ncell (the step launch adds one reduction block)
+  int nt_store;  // population stores bypass the caches (launch working set beyond the last-level cache)
   const AERO_GLOBAL void* fin;
   AERO_GLOBAL void* fout;
   const AERO_GLOBAL void* gin;
@@ -241,7 +318,7 @@ struct StepParams {
   const AERO_GLOBAL double* minv_d;
   const AERO_GLOBAL float* minv_f;
   double mrt_s[19];
-  float fsh[19];  // storage shift of f per direction (half forms): feq_i(1, (u0, 0, 0))
+  float fsh[NSHIFT];  // storage shift of f by fsh_class (half forms): feq_i(1, (u0, 0, 0))
   double omega, trt_sm, trt_lambda, u0, u_conv;
   double inlet_amp, phase;
   double rho_in, rho_out, u_top, u_bot;

@@ -49,6 +49,32 @@ __device__ __forceinline__ StepDyn dyn_of(const StepParams& p) {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Grid coordinates of a linear cell index (x fastest).
+__device__ __forceinline__ void cell_xyz(const StepParams& p, unsigned cell, int& x, int& y, int& z) {
+  const unsigned yz = udiv(cell, p.div_nx);
+  const unsigned zz = udiv(yz, p.div_ny);
+  x = (int)(cell - yz * (unsigned)p.nx);
+  y = (int)(yz - zz * (unsigned)p.ny);
+  z = (int)zz;
+}
+
+// 1/rho for the collision: the hardware reciprocal (1 ulp) when computing in
+// FP32, an IEEE division in FP64.
+template <class C>
+__device__ __forceinline__ C recip(C v) {
+  if constexpr (sizeof(C) == 4) return __builtin_amdgcn_rcpf(v);
+  else return C(1) / v;
+}
+
+// A plane base the optimiser may not fold into per-lane addresses: every
+// store of a cell then forms scalar base + the shared 32-bit lane offset
+// instead of a 64-bit vector address per plane.
+template <class T>
+__device__ __forceinline__ AERO_GLOBAL T* scalar_base(AERO_GLOBAL T* q) {
+  asm("" : "+s"(q));
+  return q;
+}
+
 template <class C>
 __device__ __forceinline__ C feq_i(C w, C rho, C eu, C usq) {
   return w * rho * (C(1) + C(3) * eu + C(4.5) * eu * eu - C(1.5) * usq);
@@ -77,7 +103,7 @@ __device__ __forceinline__ void gather(const StepParams& p, const typename Stora
     const int ys = L::EY[i] == 0 ? y : (L::EY[i] > 0 ? ym : yp);
     const int zs = L::EZ[i] == 0 ? z : (L::EZ[i] > 0 ? zm : zp);
     const unsigned src = ((unsigned)zs * (unsigned)p.ny + (unsigned)ys) * (unsigned)p.nx + (unsigned)xs;
-    fn[i] = S::ld(B + (size_t)i * N, src, KIND == 0 ? (C)p.fsh[i] : C(L::W[i]) * shift_scale);
+    fn[i] = S::ld(B + (size_t)i * N, src, KIND == 0 ? (C)p.fsh[fsh_class<L>(i)] : C(L::W[i]) * shift_scale);
   }
 }
 
@@ -120,7 +146,7 @@ __device__ __forceinline__ void apply_links_light(const StepParams& p, const typ
   if (!p.bouzidi) {
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
-      const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, (C)p.fsh[i]);
+      const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, (C)p.fsh[fsh_class<L>(i)]);
       fn[L::OPP[i]] = ((mask >> i) & 1u) ? fi : fn[L::OPP[i]];
     }
     return;
@@ -140,9 +166,9 @@ __device__ __forceinline__ void apply_links_light(const StepParams& p, const typ
     const int yn = clampi(y + L::EY[o], 0, p.ny - 1);
     const int zn = clampi(z + L::EZ[o], 0, p.nz - 1);
     const unsigned nb = ((unsigned)zn * p.ny + yn) * p.nx + xn;
-    const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, (C)p.fsh[i]);
-    const C f_up = S::ld(B + (size_t)i * N, nb, (C)p.fsh[i]);          // f_pre[i](x - e_i)
-    const C f_op = S::ld(B + (size_t)o * N, (unsigned)cell, (C)p.fsh[o]);  // f_pre[OPP[i]](x)
+    const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, (C)p.fsh[fsh_class<L>(i)]);
+    const C f_up = S::ld(B + (size_t)i * N, nb, (C)p.fsh[fsh_class<L>(i)]);          // f_pre[i](x - e_i)
+    const C f_op = S::ld(B + (size_t)o * N, (unsigned)cell, (C)p.fsh[fsh_class<L>(o)]);  // f_pre[OPP[i]](x)
     const C partner = upstream ? f_up : f_op;
     const C a = lo ? C(2) * q : C(1) / (C(2) * q);
     const C v = a * fi + (C(1) - a) * partner;
@@ -170,7 +196,7 @@ __device__ __forceinline__ void link_pre_terms(const StepParams& p, const typena
   const uint32_t mask = p.lrec[rec].mask;
   C fp[Q];
 #pragma unroll
-  for (int i = 0; i < Q; ++i) fp[i] = St::ld(B + (size_t)i * N, (unsigned)cell, (C)p.fsh[i]);
+  for (int i = 0; i < Q; ++i) fp[i] = St::ld(B + (size_t)i * N, (unsigned)cell, (C)p.fsh[fsh_class<L>(i)]);
   double rho = 0, mx = 0, my = 0, mz = 0, fx = 0, fy = 0, fz = 0;
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
@@ -518,18 +544,12 @@ __device__ __forceinline__ void scalar_step(const StepParams& p, const StepDyn& 
       for (int i = 0; i < Q; ++i) g[i] = C(L::W[i]) * (C)p.T_cold;
     }
   }
-  C T = 0;
+  C T = g[0];
 #pragma unroll
-  for (int i = 0; i < Q; ++i) T += g[i];
-  C rho = 0, mx = 0, my = 0, mz = 0;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    rho += fn[i];
-    mx += C(L::EX[i]) * fn[i];
-    my += C(L::EY[i]) * fn[i];
-    mz += C(L::EZ[i]) * fn[i];
-  }
-  const C inv = rho > C(0) ? C(1) / rho : C(0);
+  for (int i = 1; i < Q; ++i) T += g[i];
+  C rho, mx, my, mz;
+  moments<L>(fn, rho, mx, my, mz);
+  const C inv = rho > C(0) ? recip(rho) : C(0);
   const C ux = inv * mx, uy = inv * my, uz = inv * mz;
   if (p.buoyancy) {
     const C buoy = (C)p.gbeta * (T - (C)p.T_ref);
@@ -538,13 +558,13 @@ __device__ __forceinline__ void scalar_step(const StepParams& p, const StepDyn& 
   }
   if (write_g) {
     const C om = (C)p.omega_T;
-    typename S::T* gout = (typename S::T*)p.gout;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      const C eu = C(L::EX[i]) * ux + C(L::EY[i]) * uy + C(L::EZ[i]) * uz;
+      const C eu = edot<L>(i, ux, uy, uz);
       const C geq = C(L::W[i]) * T * (C(1) + C(3) * eu);
       const C gp = solid ? C(L::W[i]) * T : g[i] - om * (g[i] - geq);
-      S::st(gout + (size_t)i * N, (unsigned)cell, gp, C(L::W[i]) * (C)p.gshift);
+      S::st((typename S::T*)scalar_base((AERO_GLOBAL typename S::T*)p.gout + (size_t)i * N), (unsigned)cell, gp,
+            C(L::W[i]) * (C)p.gshift);
     }
   }
 }
@@ -555,50 +575,71 @@ template <class L, int COLL, class C>
 __device__ __forceinline__ void collide(const double* mrt_s, const double* minv_d, const float* minv_f,
                                         bool solid, C om, C sm, const C fn[L::Q], C out[L::Q]) {
   constexpr int Q = L::Q;
-  C rho = 0, mx = 0, my = 0, mz = 0;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    rho += fn[i];
-    mx = madd(mx, L::EX[i], fn[i]);
-    my = madd(my, L::EY[i], fn[i]);
-    mz = madd(mz, L::EZ[i], fn[i]);
-  }
-  const C inv = rho > C(0) ? C(1) / rho : C(0);
-  C ux = inv * mx, uy = inv * my, uz = inv * mz;
-  if (solid) {
-    ux = C(0);
-    uy = C(0);
-    uz = C(0);
-  }
-  const C usq = ux * ux + uy * uy + uz * uz;
-  // Equilibrium by opposite pairs (i, OPP[i]): with eu = e_i.u the even part
-  // is w_i rho (1 + 4.5 eu^2 - 1.5 u.u) and the odd part w_i rho 3 eu, so
-  // feq_i = even + odd and feq_OPP[i] = even - odd (w_OPP[i] = w_i).
-  const C base = C(1) - C(1.5) * usq;
   if constexpr (COLL == COLL_BGK || COLL == COLL_TRT) {
-    const C c1 = C(1) - om;  // relaxation of the even part
-    const C c2 = C(1) - sm;  // of the odd part (TRT)
-    out[0] = c1 * fn[0] + om * (C(L::W[0]) * rho * base);
+    // Opposite pairs (i, j = OPP[i], i < j): s = f_i + f_j and d = f_i - f_j
+    // give the density (f_0 + sum s), the momentum (sum e_i d, as e_j = -e_i)
+    // and the even / odd parts the relaxation acts on.
+    C sp[Q], sd[Q];
+    C rho = fn[0];
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
       const int j = L::OPP[i];
       if (i > j) continue;
-      const C eu = edot<L>(i, ux, uy, uz);
+      sp[i] = fn[i] + fn[j];
+      sd[i] = fn[i] - fn[j];
+      rho += sp[i];
+    }
+    C mx = C(0), my = C(0), mz = C(0);
+    bool sx = false, sy = false, sz = false;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      if (i > L::OPP[i]) continue;
+      if (L::EX[i] != 0) { mx = sx ? madd(mx, L::EX[i], sd[i]) : mfirst(L::EX[i], sd[i]); sx = true; }
+      if (L::EY[i] != 0) { my = sy ? madd(my, L::EY[i], sd[i]) : mfirst(L::EY[i], sd[i]); sy = true; }
+      if (L::EZ[i] != 0) { mz = sz ? madd(mz, L::EZ[i], sd[i]) : mfirst(L::EZ[i], sd[i]); sz = true; }
+    }
+    const C inv = rho > C(0) ? recip(rho) : C(0);
+    C ux = inv * mx, uy = inv * my, uz = inv * mz;
+    if (solid) {
+      ux = C(0);
+      uy = C(0);
+      uz = C(0);
+    }
+    const C base = C(1) - C(1.5) * (ux * ux + uy * uy + uz * uz);
+    // feq_i = even + odd with even = w rho (base + 4.5 eu^2), odd = 3 w rho eu
+    // (eu = e_i.u) and feq_j = even - odd.  The new pair is
+    //   a = om even + (1 - om) s/2,  b = sm odd + (1 - sm) d/2,
+    //   f_i = a + b,  f_j = a - b,
+    // which for BGK (sm = om) is (1 - om) f + om feq.  om w rho base,
+    // 4.5 om w rho and 3 sm w rho are common to a weight shell.
+    const C smv = COLL == COLL_TRT ? sm : om;
+    const C h1 = C(0.5) * (C(1) - om);
+    const C h2 = C(0.5) * (C(1) - smv);
+    out[0] = (C(1) - om) * fn[0] + om * (C(L::W[0]) * rho) * base;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const int j = L::OPP[i];
+      if (i > j) continue;
       const C wr = C(L::W[i]) * rho;
-      const C ev = wr * (base + C(4.5) * eu * eu);
-      const C od = wr * (C(3) * eu);
-      if constexpr (COLL == COLL_BGK) {
-        out[i] = c1 * fn[i] + om * (ev + od);
-        out[j] = c1 * fn[j] + om * (ev - od);
-      } else {
-        // f+- = (f_i +- f_j) / 2; out = feq + (1 - w+)(f+ - even) +- (1 - w-)(f- - odd)
-        const C a = ev + c1 * (C(0.5) * (fn[i] + fn[j]) - ev);
-        const C b = od + c2 * (C(0.5) * (fn[i] - fn[j]) - od);
-        out[i] = a + b;
-        out[j] = a - b;
-      }
+      const C eu = edot<L>(i, ux, uy, uz);
+      const C even = (C(4.5) * om * wr) * eu * eu + om * wr * base;
+      const C a = h1 * sp[i] + even;
+      const C b = h2 * sd[i] + (C(3) * smv * wr) * eu;
+      out[i] = a + b;
+      out[j] = a - b;
     }
   } else {
+    C rho, mx, my, mz;
+    moments<L>(fn, rho, mx, my, mz);
+    const C inv = rho > C(0) ? recip(rho) : C(0);
+    C ux = inv * mx, uy = inv * my, uz = inv * mz;
+    if (solid) {
+      ux = C(0);
+      uy = C(0);
+      uz = C(0);
+    }
+    const C usq = ux * ux + uy * uy + uz * uz;
+    const C base = C(1) - C(1.5) * usq;
     C fe[Q];
     fe[0] = C(L::W[0]) * rho * base;
 #pragma unroll
@@ -643,7 +684,7 @@ __device__ __forceinline__ void collide(const double* mrt_s, const double* minv_
 // for the LES finite-difference pass -- write the post-boundary populations
 // and velocity (KMODE 1).
 template <class L, int COLL, int ST, int KMODE>
-__device__ __forceinline__ void finish_cell(const StepParams& p, int cell, bool solid,
+__device__ __forceinline__ void finish_cell(const StepParams& p, int cell, unsigned wcell, bool solid,
                                             const typename Storage<ST>::C fn[L::Q]) {
   using S = Storage<ST>;
   using C = typename S::C;
@@ -653,9 +694,20 @@ __device__ __forceinline__ void finish_cell(const StepParams& p, int cell, bool 
   if constexpr (KMODE == 0) {
     C out[Q];
     collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, (C)p.omega, (C)p.trt_sm, fn, out);
-    T* Bo = (T*)p.fout;
+    // cell = wcell (the wave's first cell, scalar) + lane: the store offset is
+    // rebuilt here rather than held in a register across the whole step
+    const unsigned lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    if (p.nt_store) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) S::st(Bo + (size_t)i * N, (unsigned)cell, out[i], (C)p.fsh[i]);
+      for (int i = 0; i < Q; ++i)
+        S::st_nt((T*)scalar_base((AERO_GLOBAL T*)p.fout + (size_t)i * N + wcell), lane, out[i],
+                 (C)p.fsh[fsh_class<L>(i)]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < Q; ++i)
+        S::st((T*)scalar_base((AERO_GLOBAL T*)p.fout + (size_t)i * N + wcell), lane, out[i],
+              (C)p.fsh[fsh_class<L>(i)]);
+    }
   } else {
     C* F = (C*)p.fsplit;
     C* U = (C*)p.usplit;
@@ -776,12 +828,11 @@ struct StepWaves {
     }                                                                                                      \
     /* cell indices fit in 31 bits (checked when the context is created) */                                \
     const unsigned cu_ = (unsigned)(blk - 1) * (unsigned)BLOCK + threadIdx.x;                              \
+    const unsigned wcell_ = (unsigned)(blk - 1) * (unsigned)BLOCK + __builtin_amdgcn_readfirstlane(threadIdx.x);\
     if (cu_ >= (unsigned)p.ncell) return;                                                                  \
     const int cell = (int)cu_;                                                                             \
-    const int x = cell % p.nx;                                                                             \
-    const int yz = cell / p.nx;                                                                            \
-    const int y = yz % p.ny;                                                                               \
-    const int z = yz / p.ny;                                                                               \
+    int x, y, z;                                                                                           \
+    cell_xyz(p, cu_, x, y, z);                                                                             \
     const uint8_t flag = p.flags[cell];                                                                    \
     const bool solid = flag & FLAG_SOLID;                                                                  \
     const T_* B = (const T_*)p.fin;                                                                        \
@@ -791,7 +842,7 @@ struct StepWaves {
     const int rec = post_bc<L, ST>(p, dyn, B, cell, x, y, z, flag, true, fn);                              \
     if constexpr (THERMAL) scalar_step<L, ST>(p, dyn, cell, x, y, z, solid, fn, true);                     \
     if (rec >= 0) link_post_sum<L, C_>(p, rec, fn);                                                        \
-    finish_cell<L, COLL, ST, KMODE>(p, cell, solid, fn);                                                   \
+    finish_cell<L, COLL, ST, KMODE>(p, cell, wcell_, solid, fn);                                           \
   } while (0)
 
 // One case, parameters by value (split LES step, recycling inlet).
@@ -862,10 +913,8 @@ __global__ __launch_bounds__(BLOCK) void materialize_kernel(StepParams p) {
   double v_nan = 0, v_neg = 0, v_umax = 0, v_rho = 0, v_cnt = 0;
   if (c64 < p.ncell) {
     const int cell = (int)c64;
-    const int x = cell % p.nx;
-    const int yz = cell / p.nx;
-    const int y = yz % p.ny;
-    const int z = yz / p.ny;
+    int x, y, z;
+    cell_xyz(p, (unsigned)cell, x, y, z);
     const bool in_plane = p.check_z < 0 || z == p.check_z;
     if (in_plane) {
       const uint8_t flag = p.flags[cell];
@@ -955,7 +1004,7 @@ struct CollideParams {
   const float* minv_f;
   double mrt_s[19];
   double omega, trt_sm, trt_lambda, base_nu, les_cs, u0;
-  float fsh[19];  // storage shift of f per direction (half forms), as in StepParams
+  float fsh[NSHIFT];  // storage shift of f by fsh_class (half forms), as in StepParams
   int les;
   int trt_variant;
 };
@@ -1064,7 +1113,7 @@ __global__ __launch_bounds__(BLOCK) void collide_kernel(CollideParams p) {
   collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, om, sm, fn, out);
   T* Bo = (T*)p.fout;
 #pragma unroll
-  for (int i = 0; i < Q; ++i) S::st(Bo + (size_t)i * N, (unsigned)cell, out[i], (C)p.fsh[i]);
+  for (int i = 0; i < Q; ++i) S::st(Bo + (size_t)i * N, (unsigned)cell, out[i], (C)p.fsh[fsh_class<L>(i)]);
 }
 
 }  // namespace aero
