@@ -57,6 +57,7 @@ _SIGS = {
     "aero_set_damping": ([_P, _P], ctypes.c_int),
     "aero_set_f": ([_P, _P], ctypes.c_int),
     "aero_get_f": ([_P, _P], ctypes.c_int),
+    "aero_set_f_equilibrium": ([_P, _P, _P, _P, _P], ctypes.c_int),
     "aero_set_g": ([_P, _P], ctypes.c_int),
     "aero_get_g": ([_P, _P], ctypes.c_int),
     "aero_set_outlet_prev": ([_P, _P], ctypes.c_int),

@@ -67,6 +67,25 @@ class DeviceSolver:
             raise ValueError(f"f must have shape {self._ref_shape()}, got {arr.shape}")
         self._ctx.call("aero_set_f", _abi.ptr(arr))
 
+    def set_equilibrium(self, rho=None, ux=None, uy=None, uz=None) -> None:
+        """self.f = compute_feq(rho, ux, uy[, uz]) evaluated on the device
+        (bit-identical to the host formula, without a host field of Q planes).
+        Each argument is a full-grid array or None for the constant rho0, u0,
+        0, 0.  Not part of the reference API (it sets self.f directly)."""
+        shape = self._ref_shape()[1:]
+        args = []
+        for name, v in (("rho", rho), ("ux", ux), ("uy", uy), ("uz", uz)):
+            if v is None:
+                args.append(None)
+                continue
+            if name == "uz" and len(shape) == 2:
+                raise ValueError("uz is a 3D field")
+            a = np.ascontiguousarray(v, dtype=np.float64)
+            if a.shape != shape:
+                raise ValueError(f"{name} must have shape {shape}, got {a.shape}")
+            args.append(a)
+        self._ctx.call("aero_set_f_equilibrium", *[_abi.ptr(a) for a in args])
+
     @property
     def g(self) -> Optional[np.ndarray]:
         if not self.thermal:

@@ -242,6 +242,44 @@ hipError_t launch_compute_from_f64(int st, long long count, const double* src, v
   return hipGetLastError();
 }
 
+// f = feq(rho, u) in FP64 over the reference layout (Q x N), as
+// compute_feq / compute_feq_3d (aero/lbm/d2q9.py:66-89, d3q19.py:105-129):
+// every product and sum is rounded in the reference's order and FMA
+// contraction is off, so the field is bit-identical to the host's numpy
+// version.  Null field pointers stand for the constants rho0, ux0, 0, 0.
+template <class L>
+__global__ __launch_bounds__(BLOCK) void feq_fill_kernel(long long n, double* __restrict__ f,
+                                                         const double* __restrict__ rho, double rho0,
+                                                         const double* __restrict__ ux, double ux0,
+                                                         const double* __restrict__ uy,
+                                                         const double* __restrict__ uz) {
+#pragma clang fp contract(off)
+  const long long c = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (c >= n) return;
+  const double r = rho ? rho[c] : rho0;
+  const double u = ux ? ux[c] : ux0;
+  const double v = uy ? uy[c] : 0.0;
+  const double w = uz ? uz[c] : 0.0;
+  double usq = u * u + v * v;
+  if (L::DIM == 3) usq = usq + w * w;
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    double eu = (double)L::EX[i] * u + (double)L::EY[i] * v;
+    if (L::DIM == 3) eu = eu + (double)L::EZ[i] * w;
+    f[(size_t)i * n + c] = (L::W[i] * r) * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * usq);
+  }
+}
+
+hipError_t launch_feq_fill(int q, long long n, double* f, const double* rho, double rho0, const double* ux,
+                           double ux0, const double* uy, const double* uz, hipStream_t s) {
+  const dim3 g((unsigned)((n + BLOCK - 1) / BLOCK));
+  if (q == 19)
+    feq_fill_kernel<D3Q19><<<g, BLOCK, 0, s>>>(n, f, rho, rho0, ux, ux0, uy, uz);
+  else
+    feq_fill_kernel<D2Q9><<<g, BLOCK, 0, s>>>(n, f, rho, rho0, ux, ux0, uy, uz);
+  return hipGetLastError();
+}
+
 // Velocity of an explicit field (rho > 0 guard, solids not zeroed: les.py:510).
 template <class C, int Q>
 __global__ __launch_bounds__(BLOCK) void macro_kernel(long long n, const C* F, C* U) {

@@ -75,6 +75,30 @@ double trt_s_minus(double omega, double lam, int variant) {
 
 constexpr size_t NT_STORE_BYTES = (size_t)192 << 20;
 
+// MI355X hands the workgroups of a launch to its 8 XCDs round-robin (block b
+// to XCD b mod 8), and each XCD has its own L2.  Launch slots are grouped in
+// rounds of XCDS * run: within a round XCD x takes `run` consecutive tiles of
+// the natural order, so the rows a block pulls from its neighbours (x +- 1,
+// y +- 1) were read by the same XCD a moment earlier.  One round over the
+// whole list gives every XCD one contiguous eighth of the cells (best when
+// the z +- 1 planes of all eight eighths fit the last-level cache together:
+// 2D grids, small 3D ones); short runs keep the whole device sweeping the grid
+// together, so large 3D grids still find the planes at z +- 1 in that cache.
+constexpr int XCDS = 8;
+constexpr int XCD_RUN_LARGE = 8;
+constexpr size_t ZREUSE_BYTES = (size_t)64 << 20;
+static void xcd_order(const std::vector<int2>& nat, int run, std::vector<int2>& out) {
+  const int n = (int)nat.size();
+  const int round = XCDS * run;
+  for (int r0 = 0; r0 < n; r0 += round) {
+    const int m = std::min(round, n - r0);
+    int off[XCDS + 1];
+    off[0] = 0;
+    for (int x = 0; x < XCDS; ++x) off[x + 1] = off[x] + (m > x ? (m - 1 - x) / XCDS + 1 : 0);
+    for (int b = 0; b < m; ++b) out.push_back(nat[r0 + off[b % XCDS] + b / XCDS]);
+  }
+}
+
 size_t storage_bytes(int st) {
   switch (st) {
     case ST_F64: return 8;
@@ -451,11 +475,19 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
   std::vector<std::pair<const void*, int>> key;
   for (aero_ctx* c : cs) key.emplace_back((const void*)c, c->nblocks);
   {
-    std::vector<int2> tiles;
+    std::vector<int2> tiles, nat;
     for (size_t g = 0; g < groups.size(); ++g) {
       groups[g].toff = (int)tiles.size();
-      for (int i : members[g])
-        for (int b = 0; b <= cs[i]->nblocks; ++b) tiles.push_back(make_int2(i, b));
+      nat.clear();
+      size_t zreuse = 0;  // planes z-1, z, z+1 of every population, on all XCDs at once
+      for (int i : members[g]) {
+        for (int b = 0; b <= cs[i]->nblocks; ++b) nat.push_back(make_int2(i, b));
+        const aero_ctx* c = cs[i];
+        if (c->d.nz > 1)
+          zreuse = std::max(zreuse, (size_t)XCDS * 3 * c->Q * (size_t)c->d.nx * c->d.ny * c->esz);
+      }
+      const int run = zreuse > ZREUSE_BYTES ? XCD_RUN_LARGE : ((int)nat.size() + XCDS - 1) / XCDS;
+      xcd_order(nat, std::max(run, 1), tiles);
       groups[g].ntiles = (int)tiles.size() - groups[g].toff;
     }
     if (key != R.tile_key) {
@@ -836,12 +868,10 @@ int aero_set_damping(aero_ctx* c, const double* damp) {
   });
 }
 
-int aero_set_f(aero_ctx* c, const double* f) {
-  return guarded([&] {
-    c->set_device();
-    const size_t cnt = (size_t)c->Q * c->N;
-    c->fcache.alloc(sizeof(double) * cnt);
-    ck(hipMemcpyAsync(c->fcache.p, f, sizeof(double) * cnt, hipMemcpyHostToDevice, c->stream), "upload f");
+// The state from the FP64 field in fcache (the reference's self.f).
+static void state_from_fcache(aero_ctx* c) {
+  const size_t cnt = (size_t)c->Q * c->N;
+  {
     // F in compute precision, then B[side] = collide(F) (with LES omega from F)
     const void* F = c->fcache.p;
     if (c->d.storage != ST_F64) {
@@ -860,6 +890,48 @@ int aero_set_f(aero_ctx* c, const double* f) {
     if (c->d.storage != ST_F64) c->tmp.release();
     c->fcache_valid = true;
     c->has_state = true;  // state defined; fcache holds it until the next step
+  }
+}
+
+int aero_set_f(aero_ctx* c, const double* f) {
+  return guarded([&] {
+    c->set_device();
+    const size_t cnt = (size_t)c->Q * c->N;
+    c->fcache.alloc(sizeof(double) * cnt);
+    ck(hipMemcpyAsync(c->fcache.p, f, sizeof(double) * cnt, hipMemcpyHostToDevice, c->stream), "upload f");
+    state_from_fcache(c);
+  });
+}
+
+int aero_set_f_equilibrium(aero_ctx* c, const double* rho, const double* ux, const double* uy, const double* uz) {
+  return guarded([&] {
+    c->set_device();
+    const size_t N = (size_t)c->N;
+    const double* src[4] = {rho, ux, uy, uz};
+    int nf = 0;
+    for (const double* q : src) nf += q != nullptr;
+    if (c->Q == 9 && uz) throw std::runtime_error("aero_set_f_equilibrium: uz given for a 2D context");
+    c->fcache.alloc(sizeof(double) * (size_t)c->Q * N);
+    DevBuf fields;
+    struct Release {
+      DevBuf& b;
+      ~Release() { b.release(); }  // hipFree waits for the stream's use of it
+    } release_fields{fields};
+    const double* dev[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (nf) {
+      fields.alloc(sizeof(double) * N * nf);
+      int k = 0;
+      for (int a = 0; a < 4; ++a)
+        if (src[a]) {
+          double* d = fields.as<double>() + N * k++;
+          ck(hipMemcpyAsync(d, src[a], sizeof(double) * N, hipMemcpyHostToDevice, c->stream), "upload field");
+          dev[a] = d;
+        }
+    }
+    ck(launch_feq_fill(c->Q, (long long)N, c->fcache.as<double>(), dev[0], c->d.rho0, dev[1], c->d.u0, dev[2],
+                       dev[3], c->stream),
+       "equilibrium");
+    state_from_fcache(c);
   });
 }
 

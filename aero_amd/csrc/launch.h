@@ -45,6 +45,8 @@ hipError_t launch_materialize_g(int q, int st, const StepParams& p, double* out,
 hipError_t launch_store_from_f64(int st, int q, long long n, const double* src, void* dst, double shift_scale,
                                  hipStream_t s);
 hipError_t launch_compute_from_f64(int st, long long count, const double* src, void* dst, hipStream_t s);
+hipError_t launch_feq_fill(int q, long long n, double* f, const double* rho, double rho0, const double* ux,
+                           double ux0, const double* uy, const double* uz, hipStream_t s);
 hipError_t launch_macro_from_compute(int st, int q, long long n, const void* F, void* U, hipStream_t s);
 hipError_t launch_macro4_f64(int q, long long n, const double* F, double* out, hipStream_t s);
 hipError_t launch_check_f64(int q, int nx, int ny, int nz, int zc, const double* F, int nblocks, double* part,

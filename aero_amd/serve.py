@@ -14,7 +14,9 @@ share a kernel.  Between chunks it
   non-finite drag coefficient);
 * admits waiting requests whose arrival time has passed while the live set
   stays under ``max_live`` cases and the device keeps ``mem_reserve`` bytes
-  free after the new case's estimated footprint.
+  free after the new case's estimated footprint; admitted cases are built
+  by ``build_workers`` host threads while the device steps, and join the
+  batch when built.
 
 A case's trajectory does not depend on what else is live (the batched
 kernel computes each case's bits exactly as alone) and its stop decisions
@@ -107,7 +109,9 @@ class CaseSpec:
                                 outlet_bc="zerogradient", sponge_thickness=max(8, self.nx // 20),
                                 storage=self.storage, thermal=self.thermal, alpha_T=0.01)
             sh = (self.ny, self.nx)
-            s.f = compute_feq(np.ones(sh), np.full(sh, self.u0), rng.uniform(-1e-4, 1e-4, sh))
+            uy = rng.uniform(-1e-4, 1e-4, sh)
+            s.set_equilibrium(uy=uy)  # feq(1, u0, uy) on the device
+            s._f_outlet_prev = compute_feq(np.ones(self.ny), np.full(self.ny, self.u0), uy[:, -1])
         else:
             solid = g.mark_solid(self.nz, self.ny, self.nx)
             D = self.reference_length(solid)
@@ -118,9 +122,11 @@ class CaseSpec:
                                   outlet_bc="zerogradient", sponge_thickness=max(8, self.nx // 20),
                                   storage=self.storage, thermal=self.thermal, alpha_T=0.01)
             sh = (self.nz, self.ny, self.nx)
-            s.f = compute_feq_3d(np.ones(sh), np.full(sh, self.u0), rng.uniform(-1e-4, 1e-4, sh),
-                                 rng.uniform(-1e-4, 1e-4, sh))
-        s._f_outlet_prev = s.f[..., -1].copy()
+            uy = rng.uniform(-1e-4, 1e-4, sh)
+            uz = rng.uniform(-1e-4, 1e-4, sh)
+            s.set_equilibrium(uy=uy, uz=uz)  # feq(1, u0, uy, uz) on the device
+            ps = (self.nz, self.ny)
+            s._f_outlet_prev = compute_feq_3d(np.ones(ps), np.full(ps, self.u0), uy[..., -1], uz[..., -1])
         return s
 
 
@@ -220,8 +226,9 @@ class Server:
     def __init__(self, chunk: int = 500, max_live: int = 512, mem_reserve: int = 8 << 30,
                  convergence_window: int = 5000, convergence_tol: float = 0.01, strouhal_window: int = 4096,
                  strouhal_tol: float = 0.05, batch=None, free_memory: Optional[Callable[[], int]] = None,
-                 keep_history: bool = False):
+                 keep_history: bool = False, build_workers: int = 4):
         self.chunk = int(chunk)
+        self.build_workers = max(1, int(build_workers))
         self.max_live = int(max_live)
         self.mem_reserve = int(mem_reserve)
         self.stop_kw = dict(convergence_window=convergence_window, convergence_tol=convergence_tol,
@@ -275,34 +282,48 @@ class Server:
         """Serve (arrival_seconds, CaseSpec) requests; returns one Result per
         request in completion order.
 
-        The host's stop checks of chunk j run while the device advances chunk
-        j+1 (a worker thread holds the device call).  A case found done at
-        chunk j is retired with its history cut at the end of chunk j, exactly
-        what it would have been alone; its speculative chunk is discarded."""
-        from concurrent.futures import ThreadPoolExecutor
+        Admitted requests are built (geometry, links, device state) by a pool
+        of builder threads while the device steps the live cases; a built
+        case joins the batch at the next chunk.  The host's stop checks of
+        chunk j run while the device advances chunk j+1 (a worker thread
+        holds the device call).  A case found done at chunk j is retired with
+        its history cut at the end of chunk j, exactly what it would have
+        been alone; its speculative chunk is discarded."""
+        from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
+        if any(r[1].shape == "polygon" for r in requests):
+            import matplotlib.path  # noqa: F401  (imported once here, not raced by the builder threads)
         order = sorted(range(len(requests)), key=lambda i: requests[i][0])
         waiting = [(requests[i][0], i, requests[i][1]) for i in order]
         live: List[_Live] = []
+        building: list = []  # (future, arrival, rid, spec, bytes)
         done: List[Result] = []
         t0 = clock()
         pool = ThreadPoolExecutor(max_workers=1)
+        builders = ThreadPoolExecutor(max_workers=self.build_workers)
         try:
-            while waiting or live:
+            while waiting or live or building:
                 now = clock() - t0
                 free = None
-                while waiting and waiting[0][0] <= now and len(live) < self.max_live:
+                pending = sum(b[4] for b in building)
+                while waiting and waiting[0][0] <= now and len(live) + len(building) < self.max_live:
                     arr, rid, spec = waiting[0]
                     need = spec.device_bytes()
                     if free is None:
                         free = self._free()
-                    if live and free - need < self.mem_reserve:
+                    if (live or building) and free - pending - need < self.mem_reserve:
                         break  # wait for a live case to retire
                     waiting.pop(0)
-                    solver = spec.build()
-                    live.append(_Live(rid, spec, solver, arr, clock() - t0))
-                    free -= need
+                    building.append((builders.submit(spec.build), arr, rid, spec, need))
+                    pending += need
+                for b in [b for b in building if b[0].done()]:
+                    building.remove(b)
+                    fut, arr, rid, spec, _ = b
+                    live.append(_Live(rid, spec, fut.result(), arr, clock() - t0))
                 if not live:
-                    sleep(max(0.0, waiting[0][0] - (clock() - t0)))
+                    if building:
+                        wait([b[0] for b in building], return_when=FIRST_COMPLETED)
+                    elif waiting:
+                        sleep(max(0.0, waiting[0][0] - (clock() - t0)))
                     continue
                 n = self.chunk
                 batch = self._batch_obj()
@@ -331,6 +352,7 @@ class Server:
                         self.steps_done += n * lv.spec.cells
                 live = [lv for lv in live if id(lv) not in gone]
         finally:
+            builders.shutdown(wait=True)
             pool.shutdown(wait=True)
         return done
 

@@ -153,10 +153,10 @@ class Solver(DeviceSolver):
         l4[:, 0], l4[:, 2], l4[:, 3] = self.surface_links[:, 0], self.surface_links[:, 1], self.surface_links[:, 2]
         self._init_device(d, self.solid[None], l4, self.q_vals, damp)
 
-        rho_i = np.full((Ny, Nx), rho0)
-        f0 = compute_feq(rho_i, np.full((Ny, Nx), u0), np.zeros((Ny, Nx)))
-        self.f = f0
-        self._f_outlet_prev = f0[:, :, -1].copy()
+        # f = feq(rho0, u0, 0) (solver.py:218-223), evaluated on the device;
+        # the outlet memory is its last column
+        self.set_equilibrium()
+        self._f_outlet_prev = compute_feq(np.full(Ny, float(rho0)), np.full(Ny, float(u0)), np.zeros(Ny))
         if self.thermal:
             self.g = np.broadcast_to(
                 (np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4) * self.T_ref)[:, None, None],

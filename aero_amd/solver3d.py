@@ -147,9 +147,12 @@ class Solver3D(DeviceSolver):
         self._init_device(d, self.solid, self.surface_links, self.q_vals, damp)
 
         sh = (Nz, Ny, Nx)
-        f0 = compute_feq_3d(np.full(sh, rho0), np.full(sh, u0), np.zeros(sh), np.zeros(sh))
-        self.f = f0
-        self._f_outlet_prev = f0[:, :, :, -1].copy()
+        # f = feq(rho0, u0, 0) (solver3d.py:239-247), evaluated on the device;
+        # the outlet memory is its last x plane
+        self.set_equilibrium()
+        ps = (Nz, Ny)
+        self._f_outlet_prev = compute_feq_3d(np.full(ps, float(rho0)), np.full(ps, float(u0)), np.zeros(ps),
+                                             np.zeros(ps))
         if self.thermal:
             self.g = np.broadcast_to((W3 * self.T_ref)[:, None, None, None], (19,) + sh).copy()
         self.step_count = 0

@@ -176,6 +176,19 @@ def serving(full=False):
     wall = time.perf_counter() - t0
     out["batched_2d"] = dict(serve.summarize(res, wall, srv.steps_done, srv.busy_seconds), cases=n_cases,
                              max_steps=max_steps)
+    # the same cases in the FP16-shifted storage form (FP32 arithmetic)
+    cases16 = serve.c4_cases(n_cases, seed=0, max_steps=max_steps, storage="f16")
+    srv16 = serve.Server(chunk=500)
+    t0 = time.perf_counter()
+    res16 = srv16.serve([(0.0, c) for c in cases16])
+    wall16 = time.perf_counter() - t0
+    by32 = {r.rid: r for r in res}
+    rel = [abs(r.Cd_mean - by32[r.rid].Cd_mean) / abs(by32[r.rid].Cd_mean) for r in res16
+           if r.stop_reason != "diverged" and by32[r.rid].stop_reason != "diverged"]
+    out["batched_2d_f16"] = dict(serve.summarize(res16, wall16, srv16.steps_done, srv16.busy_seconds),
+                                 cases=n_cases, max_steps=max_steps,
+                                 cd_mean_rel_diff_vs_f32_max=float(max(rel, default=0.0)),
+                                 cd_mean_rel_diff_vs_f32_median=float(np.median(rel)) if rel else 0.0)
     seq, wall_s, upd_s = _sequential(cases)
     reasons = {}
     for r in seq:
@@ -191,6 +204,8 @@ def serving(full=False):
                                                               default=0.0))
     out["batched_over_sequential_solves_per_hour"] = round(
         out["batched_2d"]["solves_per_hour"] / out["sequential_2d"]["solves_per_hour"], 2)
+    out["batched_f16_over_sequential_f32_solves_per_hour"] = round(
+        out["batched_2d_f16"]["solves_per_hour"] / out["sequential_2d"]["solves_per_hour"], 2)
     for rate in ((2.0, 10.0) if not full else (1.0, 5.0, 10.0, 20.0, 50.0)):
         n = 48 if not full else 200
         reqs = serve.c5_requests(n, rate=rate, seed=int(rate), max_steps=4000 if not full else 30000)

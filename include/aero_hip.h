@@ -125,6 +125,14 @@ int aero_set_damping(aero_ctx* ctx, const double* damp);
  * self._f_outlet_prev, read and written in the reference layout. */
 int aero_set_f(aero_ctx* ctx, const double* f);
 int aero_get_f(aero_ctx* ctx, double* f);
+/* self.f = compute_feq(rho, ux, uy[, uz]) (d2q9.py:66-89, d3q19.py:105-129)
+ * evaluated on the device, bit-identical to the host formula: the
+ * initial states of solver.py:218-223 / solver3d.py:239-247 and of the
+ * benchmark cases without an N x Q host field and its upload.  Each field
+ * is N doubles in the reference layout or NULL for a constant: rho0 and u0
+ * of the descriptor, 0 for uy and uz (uz must be NULL in 2D). */
+int aero_set_f_equilibrium(aero_ctx* ctx, const double* rho, const double* ux, const double* uy,
+                           const double* uz);
 int aero_set_g(aero_ctx* ctx, const double* g);
 int aero_get_g(aero_ctx* ctx, double* g);
 int aero_set_outlet_prev(aero_ctx* ctx, const double* prev);
