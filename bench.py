@@ -127,6 +127,33 @@ def traffic_from_profiles(name):
     return None
 
 
+FORMS = {  # name: (nx, ny, nz, sphere radius, centre x, storage)
+    "f32_128x64x64": (128, 64, 64, 8.0, 32.0, "f32"),
+    "f16_128x64x64": (128, 64, 64, 8.0, 32.0, "f16"),
+    "f32_512x256x256": (512, 256, 256, 32.0, 128.0, "f32"),
+    "f16_512x256x256": (512, 256, 256, 32.0, 128.0, "f16"),
+    "bf16_512x256x256": (512, 256, 256, 32.0, 128.0, "bf16"),
+}
+
+
+def run_form(name, steps, warmup):
+    """One storage form of the TRT sphere alone: MLUPS over `steps` steps,
+    the step kernel's average duration and its fraction of the HBM roofline
+    (also the workload the per-form rocprof summaries are taken on)."""
+    nx, ny, nz, r, cx, st = FORMS[name]
+    s, _ = make_case(nx, ny, nz, r, cx, st)
+    w, k, rows = timed(s, steps, warmup)
+    cells = nx * ny * nz
+    bpl = 2 * 19 * {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}[st] * cells
+    s.close()
+    out = {"form": name, "MLUPS": round(cells * steps / w / 1e6, 2), "kernel_ms": round(k, 5),
+           "roofline_frac": round(bpl / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "cd_last": float(rows[-1, 0])}
+    t = traffic_from_profiles(f"batch_step_kernel_d3q19_trt_{name}")
+    if t:
+        out["traffic"] = t
+    return out
+
+
 def copy_calibration(mib=1024, reps=4):
     """A plain float32 device-to-device copy of `mib` MiB, `reps` times: the
     kernel whose FETCH_SIZE / WRITE_SIZE per dispatch (known to be `mib` MiB
@@ -227,6 +254,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving / batched-solves leg")
     ap.add_argument("--serve-full", action="store_true",
                     help="serving leg on the full 256-case workload and the full arrival-rate sweep")
+    ap.add_argument("--form", choices=sorted(FORMS), default=None,
+                    help="only time one storage form of the TRT sphere (prints one JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (calibrates the HBM byte counters)")
     a = ap.parse_args()
@@ -234,9 +263,13 @@ def main():
         copy_calibration()
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build runs one resident domain per device and "
-              "has no multi-GPU path (see DESIGN.md for how it would shard).", file=sys.stderr)
+        print("bench.py measures one GPU: the solver keeps one resident domain per device and "
+              "has no multi-GPU path (a large grid would split into x-slabs with one face exchange "
+              "per step).", file=sys.stderr)
         sys.exit(2)
+    if a.form:
+        print(json.dumps(run_form(a.form, a.steps, a.warmup)), flush=True)
+        return
 
     nx, ny, nz = 128, 64, 64
     cells = nx * ny * nz
@@ -267,7 +300,7 @@ def main():
                    "trt_lambda": 0.1875, "storage": "f32", "links": int(len(s.surface_links))},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles("step_kernel_d3q19_trt_f32_128x64x64"),
+                     "traffic": traffic_from_profiles("batch_step_kernel_d3q19_trt_f32_128x64x64"),
                      "kernel": "aero::batch_step_kernel<D3Q19, TRT, F32>", "kernel_ms": round(kern_ms, 5),
                      "algorithmic_bytes_per_launch": bytes_per_launch, "cells": ncell, "link_cells": nlink,
                      "bytes_per_cell": 2 * 19 * 4},
@@ -286,14 +319,7 @@ def main():
             s2.close()
         # the 512 x 256 x 256 sphere (radius 32) the throughput target is quoted on
         for st in ("f32", "f16"):
-            s3, _ = make_case(512, 256, 256, 32.0, 128.0, st)
-            n3 = 40
-            w3, k3, r3 = timed(s3, n3, 10)
-            c3 = 512 * 256 * 256
-            bpl = 2 * 19 * (4 if st == "f32" else 2) * 512 * 256 * 256
-            extra[f"{st}_512x256x256"] = {"MLUPS": round(c3 * n3 / w3 / 1e6, 2), "kernel_ms": round(k3, 4),
-                                          "roofline_frac": round(bpl / (k3 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-            s3.close()
+            extra[f"{st}_512x256x256"] = run_form(f"{st}_512x256x256", 40, 10)
         out["forms"] = extra
     if not a.no_serve:
         out["serving"] = serving(full=a.serve_full)
