@@ -72,6 +72,10 @@ _SIGS = {
     "aero_get_timing": ([_P, _P, _P], ctypes.c_int),
     "aero_cell_counts": ([_P, _P, _P], ctypes.c_int),
     "aero_mem_info": ([_P, _P], ctypes.c_int),
+    "aero_mesh_inside": ([_P, ctypes.c_int64, _P, _P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                          ctypes.c_int32, _P], ctypes.c_int),
+    "aero_mesh_sqdist": ([_P, ctypes.c_int64, _P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                          _P], ctypes.c_int),
     "aero_batch_create": ([ctypes.POINTER(_P)], ctypes.c_int),
     "aero_batch_destroy": ([_P], ctypes.c_int),
     "aero_batch_step": ([_P, ctypes.c_int32, _P, ctypes.c_int32, _P, _P, _P], ctypes.c_int),

@@ -903,6 +903,75 @@ int aero_set_f(aero_ctx* c, const double* f) {
   });
 }
 
+// Device buffer freed when it goes out of scope (hipFree waits for its use).
+struct ScopedBuf : DevBuf {
+  ScopedBuf() = default;
+  ScopedBuf(const ScopedBuf&) = delete;
+  ScopedBuf& operator=(const ScopedBuf&) = delete;
+  ~ScopedBuf() { release(); }
+};
+
+struct ScopedStream {
+  hipStream_t s = nullptr;
+  ScopedStream() { ck(hipStreamCreate(&s), "hipStreamCreate"); }
+  ~ScopedStream() { (void)hipStreamDestroy(s); }
+};
+
+int aero_mesh_inside(const double* tris, int64_t ntri, const double* ray, const double* points, int64_t npts,
+                     int32_t nx, int32_t ny, int32_t nz, uint8_t* inside) {
+  return guarded([&] {
+    if (ntri < 0 || (ntri > 0 && !tris) || !ray || !inside) throw std::runtime_error("aero_mesh_inside: bad arguments");
+    const bool grid = points == nullptr;
+    if (grid && (nx <= 0 || ny <= 0 || nz <= 0)) throw std::runtime_error("aero_mesh_inside: bad grid");
+    if (!grid && npts < 0) throw std::runtime_error("aero_mesh_inside: bad point count");
+    const long long n = grid ? (long long)nx * ny * nz : (long long)npts;
+    if (n == 0) return;
+    ScopedStream st;
+    ScopedBuf dt, scr, dp, dout;
+    if (ntri > 0) {
+      dt.alloc(sizeof(double) * 9 * (size_t)ntri);
+      ck(hipMemcpyAsync(dt.p, tris, sizeof(double) * 9 * (size_t)ntri, hipMemcpyHostToDevice, st.s), "upload mesh");
+    }
+    scr.alloc(mesh_inside_scratch_bytes(ntri));
+    if (!grid) {
+      dp.alloc(sizeof(double) * 3 * (size_t)n);
+      ck(hipMemcpyAsync(dp.p, points, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, st.s), "upload points");
+    }
+    dout.alloc((size_t)n);
+    ck(launch_mesh_inside(dt.as<double>(), (long long)ntri, scr.p, dp.as<double>(), n, grid ? nx : 1,
+                          grid ? ny : 1, ray, dout.as<uint8_t>(), st.s),
+       "mesh inside");
+    ck(hipMemcpyAsync(inside, dout.p, (size_t)n, hipMemcpyDeviceToHost, st.s), "download inside");
+    ck(hipStreamSynchronize(st.s), "sync");
+  });
+}
+
+int aero_mesh_sqdist(const double* tris, int64_t ntri, const int64_t* cells, int64_t ncells, int32_t nx, int32_t ny,
+                     int32_t nz, double* sqdist) {
+  return guarded([&] {
+    if (ntri < 0 || (ntri > 0 && !tris) || ncells < 0 || (ncells > 0 && (!cells || !sqdist)))
+      throw std::runtime_error("aero_mesh_sqdist: bad arguments");
+    if (nx <= 0 || ny <= 0 || nz <= 0) throw std::runtime_error("aero_mesh_sqdist: bad grid");
+    const long long N = (long long)nx * ny * nz;
+    for (int64_t k = 0; k < ncells; ++k)
+      if (cells[k] < 0 || cells[k] >= N) throw std::runtime_error("aero_mesh_sqdist: cell index outside the grid");
+    if (ncells == 0) return;
+    ScopedStream st;
+    ScopedBuf dt, dc, dout;
+    dt.alloc(sizeof(double) * 9 * (size_t)(ntri > 0 ? ntri : 1));
+    if (ntri > 0)
+      ck(hipMemcpyAsync(dt.p, tris, sizeof(double) * 9 * (size_t)ntri, hipMemcpyHostToDevice, st.s), "upload mesh");
+    dc.alloc(sizeof(long long) * (size_t)ncells);
+    ck(hipMemcpyAsync(dc.p, cells, sizeof(long long) * (size_t)ncells, hipMemcpyHostToDevice, st.s), "upload cells");
+    dout.alloc(sizeof(double) * (size_t)ncells);
+    ck(launch_mesh_sqdist(dt.as<double>(), (long long)ntri, dc.as<long long>(), (long long)ncells, nx, ny,
+                          dout.as<double>(), st.s),
+       "mesh distance");
+    ck(hipMemcpyAsync(sqdist, dout.p, sizeof(double) * (size_t)ncells, hipMemcpyDeviceToHost, st.s), "download");
+    ck(hipStreamSynchronize(st.s), "sync");
+  });
+}
+
 int aero_set_f_equilibrium(aero_ctx* c, const double* rho, const double* ux, const double* uy, const double* uz) {
   return guarded([&] {
     c->set_device();

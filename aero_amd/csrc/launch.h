@@ -45,6 +45,12 @@ hipError_t launch_materialize_g(int q, int st, const StepParams& p, double* out,
 hipError_t launch_store_from_f64(int st, int q, long long n, const double* src, void* dst, double shift_scale,
                                  hipStream_t s);
 hipError_t launch_compute_from_f64(int st, long long count, const double* src, void* dst, hipStream_t s);
+// STL geometry (mesh.hip).
+size_t mesh_inside_scratch_bytes(long long ntri);
+hipError_t launch_mesh_inside(const double* tris, long long ntri, void* scratch, const double* pts, long long npts,
+                              int nx, int ny, const double ray[3], uint8_t* out, hipStream_t s);
+hipError_t launch_mesh_sqdist(const double* tris, long long ntri, const long long* cells, long long ncells, int nx,
+                              int ny, double* out, hipStream_t s);
 hipError_t launch_feq_fill(int q, long long n, double* f, const double* rho, double rho0, const double* ux,
                            double ux0, const double* uy, const double* uz, hipStream_t s);
 hipError_t launch_macro_from_compute(int st, int q, long long n, const void* F, void* U, hipStream_t s);

@@ -109,6 +109,25 @@ const char* aero_last_error(void);
 int aero_version(void);
 int aero_device_count(int32_t* n);
 
+/* STL geometry on the device (no context needed).  tris: ntri triangles of
+ * three (x, y, z) vertices, FP64, in grid coordinates.
+ *
+ * aero_mesh_inside: parity of the forward hits of the ray (unit direction
+ * ray[3]) from each point over all triangles, with the Moller-Trumbore test
+ * and eps = 1e-9 of points_inside_mesh (aero/geometry3d/mesh_mask.py:14-45).
+ * points: npts (x, y, z) triples, or NULL for the nz*ny*nx cell centres
+ * (x + 0.5, y + 0.5, z + 0.5), x fastest (MeshMask.mark_solid,
+ * mesh_mask.py:114-124).  inside: one byte per point, 1 = inside.
+ *
+ * aero_mesh_sqdist: squared Euclidean distance from the centre of each listed
+ * cell (linear index z*ny*nx + y*nx + x) to the nearest triangle, as
+ * _sq_dist_points_to_triangle + the minimum over triangles of
+ * compute_phi_field (signed_distance.py:38-137, 222-231). */
+int aero_mesh_inside(const double* tris, int64_t ntri, const double* ray, const double* points, int64_t npts,
+                     int32_t nx, int32_t ny, int32_t nz, uint8_t* inside);
+int aero_mesh_sqdist(const double* tris, int64_t ntri, const int64_t* cells, int64_t ncells, int32_t nx, int32_t ny,
+                     int32_t nz, double* sqdist);
+
 /* Context lifecycle.  Replaces Solver.__init__ / Solver3D.__init__ state
  * allocation (solver.py:143-257, solver3d.py:136-295). */
 int aero_create(const aero_desc* desc, aero_ctx** out);
