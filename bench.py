@@ -154,6 +154,34 @@ def run_form(name, steps, warmup):
     return out
 
 
+def stl_geometry(nz=128, ny=128, nx=256, sample=4000):
+    """STL preprocessing on the device: the sample airplane (3,458 triangles)
+    voxelised and its narrow-band SDF on a 256x128x128 grid, beside the
+    numpy oracle's parity test timed on a bounded sample of cells."""
+    from aero_amd import mesh
+    from oracle import mesh_oracle
+    raw = mesh.load_stl_triangles(str(ROOT / "tests" / "golden" / "stl" / "simple_plane.stl"))
+    tris, _ = mesh.prepare_mesh_triangles(raw, nz, ny, nx, fit_frac=0.35)
+    mesh.grid_inside_mesh(tris, 8, 8, 8)
+    t0 = time.perf_counter()
+    ins = mesh.grid_inside_mesh(tris, nz, ny, nx)
+    t_vox = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    phi = mesh.compute_phi_field(raw, nz, ny, nx, fit_frac=0.35)
+    t_phi = time.perf_counter() - t0
+    cells = nz * ny * nx
+    idx = np.random.default_rng(0).choice(cells, sample, replace=False)
+    t0 = time.perf_counter()
+    ref = mesh_oracle.inside(mesh_oracle.grid_centres(nz, ny, nx)[idx], tris)
+    t_cpu = time.perf_counter() - t0
+    return {"mesh": "simple_plane.stl", "triangles": int(len(tris)), "grid": [nz, ny, nx],
+            "voxelise_s": round(t_vox, 4), "ray_triangle_tests_per_s": round(cells * len(tris) / t_vox, 1),
+            "phi_field_s": round(t_phi, 4), "band_cells": int((np.abs(phi) < 16.0).sum()),
+            "solid_cells": int(ins.sum()), "sample_agrees_with_oracle": bool((ref == ins.reshape(-1)[idx]).all()),
+            "cpu_oracle_voxelise_s_extrapolated": round(t_cpu * cells / sample, 1),
+            "cpu_oracle_sample": f"{sample} cells, numpy, 1 thread"}
+
+
 def copy_calibration(mib=1024, reps=4):
     """A plain float32 device-to-device copy of `mib` MiB, `reps` times: the
     kernel whose FETCH_SIZE / WRITE_SIZE per dispatch (known to be `mib` MiB
@@ -321,6 +349,7 @@ def main():
         for st in ("f32", "f16"):
             extra[f"{st}_512x256x256"] = run_form(f"{st}_512x256x256", 40, 10)
         out["forms"] = extra
+        out["stl_geometry"] = stl_geometry()
     if not a.no_serve:
         out["serving"] = serving(full=a.serve_full)
     if not a.no_cpu:
