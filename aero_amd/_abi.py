@@ -55,6 +55,7 @@ _SIGS = {
     "aero_destroy": ([_P], ctypes.c_int),
     "aero_set_geometry": ([_P, _P, ctypes.c_int64, _P, _P], ctypes.c_int),
     "aero_set_damping": ([_P, _P], ctypes.c_int),
+    "aero_set_ibm": ([_P, _P], ctypes.c_int),
     "aero_set_f": ([_P, _P], ctypes.c_int),
     "aero_get_f": ([_P, _P], ctypes.c_int),
     "aero_set_f_equilibrium": ([_P, _P, _P, _P, _P], ctypes.c_int),

@@ -49,6 +49,14 @@ class DeviceSolver:
         self._nrow = desc.nz * desc.ny
         self._N = desc.nz * desc.ny * desc.nx
 
+    def _set_ibm(self, phi: Optional[np.ndarray]) -> None:
+        arr = None
+        if phi is not None:
+            arr = np.ascontiguousarray(phi, dtype=np.float64).reshape(-1)
+            if arr.size != self._N:
+                raise ValueError(f"phi must have {self._N} cells, got {arr.size}")
+        self._ctx.call("aero_set_ibm", _abi.ptr(arr))
+
     # ------------------------------------------------------------- state
     def _ref_shape(self):
         raise NotImplementedError

@@ -234,6 +234,18 @@ __global__ __launch_bounds__(BLOCK) void f64_to_f32_kernel(long long count, cons
   if (k < count) dst[k] = (float)src[k];
 }
 
+hipError_t launch_ibm_field(int q, int st, long long n, void* F, const uint8_t* flags, double coeff, hipStream_t s) {
+  const dim3 g((unsigned)((n + BLOCK - 1) / BLOCK));
+  if (st == ST_F64) {
+    if (q == 19) ibm_field_kernel<D3Q19, double><<<g, BLOCK, 0, s>>>(n, (double*)F, flags, coeff);
+    else ibm_field_kernel<D2Q9, double><<<g, BLOCK, 0, s>>>(n, (double*)F, flags, coeff);
+  } else {
+    if (q == 19) ibm_field_kernel<D3Q19, float><<<g, BLOCK, 0, s>>>(n, (float*)F, flags, coeff);
+    else ibm_field_kernel<D2Q9, float><<<g, BLOCK, 0, s>>>(n, (float*)F, flags, coeff);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_compute_from_f64(int st, long long count, const double* src, void* dst, hipStream_t s) {
   if (st == ST_F64) {
     return hipMemcpyAsync(dst, src, sizeof(double) * count, hipMemcpyDeviceToDevice, s);

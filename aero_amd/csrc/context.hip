@@ -201,6 +201,8 @@ struct aero_ctx {
   int last_g_skip = 0;
   int g_skip_pending = 0;
   bool has_state = false;  // set_f has defined the state
+  int ibm_on = 0;          // immersed-boundary mode (aero_set_ibm)
+  double ibm_coeff = 0.0;
   // LES split buffers
   DevBuf fsplit, usplit;
   // caches of the reference state in f64
@@ -229,6 +231,8 @@ struct aero_ctx {
     p.ny = d.ny;
     p.nz = d.nz;
     p.nrow = nrow;
+    p.ibm_on = ibm_on;
+    p.ibm_coeff = ibm_coeff;
     p.div_nx = make_div_magic((unsigned)d.nx);
     p.div_ny = make_div_magic((unsigned)d.ny);
     p.ncell = N;
@@ -874,12 +878,16 @@ static void state_from_fcache(aero_ctx* c) {
   {
     // F in compute precision, then B[side] = collide(F) (with LES omega from F)
     const void* F = c->fcache.p;
-    if (c->d.storage != ST_F64) {
-      c->tmp.alloc(sizeof(float) * cnt);
+    const bool own = c->d.storage != ST_F64 || c->ibm_on;  // fcache itself stays the unforced self.f
+    if (own) {
+      c->tmp.alloc((c->d.storage == ST_F64 ? sizeof(double) : sizeof(float)) * cnt);
       ck(launch_compute_from_f64(c->d.storage, (long long)cnt, c->fcache.as<double>(), c->tmp.p, c->stream),
          "convert f");
       F = c->tmp.p;
     }
+    if (c->ibm_on)  // the first step's forcing acts on this state (solver.py:280-285)
+      ck(launch_ibm_field(c->Q, c->d.storage, c->N, c->tmp.p, c->flags.as<uint8_t>(), c->ibm_coeff, c->stream),
+         "ibm forcing");
     const void* U = nullptr;
     if (c->les_split()) {
       ck(launch_macro_from_compute(c->d.storage, c->Q, c->N, F, c->usplit.p, c->stream), "macro");
@@ -887,10 +895,28 @@ static void state_from_fcache(aero_ctx* c) {
     }
     c->collide_field(F, U, c->side);
     ck(hipStreamSynchronize(c->stream), "sync");
-    if (c->d.storage != ST_F64) c->tmp.release();
+    if (own) c->tmp.release();
     c->fcache_valid = true;
     c->has_state = true;  // state defined; fcache holds it until the next step
   }
+}
+
+int aero_set_ibm(aero_ctx* c, const double* phi) {
+  return guarded([&] {
+    c->set_device();
+    if (!c->flags.p) throw std::runtime_error("aero_set_ibm: call aero_set_geometry first");
+    std::vector<uint8_t> fl((size_t)c->N);
+    ck(hipMemcpy(fl.data(), c->flags.p, (size_t)c->N, hipMemcpyDeviceToHost), "read flags");
+    for (long long k = 0; k < c->N; ++k) {
+      fl[k] &= (uint8_t)~FLAG_IBM;
+      // ibm_guo.py:133: skipped where solid, phi <= 0 or phi > 1.5 (a NaN phi is not skipped)
+      if (phi && !(fl[k] & FLAG_SOLID) && !(phi[k] <= 0.0) && !(phi[k] > 1.5)) fl[k] |= FLAG_IBM;
+    }
+    ck(hipMemcpy(c->flags.p, fl.data(), (size_t)c->N, hipMemcpyHostToDevice), "write flags");
+    c->ibm_on = 1;
+    const double tau = 1.0 / c->d.omega;  // solver.py:184
+    c->ibm_coeff = 1.0 - 0.5 / tau;
+  });
 }
 
 int aero_set_f(aero_ctx* c, const double* f) {

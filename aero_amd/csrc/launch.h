@@ -45,6 +45,7 @@ hipError_t launch_materialize_g(int q, int st, const StepParams& p, double* out,
 hipError_t launch_store_from_f64(int st, int q, long long n, const double* src, void* dst, double shift_scale,
                                  hipStream_t s);
 hipError_t launch_compute_from_f64(int st, long long count, const double* src, void* dst, hipStream_t s);
+hipError_t launch_ibm_field(int q, int st, long long n, void* F, const uint8_t* flags, double coeff, hipStream_t s);
 // STL geometry (mesh.hip).
 size_t mesh_inside_scratch_bytes(long long ntri);
 hipError_t launch_mesh_inside(const double* tris, long long ntri, void* scratch, const double* pts, long long npts,

@@ -27,7 +27,7 @@ enum OutletMode { OUT_NONE = 0, OUT_CONVECTIVE = 1, OUT_ZEROGRAD = 2, OUT_PRESSU
 enum WallMode { WALL_NONE = 0, WALL_SLIP = 1, WALL_NOSLIP = 2, WALL_MOVING = 3 };
 enum LesMode { LES_OFF = 0, LES_FD_SMAG = 1, LES_FD_WALE = 2 };
 // Per-cell flags: solid, or a fluid cell with at least one surface link.
-enum CellFlag { FLAG_SOLID = 1, FLAG_LINKS = 2 };
+enum CellFlag { FLAG_SOLID = 1, FLAG_LINKS = 2, FLAG_IBM = 4 };  // FLAG_IBM: fluid cell with 0 < phi <= 1.5
 
 // ------------------------------------------------------------------ lattices
 // D2Q9: aero/lbm/d2q9.py:18-41.  D3Q19: aero/lbm/d3q19.py:36-73.
@@ -294,6 +294,8 @@ struct StepParams {
   long long ncell;
   int nblocks;  // blocks covering ncell (the step launch adds one reduction block)
   int nt_store;  // population stores bypass the caches (launch working set beyond the last-level cache)
+  int ibm_on;      // immersed-boundary mode: no link bounce-back; Guo forcing of FLAG_IBM cells
+  double ibm_coeff;  // 1 - 1/(2 tau)
   const AERO_GLOBAL void* fin;
   AERO_GLOBAL void* fout;
   const AERO_GLOBAL void* gin;

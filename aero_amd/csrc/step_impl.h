@@ -75,6 +75,19 @@ __device__ __forceinline__ AERO_GLOBAL T* scalar_base(AERO_GLOBAL T* q) {
   return q;
 }
 
+// Guo immersed-boundary forcing toward a resting wall (ibm_guo.py:110-182):
+// F = coeff (0 - u) from the cell's own populations (none where rho <= 0),
+// then f_i += w_i e_i.F.
+template <class L, class C>
+__device__ __forceinline__ void ibm_force(C coeff, C fn[L::Q]) {
+  C rho, mx, my, mz;
+  moments<L>(fn, rho, mx, my, mz);
+  if (!(rho > C(0))) return;
+  const C Fx = coeff * (C(0) - mx / rho), Fy = coeff * (C(0) - my / rho), Fz = coeff * (C(0) - mz / rho);
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) fn[i] += C(L::W[i]) * edot<L>(i, Fx, Fy, Fz);
+}
+
 template <class C>
 __device__ __forceinline__ C feq_i(C w, C rho, C eu, C usq) {
   return w * rho * (C(1) + C(3) * eu + C(4.5) * eu * eu - C(1.5) * usq);
@@ -468,7 +481,7 @@ __device__ __forceinline__ int pre_outlet(const StepParams& p, const StepDyn& dy
   int rec = -1;
   if (flag & FLAG_LINKS) {
     rec = link_record(p, cell);
-    apply_links_light<L, ST>(p, B, rec, cell, x, y, z, fn);
+    if (!p.ibm_on) apply_links_light<L, ST>(p, B, rec, cell, x, y, z, fn);  // IBM: solver.py:379-383
   }
   if (x == 0) apply_inlet<L, C>(p, dyn, y, z, fn);
   return rec;
@@ -842,6 +855,8 @@ struct StepWaves {
     const int rec = post_bc<L, ST>(p, dyn, B, cell, x, y, z, flag, true, fn);                              \
     if constexpr (THERMAL) scalar_step<L, ST>(p, dyn, cell, x, y, z, solid, fn, true);                     \
     if (rec >= 0) link_post_sum<L, C_>(p, rec, fn);                                                        \
+    /* the next step's IBM forcing acts on this post-boundary state (solver.py:280-285) */                 \
+    if (p.ibm_on && (flag & FLAG_IBM)) ibm_force<L, C_>((C_)p.ibm_coeff, fn);                             \
     finish_cell<L, COLL, ST, KMODE>(p, cell, wcell_, solid, fn);                                           \
   } while (0)
 
@@ -1114,6 +1129,21 @@ __global__ __launch_bounds__(BLOCK) void collide_kernel(CollideParams p) {
   T* Bo = (T*)p.fout;
 #pragma unroll
   for (int i = 0; i < Q; ++i) S::st(Bo + (size_t)i * N, (unsigned)cell, out[i], (C)p.fsh[fsh_class<L>(i)]);
+}
+
+// IBM forcing applied in place to a compute-precision field (Q x N), for
+// states set from the host before their first collision.
+template <class L, class C>
+__global__ __launch_bounds__(BLOCK) void ibm_field_kernel(long long n, C* __restrict__ F,
+                                                          const uint8_t* __restrict__ flags, double coeff) {
+  const long long c = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (c >= n || !(flags[c] & FLAG_IBM)) return;
+  C fn[L::Q];
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) fn[i] = F[(size_t)i * n + c];
+  ibm_force<L, C>((C)coeff, fn);
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) F[(size_t)i * n + c] = fn[i];
 }
 
 }  // namespace aero

@@ -71,8 +71,6 @@ class Solver(DeviceSolver):
             raise ValueError("les_model must be 'smagorinsky' or 'wale'.")
         self.les_model = str(les_model)
         self.ibm_enabled = bool(ibm_enabled)
-        if self.ibm_enabled:
-            raise NotImplementedError("immersed-boundary forcing (ibm_enabled=True) is not part of this build")
         self.phi = phi
         self.bouzidi = bool(bouzidi)
         self.van_driest, self.van_driest_A = bool(van_driest), float(van_driest_A)
@@ -152,6 +150,10 @@ class Solver(DeviceSolver):
         l4 = np.zeros((len(self.surface_links), 4), np.int32)
         l4[:, 0], l4[:, 2], l4[:, 3] = self.surface_links[:, 0], self.surface_links[:, 1], self.surface_links[:, 2]
         self._init_device(d, self.solid[None], l4, self.q_vals, damp)
+        if self.ibm_enabled:
+            # Guo IBM: links stop bouncing back; forcing only where phi is given
+            # (the reference's `ibm_enabled and phi is not None`)
+            self._set_ibm(self.phi)
 
         # f = feq(rho0, u0, 0) (solver.py:218-223), evaluated on the device;
         # the outlet memory is its last column

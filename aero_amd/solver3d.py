@@ -61,8 +61,6 @@ class Solver3D(DeviceSolver):
             raise ValueError("les_model must be 'smagorinsky' or 'wale'.")
         self.les_model = str(les_model)
         self.ibm_enabled = bool(ibm_enabled)
-        if self.ibm_enabled:
-            raise NotImplementedError("immersed-boundary forcing (ibm_enabled=True) is not part of this build")
         self.phi = phi
         self.bouzidi = bool(bouzidi)
         self.van_driest, self.van_driest_A = bool(van_driest), float(van_driest_A)
@@ -145,6 +143,10 @@ class Solver3D(DeviceSolver):
         if self.les and self.van_driest and phi is not None:
             damp = (1.0 - np.exp(-np.abs(phi) / self.van_driest_A)) ** 2
         self._init_device(d, self.solid, self.surface_links, self.q_vals, damp)
+        if self.ibm_enabled:
+            # Guo IBM: links stop bouncing back; forcing only where phi is given
+            # (the reference's `ibm_enabled and phi is not None`)
+            self._set_ibm(self.phi)
 
         sh = (Nz, Ny, Nx)
         # f = feq(rho0, u0, 0) (solver3d.py:239-247), evaluated on the device;

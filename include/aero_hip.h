@@ -139,6 +139,13 @@ int aero_set_geometry(aero_ctx* ctx, const uint8_t* solid, int64_t nlinks, const
                       const float* q_vals);
 /* Van Driest factor (1 - exp(-|phi|/A))^2 per cell (les.py:512-514), or NULL. */
 int aero_set_damping(aero_ctx* ctx, const double* damp);
+/* Immersed-boundary mode (ibm_enabled=True, solver.py:280-285, 379-383,
+ * solver3d.py:328-333, 426): the surface links stop bouncing back (they
+ * still carry the momentum-exchange sums) and, when phi (N doubles, the
+ * SDF) is given, the fluid cells with 0 < phi <= 1.5 receive Guo forcing
+ * toward a resting wall before every collision (ibm_guo.py:110-182).  Call
+ * after aero_set_geometry and before the state is set. */
+int aero_set_ibm(aero_ctx* ctx, const double* phi);
 
 /* State: the reference's self.f (post-boundary, pre-collision), self.g and
  * self._f_outlet_prev, read and written in the reference layout. */

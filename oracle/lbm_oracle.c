@@ -453,6 +453,34 @@ int orc_body_force(int Q, int nz, int ny, int nx, double *f, const uint8_t *soli
     return 0;
 }
 
+/* Guo immersed-boundary forcing toward a resting wall (ibm_guo.py:110-182):
+   on fluid cells with 0 < phi <= 1.5 (a NaN phi is not skipped, as there),
+   F = (1 - 1/(2 tau)) (0 - u) and f_i += w_i e_i.F, with rho and u of the
+   cell's own populations (skipped where rho <= 0). */
+int orc_ibm_guo(int Q, int nz, int ny, int nx, double *f, const double *phi, const uint8_t *solid,
+                double tau, const int *ex, const int *ey, const int *ez, const double *w) {
+    const size_t N = (size_t)nz * ny * nx;
+    const double delta = 1.5;
+    const double coeff = 1.0 - 0.5 / tau;
+#pragma omp parallel for schedule(static)
+    for (long long c = 0; c < (long long)N; ++c) {
+        if (solid[c] || phi[c] <= 0.0 || phi[c] > delta) continue;
+        double rho = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+        for (int i = 0; i < Q; ++i) {
+            double fi = f[(size_t)i * N + c];
+            rho += fi;
+            mx += ex[i] * fi;
+            my += ey[i] * fi;
+            mz += ez[i] * fi;
+        }
+        if (rho <= 0.0) continue;
+        double Fx = coeff * (0.0 - mx / rho), Fy = coeff * (0.0 - my / rho), Fz = coeff * (0.0 - mz / rho);
+        for (int i = 0; i < Q; ++i)
+            f[(size_t)i * N + c] += w[i] * (ex[i] * Fx + ey[i] * Fy + ez[i] * Fz);
+    }
+    return 0;
+}
+
 /* Macroscopic fields with the rho > 0 guard (d2q9.py:44-63); u is not zeroed
    in solids. Outputs (N,) arrays; uz may be NULL. */
 int orc_macroscopic(int Q, int nz, int ny, int nx, const double *f, const int *ex, const int *ey,

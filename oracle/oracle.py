@@ -226,7 +226,7 @@ class OracleSolver:
                  wall_velocity_top=0.0, wall_velocity_bottom=0.0, inlet_mode="zouhe",
                  thermal=False, T_hot=1.0, T_cold=0.0, alpha_T=1e-3, buoyancy=False,
                  g_gravity=0.0, beta=1e-3, T_ref=0.5, links=None, q_vals=None, trt_variant="reference",
-                 bouzidi_variant="reference"):
+                 bouzidi_variant="reference", ibm=False):
         self.is3d = solid.ndim == 3
         self.solid = np.ascontiguousarray(solid, dtype=bool)
         s3 = self.solid if self.is3d else self.solid[None]
@@ -251,6 +251,9 @@ class OracleSolver:
         self.les, self.les_cs = bool(les), float(les_cs)
         self.les_model = {"smagorinsky": 0, "wale": 1}[les_model]
         self.phi = phi
+        # Guo IBM (solver.py:280-285, 379-383): forcing of band cells before the
+        # collision when phi is given; the surface links never bounce back
+        self.ibm = bool(ibm)
         self.van_driest, self.van_driest_A = bool(van_driest), float(van_driest_A)
         self.body_force = tuple(float(v) for v in body_force)
         self.u_top, self.u_bot = float(wall_velocity_top), float(wall_velocity_bottom)
@@ -386,6 +389,11 @@ class OracleSolver:
         L = lib()
         d = ctypes.c_double
         f = _f64(self.f)
+        if self.ibm and self.phi is not None:
+            f = f.copy()
+            phi3 = _f64(np.asarray(self.phi, dtype=np.float64).reshape(nz, ny, nx))
+            L.orc_ibm_guo(Q, nz, ny, nx, _p(f), _p(phi3), _p(self.solid_u8), d(self.tau), _p(self.ex), _p(self.ey),
+                          _p(self.ez), _p(self.w))
         om = None
         if self.les and self.collision != 2:  # MRT never receives omega_field
             om = self.omega_field(f)
@@ -395,9 +403,10 @@ class OracleSolver:
             L.orc_body_force(Q, nz, ny, nx, _p(fnew), _p(self.solid_u8), d(self.tau),
                              d(self.body_force[0]), d(self.body_force[1]), d(self.body_force[2]),
                              _p(self.ex), _p(self.ey), _p(self.ez), _p(self.w))
-        L.orc_bounce_back(Q, nz, ny, nx, _p(fnew), _p(fpre), ctypes.c_longlong(len(self.links4)),
-                          _p(self.links4), _p(self.q_vals) if self.bouzidi else None,
-                          _p(self.ex), _p(self.ey), _p(self.ez), _p(self.opp), self.bouzidi_variant)
+        if not self.ibm:
+            L.orc_bounce_back(Q, nz, ny, nx, _p(fnew), _p(fpre), ctypes.c_longlong(len(self.links4)),
+                              _p(self.links4), _p(self.q_vals) if self.bouzidi else None,
+                              _p(self.ex), _p(self.ey), _p(self.ez), _p(self.opp), self.bouzidi_variant)
         step_no = self.step_count + 1
         if self.is3d:
             if self.streamwise_bc == "open":

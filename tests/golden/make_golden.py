@@ -3,7 +3,8 @@
 Run once, in a container that holds the reference checkout at
 /root/reference (it is never needed at test time):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          # every fixture
+    python tests/golden/make_golden.py ibm      # only the step cases named *ibm*
 
 The reference is pure Python.  Numba is absent here, so every call takes the
 reference's NumPy fallback path, which its own tests hold equal to the Numba
@@ -106,6 +107,9 @@ CASES_2D = {
         kw=dict(collision="bgk", synthetic_inflow=True, synthetic_inflow_intensity=0.05),
         geom="cyl"),
     "bgk_polygon": dict(kw=dict(collision="bgk"), geom="poly"),
+    "bgk_ibm": dict(kw=dict(collision="bgk", ibm_enabled=True), geom="cyl"),
+    "mrt_ibm_zg_sponge": dict(
+        kw=dict(collision="mrt", ibm_enabled=True, outlet_bc="zerogradient", sponge_thickness=6), geom="rect"),
 }
 
 CASES_3D = {
@@ -137,6 +141,8 @@ CASES_3D = {
     "bgk_synth_inflow": dict(
         kw=dict(collision="bgk", synthetic_inflow=True, synthetic_inflow_intensity=0.05),
         geom="sphere"),
+    "bgk_ibm": dict(kw=dict(collision="bgk", ibm_enabled=True), geom="sphere"),
+    "trt_ibm_les": dict(kw=dict(collision="trt", ibm_enabled=True, les=True, les_cs=0.2), geom="box", omega=1.3),
 }
 
 NY2, NX2 = 20, 40
@@ -175,7 +181,7 @@ def run_2d(R, name, spec, seed):
     u0 = 0.05
     omega = spec.get("omega", 1.0 / (3.0 * u0 * D / 60.0 + 0.5))
     kw = dict(spec["kw"])
-    use_phi = kw.get("bouzidi", False) or kw.get("van_driest", False)
+    use_phi = kw.get("bouzidi", False) or kw.get("van_driest", False) or kw.get("ibm_enabled", False)
     s = R["Solver"](Ny=NY2, Nx=NX2, solid=solid, omega=omega, u0=u0, D=D,
                     backend="numpy", phi=phi if use_phi else None, **kw)
     f0, uy0 = noisy_feq_2d(R, NY2, NX2, u0, seed)
@@ -212,7 +218,7 @@ def run_3d(R, name, spec, seed):
     u0 = 0.05
     omega = spec.get("omega", 1.0 / (3.0 * u0 * D / 40.0 + 0.5))
     kw = dict(spec["kw"])
-    use_phi = kw.get("bouzidi", False)
+    use_phi = kw.get("bouzidi", False) or kw.get("ibm_enabled", False)
     s = R["Solver3D"](Nz=NZ3, Ny=NY3, Nx=NX3, solid=solid, omega=omega, u0=u0, D=D,
                       backend="numpy", phi=phi if use_phi else None, **kw)
     f0, uy0, uz0 = noisy_feq_3d(R, NZ3, NY3, NX3, u0, seed)
@@ -283,21 +289,30 @@ def sem_fixture(R):
     return dict(planes=np.array(planes), xk=sem.xk, yk=sem.yk, zk=sem.zk, eps=sem.eps)
 
 
-def main():
+def main(only=None):
+    """Write every fixture, or with `only` (a substring) just the matching
+    step cases; a case's seed is its position in its table, so adding cases at
+    the end leaves the existing fixtures' inputs unchanged."""
     R = _import_reference()
-    np.savez_compressed(OUT / "mrt_inverses.npz", M9_inv=R["M9_inv"], M19_inv=R["M19_inv"])
+    if only is None:
+        np.savez_compressed(OUT / "mrt_inverses.npz", M9_inv=R["M9_inv"], M19_inv=R["M19_inv"])
     for k, (name, spec) in enumerate(CASES_2D.items()):
+        if only is not None and only not in name:
+            continue
         out = run_2d(R, name, spec, seed=100 + k)
         np.savez_compressed(OUT / f"ref2d_{name}.npz", **out)
         print("2d", name, "links", len(out["links"]), "Cd", out["Cd_hist"][-1], flush=True)
     for k, (name, spec) in enumerate(CASES_3D.items()):
+        if only is not None and only not in name:
+            continue
         out = run_3d(R, name, spec, seed=200 + k)
         np.savez_compressed(OUT / f"ref3d_{name}.npz", **out)
         print("3d", name, "links", len(out["links"]), "Cd", out["Cd_hist"][-1], flush=True)
-    np.savez_compressed(OUT / "links_c1_c2.npz", **links_fixtures(R))
-    np.savez_compressed(OUT / "sem_inlet.npz", **sem_fixture(R))
+    if only is None:
+        np.savez_compressed(OUT / "links_c1_c2.npz", **links_fixtures(R))
+        np.savez_compressed(OUT / "sem_inlet.npz", **sem_fixture(R))
     print("done")
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1] if len(sys.argv) > 1 else None)

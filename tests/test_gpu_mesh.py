@@ -121,3 +121,38 @@ def test_stl_obstacle_runs_in_the_solver():
     r = s.run(steps=200, check_every=100, verbose=False)
     assert np.isfinite(r["Cd_mean"]) and r["Cd_mean"] > 0
     s.close()
+
+
+def test_ibm_short_run_on_the_cube():
+    """tests/test_ibm.py:46-59: Guo IBM with an STL SDF runs without NaN."""
+    import aero_amd
+    from aero_amd import mesh
+    phi = mesh.compute_phi_field(mesh.load_stl_triangles(str(STL / "unit_cube.stl")), 12, 12, 24)
+    s = aero_amd.Solver3D(Nz=12, Ny=12, Nx=24, solid=phi <= 0, omega=1.0, u0=0.05, D=4.0, ibm_enabled=True,
+                          phi=phi)
+    print("[gpu] ibm cube", flush=True)
+    r = s.run(steps=20, check_every=20, verbose=False)
+    assert not np.isnan(r["Cd_mean"])
+
+
+def test_ibm_and_voxel_drag_agree_at_re20():
+    """tests/test_signed_distance.py:180-264: on the unit-sphere STL at Re 20
+    both the voxel (bounce-back) and the IBM run give Cd in [1, 8], within
+    40% of each other."""
+    import aero_amd
+    from aero_amd import mesh
+    nz, ny, nx, fit, u0 = 16, 16, 32, 0.4, 0.05
+    phi = mesh.compute_phi_field(mesh.load_stl_triangles(str(STL / "unit_sphere.stl")), nz, ny, nx, fit_frac=fit,
+                                 mesh_orient="none", band_width=3, far_value=16.0)
+    solid = phi <= 0.0
+    D = 2.0 * fit * min(ny, nz) / 2.0
+    omega = 1.0 / (3.0 * u0 * D / 20.0 + 0.5)
+    cds = []
+    for ibm in (False, True):
+        s = aero_amd.Solver3D(Nz=nz, Ny=ny, Nx=nx, solid=solid, omega=omega, u0=u0, D=D, collision="bgk",
+                              ibm_enabled=ibm, phi=phi if ibm else None)
+        print(f"[gpu] re20 ibm={ibm}", flush=True)
+        cds.append(s.run(steps=400, check_every=400, verbose=False)["Cd_mean"])
+    print(f"[gpu] re20 Cd voxel {cds[0]:.3f} ibm {cds[1]:.3f}", flush=True)
+    assert all(1.0 <= c <= 8.0 for c in cds)
+    assert abs(cds[1] - cds[0]) / max(abs(cds[0]), 1e-6) < 0.40

@@ -29,7 +29,7 @@ def _aero():
 def make_solver(dim, case, storage="f64"):
     A = _aero()
     kw = dict(case["kw"])
-    use_phi = kw.get("bouzidi", False) or kw.get("van_driest", False)
+    use_phi = kw.get("bouzidi", False) or kw.get("van_driest", False) or kw.get("ibm_enabled", False)
     phi = case.get("phi") if use_phi else None
     solid = case["solid"]
     args = dict(omega=float(case["omega"]), u0=float(case["u0"]), D=float(case["D"]), phi=phi,

@@ -6,6 +6,8 @@ boundary order) fails one of the composite-step cases below, each of which
 exercises a different collision/boundary combination.
 """
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -88,9 +90,10 @@ def test_links_brute_force_small():
 def make_oracle(dim, case):
     kw = dict(case["kw"])
     solid = case["solid"]
-    use_phi = kw.get("bouzidi", False) or kw.get("van_driest", False)
+    use_phi = kw.get("bouzidi", False) or kw.get("van_driest", False) or kw.get("ibm_enabled", False)
     phi = case.get("phi") if use_phi else None
-    args = dict(omega=float(case["omega"]), u0=float(case["u0"]), D=float(case["D"]), phi=phi)
+    args = dict(omega=float(case["omega"]), u0=float(case["u0"]), D=float(case["D"]), phi=phi,
+                ibm=bool(kw.pop("ibm_enabled", False)))
     inlet_mode = "zouhe"
     if kw.pop("synthetic_inflow", False):
         inlet_mode = "field"
@@ -239,3 +242,35 @@ def test_linear_bouzidi_puts_the_wall_at_the_link_fraction(q):
     else:
         assert err < 3e-2
         assert _poiseuille(q, qshift=1.0 - q) > 0.1
+
+
+def test_ibm_forcing_closed_forms():
+    """Guo forcing f_i += w_i e_i.F (ibm_guo.py:110-182) keeps each band cell's
+    mass, adds F/3 to its momentum (sum_i w_i e_i e_i = I/3) with
+    F = (1 - 1/(2 tau)) (0 - u), and leaves solid, far and phi <= 0 cells alone."""
+    rng = np.random.default_rng(12)
+    for Q, E, W in ((9, orc.E9, orc.W9), (19, orc.E19, orc.W19)):
+        nz, ny, nx = (1, 5, 7) if Q == 9 else (3, 4, 5)
+        shape = (Q, nz, ny, nx)
+        rho = 1.0 + rng.uniform(-0.05, 0.05, shape[1:])
+        vel = [rng.uniform(-0.1, 0.1, shape[1:]) for _ in range(E.shape[1])]
+        f = orc.feq(E, W, rho, *vel)
+        f += rng.uniform(-1e-3, 1e-3, shape)
+        phi = rng.uniform(-1.0, 3.0, shape[1:])
+        solid = (rng.random(shape[1:]) < 0.2).astype(np.uint8)
+        tau = 0.8
+        g = f.copy()
+        ex, ey = np.ascontiguousarray(E[:, 0], np.int32), np.ascontiguousarray(E[:, 1], np.int32)
+        ez = np.ascontiguousarray(E[:, 2], np.int32) if Q == 19 else np.zeros(9, np.int32)
+        orc.lib().orc_ibm_guo(Q, nz, ny, nx, orc._p(g), orc._p(np.ascontiguousarray(phi)), orc._p(solid),
+                              ctypes.c_double(tau), orc._p(ex), orc._p(ey), orc._p(ez),
+                              orc._p(np.ascontiguousarray(W)))
+        band = (solid == 0) & (phi > 0) & (phi <= 1.5)
+        np.testing.assert_array_equal(g[:, ~band], f[:, ~band])
+        r0 = f.sum(axis=0)
+        np.testing.assert_allclose(g.sum(axis=0)[band], r0[band], rtol=0, atol=1e-15)
+        for a in range(E.shape[1]):
+            m0 = np.tensordot(E[:, a].astype(float), f, axes=(0, 0))
+            m1 = np.tensordot(E[:, a].astype(float), g, axes=(0, 0))
+            F = (1.0 - 0.5 / tau) * (0.0 - m0 / r0)
+            np.testing.assert_allclose((m1 - m0)[band], F[band] / 3.0, rtol=1e-12, atol=1e-16)
