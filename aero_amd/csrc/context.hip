@@ -184,7 +184,7 @@ struct aero_ctx {
   int side = 0;
   int prev_side = 0;
   // geometry
-  DevBuf flags, lrec, rec_of, qdir, fpart[2], sigma, minv_d, minv_f, damp;
+  DevBuf flags, wave_any, lrec, rec_of, qdir, fpart[2], sigma, minv_d, minv_f, damp;
   int fpart_side = 0;  // buffer holding the last completed step's per-link-cell force sums
   int nrec = 0;
   std::vector<int> rec_cells;
@@ -202,6 +202,7 @@ struct aero_ctx {
   int g_skip_pending = 0;
   bool has_state = false;  // set_f has defined the state
   int ibm_on = 0;          // immersed-boundary mode (aero_set_ibm)
+  int sponge_x0 = 0;       // first sponge column (nx: none)
   double ibm_coeff = 0.0;
   // LES split buffers
   DevBuf fsplit, usplit;
@@ -233,11 +234,13 @@ struct aero_ctx {
     p.nrow = nrow;
     p.ibm_on = ibm_on;
     p.ibm_coeff = ibm_coeff;
+    p.sponge_x0 = sponge_x0;
     p.div_nx = make_div_magic((unsigned)d.nx);
     p.div_ny = make_div_magic((unsigned)d.ny);
     p.ncell = N;
     p.nblocks = nblocks;
     p.flags = (decltype(p.flags))(flags.as<uint8_t>());
+    p.wave_any = (decltype(p.wave_any))(wave_any.as<uint32_t>());
     p.lrec = (decltype(p.lrec))(lrec.as<LinkCell>());
     p.rec_of = (decltype(p.rec_of))(rec_of.as<int>());
     p.qdir = (decltype(p.qdir))(qdir.as<float>());
@@ -672,6 +675,10 @@ int aero_create(const aero_desc* desc, aero_ctx** out) {
     }
     c->sigma.alloc(sizeof(double) * d.nx);
     ck(hipMemcpy(c->sigma.p, sig.data(), sizeof(double) * d.nx, hipMemcpyHostToDevice), "copy sigma");
+    c->sponge_x0 = d.nx;
+    if (d.sponge_thickness > 0)
+      for (int x = d.nx - 1; x >= 0; --x)
+        if (sig[x] > 0.0) c->sponge_x0 = x;
     c->minv_d.alloc(sizeof(double) * 361);
     c->minv_f.alloc(sizeof(float) * 361);
     std::vector<float> mf(361);
@@ -712,7 +719,7 @@ int aero_destroy(aero_ctx* c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags,
+  DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags, &c->wave_any,
                     &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
                     &c->damp, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
                     &c->gcache, &c->tmp, &c->checkpart, &c->checkout};
@@ -798,6 +805,20 @@ int aero_batch_step(aero_batch* b, int32_t n, aero_ctx* const* ctxs, int32_t nst
 
 }  // extern "C"
 
+// Upload the cell flags and their per-wave summary: bit w is set when any of
+// the 64 cells [64 w, 64 w + 64) is flagged, so a step wave can tell from one
+// scalar load that every cell it covers is plain fluid.
+static void upload_flags(aero_ctx* c, const std::vector<uint8_t>& fl) {
+  ck(hipMemcpy(c->flags.p, fl.data(), (size_t)c->N, hipMemcpyHostToDevice), "copy flags");
+  const size_t nwave = ((size_t)c->N + 63) / 64;
+  std::vector<uint32_t> words((nwave + 31) / 32, 0u);
+  for (size_t k = 0; k < (size_t)c->N; ++k)
+    if (fl[k]) words[(k >> 6) >> 5] |= 1u << ((k >> 6) & 31);
+  c->wave_any.alloc(sizeof(uint32_t) * words.size());
+  ck(hipMemcpy(c->wave_any.p, words.data(), sizeof(uint32_t) * words.size(), hipMemcpyHostToDevice),
+     "copy wave flags");
+}
+
 // Cell flags and link-cell records for one geometry.
 static void set_geometry_impl(aero_ctx* c, const uint8_t* solid, int64_t nlinks, const int32_t* links,
                               const float* q) {
@@ -839,7 +860,7 @@ static void set_geometry_impl(aero_ctx* c, const uint8_t* solid, int64_t nlinks,
       c->rec_cells.push_back(r.cell);
       c->rec_masks.push_back(r.mask);
     }
-    ck(hipMemcpy(c->flags.p, fl.data(), (size_t)c->N, hipMemcpyHostToDevice), "copy flags");
+    upload_flags(c, fl);
     c->lrec.alloc(sizeof(LinkCell) * std::max<size_t>(1, recs.size()));
     c->qdir.alloc(sizeof(float) * std::max<size_t>(1, qv.size()));
     c->rec_of.alloc(sizeof(int) * (size_t)c->N);
@@ -912,7 +933,7 @@ int aero_set_ibm(aero_ctx* c, const double* phi) {
       // ibm_guo.py:133: skipped where solid, phi <= 0 or phi > 1.5 (a NaN phi is not skipped)
       if (phi && !(fl[k] & FLAG_SOLID) && !(phi[k] <= 0.0) && !(phi[k] > 1.5)) fl[k] |= FLAG_IBM;
     }
-    ck(hipMemcpy(c->flags.p, fl.data(), (size_t)c->N, hipMemcpyHostToDevice), "write flags");
+    upload_flags(c, fl);
     c->ibm_on = 1;
     const double tau = 1.0 / c->d.omega;  // solver.py:184
     c->ibm_coeff = 1.0 - 0.5 / tau;
@@ -1285,7 +1306,7 @@ int aero_synchronize(aero_ctx* c) {
 
 int aero_device_bytes(aero_ctx* c, int64_t* bytes) {
   return guarded([&] {
-    const DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags,
+    const DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags, &c->wave_any,
                             &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
                             &c->damp, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
                             &c->gcache, &c->tmp, &c->checkpart, &c->checkout};

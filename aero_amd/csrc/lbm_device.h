@@ -295,6 +295,7 @@ struct StepParams {
   int nblocks;  // blocks covering ncell (the step launch adds one reduction block)
   int nt_store;  // population stores bypass the caches (launch working set beyond the last-level cache)
   int ibm_on;      // immersed-boundary mode: no link bounce-back; Guo forcing of FLAG_IBM cells
+  int sponge_x0;   // first column with sponge sigma > 0 (nx when there is none)
   double ibm_coeff;  // 1 - 1/(2 tau)
   const AERO_GLOBAL void* fin;
   AERO_GLOBAL void* fout;
@@ -303,6 +304,7 @@ struct StepParams {
   AERO_GLOBAL void* fsplit;  // split mode: post-BC populations, compute type, Q x N
   AERO_GLOBAL void* usplit;  // split mode: ux, uy, uz, compute type, 3 x N
   const AERO_GLOBAL uint8_t* flags;
+  const AERO_GLOBAL uint32_t* wave_any;  // bit w: some cell of [64 w, 64 w + 64) is flagged
   const AERO_GLOBAL LinkCell* lrec;
   const AERO_GLOBAL int* rec_of;   // per cell: its link record (read only where FLAG_LINKS is set)
   const AERO_GLOBAL float* qdir;   // Q-1 wall fractions per link record

@@ -815,7 +815,7 @@ __device__ __forceinline__ void reduce_forces_block(const ForceReduce& r) {
 // pull + collide path needs ~50, the rare link / outlet / reduction paths
 // would otherwise raise the whole kernel to ~85).  FP64 and MRT keep the
 // compiler's choice.
-template <int COLL, int ST>
+template <class L, int COLL, int ST>
 struct StepWaves {
   static constexpr int value = (ST != ST_F64 && COLL != COLL_MRT) ? 7 : 1;
 };
@@ -846,9 +846,27 @@ struct StepWaves {
     const int cell = (int)cu_;                                                                             \
     int x, y, z;                                                                                           \
     cell_xyz(p, cu_, x, y, z);                                                                             \
+    const T_* B = (const T_*)p.fin;                                                                        \
+    /* Bulk waves: no cell of the wave flagged (solid, link, IBM; one bit per 64 cells, a scalar */        \
+    /* load) and no lane at a domain edge or in the sponge -> pull, [body force,] collide, store, */      \
+    /* straight-line.  Every boundary condition below is a no-op for such a cell, so this is the */       \
+    /* same arithmetic; being a separate copy, the compiler may contract it into fused multiply-adds */   \
+    /* differently, so a rare population can differ in its last bit from the full sequence.  MRT */      \
+    /* kernels (register-bound) keep the single full sequence. */                                         \
+    if constexpr (!THERMAL && COLL != COLL_MRT) {                                                          \
+      const unsigned w_ = wcell_ >> 6;                                                                     \
+      const bool edge_ = x == 0 || x == p.nx - 1 || x >= p.sponge_x0 ||                                    \
+                         (p.wall_mode != WALL_NONE && (y == 0 || y == p.ny - 1));                          \
+      if (!((p.wave_any[w_ >> 5] >> (w_ & 31)) & 1u) && __builtin_amdgcn_ballot_w64(edge_) == 0) {        \
+        C_ fq_[L::Q];                                                                                      \
+        gather<L, ST>(p, B, x, y, z, (C_)p.u0, fq_);                                                       \
+        if (L::DIM == 3 && p.body_force_on) body_force<L, C_>(p, fq_);                                     \
+        finish_cell<L, COLL, ST, KMODE>(p, cell, wcell_, false, fq_);                                      \
+        break;                                                                                             \
+      }                                                                                                    \
+    }                                                                                                      \
     const uint8_t flag = p.flags[cell];                                                                    \
     const bool solid = flag & FLAG_SOLID;                                                                  \
-    const T_* B = (const T_*)p.fin;                                                                        \
     if (flag & FLAG_LINKS) link_pre_terms<L, ST>(p, B, link_record(p, cell), cell);                       \
     __builtin_amdgcn_sched_barrier(0);                                                                     \
     C_ fn[L::Q];                                                                                           \
@@ -862,7 +880,7 @@ struct StepWaves {
 
 // One case, parameters by value (split LES step, recycling inlet).
 template <class L, int COLL, int ST, bool THERMAL, int KMODE>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(StepWaves<COLL, ST>::value)))
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(StepWaves<L, COLL, ST>::value)))
 void step_kernel(StepParams p) {
   const StepDyn dyn = dyn_of(p);
   const int blk = blockIdx.x;
@@ -876,7 +894,7 @@ void step_kernel(StepParams p) {
 // A single case runs through this same kernel as a batch of one, so a case
 // computes the same bits alone or among others.
 template <class L, int COLL, int ST, bool THERMAL>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(StepWaves<COLL, ST>::value)))
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(StepWaves<L, COLL, ST>::value)))
 void batch_step_kernel(const BatchCase* __restrict__ cases, const int2* __restrict__ tiles, int k) {
   const int2 t = tiles[blockIdx.x];
   const BatchCase& bc = cases[t.x];
