@@ -471,6 +471,40 @@ __device__ __forceinline__ void sponge_and_walls(const StepParams& p, int x, int
 // State of a cell after streaming, body force, links and inlet: what the
 // outlet and the recycling inlet read at x = nx-2 (boundary.py:68-113,
 // boundary3d.py:185-190).  Returns the cell's link record or -1.
+// Value of lane `i` of a float held in a VGPR, as a wave-uniform value.
+__device__ __forceinline__ float lane_value(float v, int i) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), i));
+}
+
+// The previous convective-outlet state of the one outlet row in a wave,
+// spread over lanes 0..Q-1 (lane i: population i), loaded with one coalesced
+// instruction before the pull so its latency overlaps the pull's.  Only for
+// waves holding exactly one x = nx-1 lane (always so when nx > 64) and the
+// 2-byte storage forms, whose outlet waves are latency-bound (the FP32 form
+// measured slower with the extra live register); otherwise `coop` is false
+// and the outlet lane loads its own state.
+template <class L, int ST>
+struct OutletRow {
+  using C = typename Storage<ST>::C;
+  C pv = 0;
+  bool coop = false;
+  int src = 0;  // the outlet lane
+  __device__ __forceinline__ OutletRow(const StepParams& p, int x, int y, int z) {
+    if constexpr (sizeof(C) == 4 && sizeof(typename Storage<ST>::T) == 2) {
+      if (p.outlet_mode == OUT_CONVECTIVE) {
+        const uint64_t m = __builtin_amdgcn_ballot_w64(x == p.nx - 1);
+        if (m != 0 && (m & (m - 1)) == 0) {
+          src = __builtin_ctzll(m);
+          const int row = __builtin_amdgcn_readlane(z * p.ny + y, src);
+          const unsigned lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+          if (lane < (unsigned)L::Q) pv = (C)p.prev_in[(size_t)lane * p.nrow + row];
+          coop = true;
+        }
+      }
+    }
+  }
+};
+
 template <class L, int ST>
 __device__ __forceinline__ int pre_outlet(const StepParams& p, const StepDyn& dyn,
                                           const typename Storage<ST>::T* __restrict__ B, int cell,
@@ -493,19 +527,33 @@ __device__ __forceinline__ int pre_outlet(const StepParams& p, const StepDyn& dy
 // x = nx-2 (boundary.py:68-113), so that column pulls the x = nx-2 state
 // instead of its own.  With commit set the convective outlet's state is
 // advanced.  Returns the cell's own link record or -1.
+//
+// `nbr_flags` false: the x = nx-2 source of an outlet lane is known to be
+// unflagged (no flag load).  `coop`: the wave holds exactly one outlet lane
+// and lane i of `pv` already holds population i of that row's previous
+// outlet state (OutletRow), so the outlet lane reads it across lanes instead
+// of issuing its own dependent loads after the pull.
 template <class L, int ST>
 __device__ __forceinline__ int post_bc(const StepParams& p, const StepDyn& dyn,
                                        const typename Storage<ST>::T* __restrict__ B, int cell,
                                        int x, int y, int z, uint8_t flag, bool commit,
-                                       typename Storage<ST>::C fn[L::Q]) {
+                                       typename Storage<ST>::C fn[L::Q], bool nbr_flags = true,
+                                       typename Storage<ST>::C pv = 0, bool coop = false, int src = 0) {
   using C = typename Storage<ST>::C;
   constexpr int Q = L::Q;
   const int row = z * p.ny + y;
   const bool redirect = x == p.nx - 1 && (p.outlet_mode == OUT_ZEROGRAD || p.outlet_mode == OUT_CONVECTIVE);
   const int xg = redirect ? p.nx - 2 : x;
   const int cg = redirect ? cell - 1 : cell;
-  const uint8_t fg = redirect ? p.flags[cg] : flag;
+  const uint8_t fg = redirect ? (nbr_flags ? p.flags[cg] : (uint8_t)0) : flag;
   int rec = pre_outlet<L, ST>(p, dyn, B, cg, xg, y, z, fg, fn);
+  C pvs[Q];  // wave-uniform copies of the outlet row's previous state (coop only)
+  if constexpr (sizeof(C) == 4) {
+    if (coop) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) pvs[i] = lane_value(pv, i);
+    }
+  }
   if (redirect) {
     if (p.outlet_mode == OUT_CONVECTIVE) {  // boundary.py:78-113
       const double u = p.u_conv < 0.0 ? 0.0 : (p.u_conv > 1.0 ? 1.0 : p.u_conv);
@@ -513,12 +561,28 @@ __device__ __forceinline__ int post_bc(const StepParams& p, const StepDyn& dyn,
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
         const size_t k = (size_t)i * p.nrow + row;
-        const C v = (C(1) - cu) * (C)p.prev_in[k] + cu * fn[i];
+        C prev_i;
+        if constexpr (sizeof(C) == 4) prev_i = coop ? pvs[i] : (C)p.prev_in[k];
+        else prev_i = (C)p.prev_in[k];
+        const C v = (C(1) - cu) * prev_i + cu * fn[i];
         fn[i] = v;
-        if (commit) p.prev_out[k] = (double)v;
+        if (commit && !coop) p.prev_out[k] = (double)v;
       }
     }  // zero gradient (boundary.py:68-75): fn already is the x = nx-2 state
     rec = (flag & FLAG_LINKS) ? link_record(p, cell) : -1;
+  }
+  if constexpr (sizeof(C) == 4) {
+    if (coop && commit) {  // lane i stores population i of the outlet lane's new state
+      const unsigned lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+      const int orow = __builtin_amdgcn_readlane(row, src);
+      C mine = 0;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const C vi = lane_value(fn[i], src);
+        mine = lane == (unsigned)i ? vi : mine;
+      }
+      if (lane < (unsigned)Q) p.prev_out[(size_t)lane * p.nrow + orow] = (double)mine;
+    }
   }
   if constexpr (L::DIM == 2) {
     if (x == p.nx - 1 && p.outlet_mode == OUT_PRESSURE) {  // boundary.py:139-159
@@ -865,12 +929,21 @@ struct StepWaves {
         break;                                                                                             \
       }                                                                                                    \
     }                                                                                                      \
-    const uint8_t flag = p.flags[cell];                                                                    \
+    /* Remaining waves: the cell flags are loaded only where the wave (or, for the outlet's */            \
+    /* x = nx-2 source, the wave before) holds a flagged cell, and a convective outlet row's */            \
+    /* previous state is fetched before the pull, so neither adds a dependent memory round trip. */       \
+    const OutletRow<L, ST> orow_(p, x, y, z);                                                              \
+    const unsigned wv_ = wcell_ >> 6;                                                                      \
+    const unsigned wp_ = wv_ > 0 ? wv_ - 1 : 0;                                                            \
+    const bool wflag_ = (p.wave_any[wv_ >> 5] >> (wv_ & 31)) & 1u;                                         \
+    const bool pflag_ = wflag_ || ((p.wave_any[wp_ >> 5] >> (wp_ & 31)) & 1u);                             \
+    const uint8_t flag = wflag_ ? p.flags[cell] : (uint8_t)0;                                              \
     const bool solid = flag & FLAG_SOLID;                                                                  \
     if (flag & FLAG_LINKS) link_pre_terms<L, ST>(p, B, link_record(p, cell), cell);                       \
     __builtin_amdgcn_sched_barrier(0);                                                                     \
     C_ fn[L::Q];                                                                                           \
-    const int rec = post_bc<L, ST>(p, dyn, B, cell, x, y, z, flag, true, fn);                              \
+    const int rec = post_bc<L, ST>(p, dyn, B, cell, x, y, z, flag, true, fn, pflag_, orow_.pv, orow_.coop,  \
+                                  orow_.src);                                                              \
     if constexpr (THERMAL) scalar_step<L, ST>(p, dyn, cell, x, y, z, solid, fn, true);                     \
     if (rec >= 0) link_post_sum<L, C_>(p, rec, fn);                                                        \
     /* the next step's IBM forcing acts on this post-boundary state (solver.py:280-285) */                 \
