@@ -915,9 +915,11 @@ struct StepWaves {
     /* load) and no lane at a domain edge or in the sponge -> pull, [body force,] collide, store, */      \
     /* straight-line.  Every boundary condition below is a no-op for such a cell, so this is the */       \
     /* same arithmetic; being a separate copy, the compiler may contract it into fused multiply-adds */   \
-    /* differently, so a rare population can differ in its last bit from the full sequence.  MRT */      \
-    /* kernels (register-bound) keep the single full sequence. */                                         \
-    if constexpr (!THERMAL && COLL != COLL_MRT) {                                                          \
+    /* differently, so a rare population can differ in its last bit from the full sequence.  Only */      \
+    /* D2Q9 BGK/TRT keep it (+6-8% in FP16 batches); in D3Q19 the full sequence below, which skips */     \
+    /* the flag load in unflagged waves, measured as fast (and faster on small grids), and MRT */         \
+    /* kernels are register-bound. */                                                                     \
+    if constexpr (!THERMAL && COLL != COLL_MRT && L::DIM == 2) {                                           \
       const unsigned w_ = wcell_ >> 6;                                                                     \
       const bool edge_ = x == 0 || x == p.nx - 1 || x >= p.sponge_x0 ||                                    \
                          (p.wall_mode != WALL_NONE && (y == 0 || y == p.ny - 1));                          \
