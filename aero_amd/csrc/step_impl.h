@@ -479,7 +479,8 @@ __device__ __forceinline__ float lane_value(float v, int i) {
 // The previous convective-outlet state of the one outlet row in a wave,
 // spread over lanes 0..Q-1 (lane i: population i), loaded with one coalesced
 // instruction before the pull so its latency overlaps the pull's.  Only for
-// waves holding exactly one x = nx-1 lane (always so when nx > 64) and the
+// waves holding exactly one x = nx-1 lane (always so when nx > 64) whose
+// lanes 0..Q-1 are active (the grid's last wave can be partial), and for the
 // 2-byte storage forms, whose outlet waves are latency-bound (the FP32 form
 // measured slower with the extra live register); otherwise `coop` is false
 // and the outlet lane loads its own state.
@@ -493,7 +494,8 @@ struct OutletRow {
     if constexpr (sizeof(C) == 4 && sizeof(typename Storage<ST>::T) == 2) {
       if (p.outlet_mode == OUT_CONVECTIVE) {
         const uint64_t m = __builtin_amdgcn_ballot_w64(x == p.nx - 1);
-        if (m != 0 && (m & (m - 1)) == 0) {
+        const uint64_t lowq = (1ull << L::Q) - 1;
+        if (m != 0 && (m & (m - 1)) == 0 && (__builtin_amdgcn_ballot_w64(true) & lowq) == lowq) {
           src = __builtin_ctzll(m);
           const int row = __builtin_amdgcn_readlane(z * p.ny + y, src);
           const unsigned lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));

@@ -185,31 +185,42 @@ def test_shifted_half_storage_accuracy(storage):
     assert cd_err < HALF_CD_TOL[storage]
 
 
+@pytest.mark.parametrize("shape", [(4, 6, 96), (3, 3, 66), (5, 65)])
 @pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
-def test_convective_outlet_with_one_outlet_cell_per_wave(storage):
-    """Rows wider than a 64-lane wavefront (nx = 96) put exactly one outlet
-    cell in each wave holding one, at varying lane positions; there the 2-byte
+def test_convective_outlet_with_one_outlet_cell_per_wave(storage, shape):
+    """Rows wider than a 64-lane wavefront put exactly one outlet cell in
+    each wave holding one, at varying lane positions; there the 2-byte
     storage forms fetch the row's previous outlet state with one load spread
-    over the wave's lanes.  The state and the outlet state vary smoothly by
-    1-2% from row to row (tau = 1: the run is insensitive to rounding, 2e-8
-    after 20 steps from a float32-rounded start), while the outlet state of
-    the neighbouring row in place of the right one moves the result by 6e-3
-    of max |f|: far above the stated tolerances.  Float64 oracle from the same
-    state."""
+    over the wave's lanes 0..Q-1.  The 3 x 3 x 66 and 5 x 65 grids end in a
+    partial wave with fewer than Q valid lanes (594 = 9 * 64 + 18 cells,
+    325 = 5 * 64 + 5), where that fetch must not be used.  The state and the
+    outlet state vary smoothly by 1-2% from row to row (tau = 1: the run is
+    insensitive to rounding, 2e-8 after 20 steps from a float32-rounded start
+    on the 4 x 6 x 96 grid), while the outlet state of the neighbouring row in
+    place of the right one moves that result by 6e-3 of max |f|: far above
+    the stated tolerances.  Float64 oracle from the same state."""
     import aero_amd
     from oracle import oracle as orc
-    nz, ny, nx, u0 = 4, 6, 96, 0.05
-    sh = (nz, ny, nx)
-    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
-    rho = 1.0 + 0.01 * np.sin(2 * np.pi * (y / ny + 2 * z / nz))
-    f0 = orc.feq(orc.E19, orc.W19, rho, u0 + 0.02 * np.cos(2 * np.pi * (y / ny + z / nz)),
-                 0.01 * np.sin(2 * np.pi * y / ny), 0.01 * np.cos(2 * np.pi * z / nz))
-    prev = np.roll(np.roll(f0[..., -1], 1, axis=2), 1, axis=1)  # valid, and different from each row's own
-    solid = np.zeros(sh, dtype=bool)
-    omega = 1.0
-    s = aero_amd.Solver3D(Nz=nz, Ny=ny, Nx=nx, solid=solid, omega=omega, u0=u0, D=8.0, collision="trt",
-                          trt_lambda=0.1875, outlet_bc="convective", storage=storage)
-    o = orc.OracleSolver(solid, omega=omega, u0=u0, D=8.0, collision="trt", trt_lambda=0.1875)
+    u0 = 0.05
+    solid = np.zeros(shape, dtype=bool)
+    if len(shape) == 3:
+        nz, ny, nx = shape
+        z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        rho = 1.0 + 0.01 * np.sin(2 * np.pi * (y / ny + 2 * z / nz))
+        f0 = orc.feq(orc.E19, orc.W19, rho, u0 + 0.02 * np.cos(2 * np.pi * (y / ny + z / nz)),
+                     0.01 * np.sin(2 * np.pi * y / ny), 0.01 * np.cos(2 * np.pi * z / nz))
+        prev = np.roll(np.roll(f0[..., -1], 1, axis=2), 1, axis=1)  # valid, and not each row's own
+        s = aero_amd.Solver3D(Nz=nz, Ny=ny, Nx=nx, solid=solid, omega=1.0, u0=u0, D=8.0, collision="trt",
+                              trt_lambda=0.1875, outlet_bc="convective", storage=storage)
+    else:
+        ny, nx = shape
+        y, x = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+        rho = 1.0 + 0.01 * np.sin(2 * np.pi * y / ny)
+        f0 = orc.feq(orc.E9, orc.W9, rho, u0 + 0.02 * np.cos(2 * np.pi * y / ny), 0.01 * np.sin(4 * np.pi * y / ny))
+        prev = np.roll(f0[..., -1], 1, axis=1)
+        s = aero_amd.Solver(Ny=ny, Nx=nx, solid=solid, omega=1.0, u0=u0, D=8.0, collision="trt",
+                            trt_lambda=0.1875, outlet_bc="convective", storage=storage)
+    o = orc.OracleSolver(solid, omega=1.0, u0=u0, D=8.0, collision="trt", trt_lambda=0.1875)
     s.f = f0
     s._f_outlet_prev = prev
     o.set_state(f0, prev=prev)
@@ -218,7 +229,7 @@ def test_convective_outlet_with_one_outlet_cell_per_wave(storage):
         o.step()
     tol = {"f32": F32_FIELD_TOL, "f16": HALF_FIELD_TOL["f16"], "bf16": HALF_FIELD_TOL["bf16"]}[storage]
     err_f, err_p = G.rel_err(s.f, o.f_ref()), G.rel_err(s._f_outlet_prev, o.prev_ref())
-    print(f"[gpu] outlet rows {storage}: field {err_f:.2e} outlet state {err_p:.2e}", flush=True)
+    print(f"[gpu] outlet rows {shape} {storage}: field {err_f:.2e} outlet state {err_p:.2e}", flush=True)
     assert err_f < tol and err_p < tol
     s.close()
 
