@@ -648,6 +648,51 @@ __device__ __forceinline__ void scalar_step(const StepParams& p, const StepDyn& 
   }
 }
 
+// Equilibrium moments of the MRT basis in closed form.  With
+// feq_b = w_b rho (1 + 3 e_b.u + 4.5 (e_b.u)^2 - 1.5 u^2), row a of M feq is
+// rho (A_a + 3 B_a.u + sum_d Cq_a,d q_d / 72) / 36, where 36 w_b is an integer
+// for both lattices, A_a = sum_b M_ab 36 w_b, B_a = sum_b M_ab 36 w_b e_b and
+// Cq holds the coefficients of q = (ux^2, uy^2, uz^2, ux uy, ux uz, uy uz)
+// times 72: all exact integers, so a term whose coefficient vanishes is
+// dropped at compile time.  D_a = sum_b M_ab^2 (the row norms).
+template <class L>
+struct MrtEq {
+  struct Tab {
+    long A[L::Q];
+    long B[L::Q][3];
+    long Cq[L::Q][6];
+    long D[L::Q];
+    bool even[L::Q];  // row a is even under i <-> opp(i) (else odd)
+  };
+  static constexpr Tab make() {
+    Tab t{};
+    for (int a = 0; a < L::Q; ++a) {
+      for (int b = 0; b < L::Q; ++b) {
+        const long w = (long)(L::W[b] * 36.0 + 0.5);
+        const long m = (long)L::M[a][b] * w;
+        const long e[3] = {L::EX[b], L::EY[b], L::EZ[b]};
+        t.A[a] += m;
+        for (int d = 0; d < 3; ++d) t.B[a][d] += m * e[d];
+        // 72 * (4.5 (e.u)^2): 324 e_d e_f over ordered pairs; diagonal 324 e_d^2,
+        // each cross pair 2 * 324 e_d e_f
+        t.Cq[a][0] += 324 * m * e[0] * e[0];
+        t.Cq[a][1] += 324 * m * e[1] * e[1];
+        t.Cq[a][2] += 324 * m * e[2] * e[2];
+        t.Cq[a][3] += 648 * m * e[0] * e[1];
+        t.Cq[a][4] += 648 * m * e[0] * e[2];
+        t.Cq[a][5] += 648 * m * e[1] * e[2];
+        t.D[a] += (long)L::M[a][b] * L::M[a][b];
+      }
+      // 72 * (-1.5 u^2) per unit of 36 w: -108 A on each square
+      for (int d = 0; d < 3; ++d) t.Cq[a][d] -= 108 * t.A[a];
+      t.even[a] = true;
+      for (int b = 0; b < L::Q; ++b) t.even[a] = t.even[a] && L::M[a][b] == L::M[a][L::OPP[b]];
+    }
+    return t;
+  }
+  static constexpr Tab T = make();
+};
+
 // Collision (BGK kernels.py:42-92, TRT kernels_trt.py:228-277,
 // MRT kernels_mrt.py:30-82 / kernels3d_mrt.py:158-217).  u = 0 in solids.
 template <class L, int COLL, class C>
@@ -717,43 +762,133 @@ __device__ __forceinline__ void collide(const double* mrt_s, const double* minv_
       uy = C(0);
       uz = C(0);
     }
-    const C usq = ux * ux + uy * uy + uz * uz;
-    const C base = C(1) - C(1.5) * usq;
-    C fe[Q];
-    fe[0] = C(L::W[0]) * rho * base;
+    // Moments m = M f; equilibrium moments M feq in closed form (MrtEq:
+    // rho * (A + B.u + u^T C u), the same sums the reference's M @ feq forms);
+    // relaxed moments mp = m - s (m - m_eq).
+    using E_ = MrtEq<L>;
+    const C uq[6] = {ux * ux, uy * uy, uz * uz, ux * uy, ux * uz, uy * uz};
+    const C uv[3] = {ux, uy, uz};
+    // Every row of M is even or odd under i <-> opp(i), so each row is a sum
+    // over direction pairs of f_i + f_opp(i) (even) or f_i - f_opp(i) (odd):
+    // half the terms.
+    C ps[Q], pd[Q];
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
-      const int j = L::OPP[i];
-      if (i > j) continue;
-      const C eu = edot<L>(i, ux, uy, uz);
-      const C wr = C(L::W[i]) * rho;
-      const C ev = wr * (base + C(4.5) * eu * eu);
-      const C od = wr * (C(3) * eu);
-      fe[i] = ev + od;
-      fe[j] = ev - od;
+      if (i < L::OPP[i]) {
+        ps[i] = fn[i] + fn[L::OPP[i]];
+        pd[i] = fn[i] - fn[L::OPP[i]];
+      }
     }
     C mp[Q];
 #pragma unroll
     for (int a = 0; a < Q; ++a) {
-      C m = 0, me = 0;
+      C m = 0;
+      bool mfirst_ = true;
+      if (E_::T.even[a] && L::M[a][0] != 0) {
+        m = L::M[a][0] == 1 ? fn[0] : C(L::M[a][0]) * fn[0];
+        mfirst_ = false;
+      }
 #pragma unroll
-      for (int b = 0; b < Q; ++b) {
-        if (L::M[a][b] != 0) {
-          m += C(L::M[a][b]) * fn[b];
-          me += C(L::M[a][b]) * fe[b];
+      for (int b = 1; b < Q; ++b) {
+        if (b < L::OPP[b] && L::M[a][b] != 0) {
+          const C v = E_::T.even[a] ? ps[b] : pd[b];
+          const C term = L::M[a][b] == 1 ? v : (L::M[a][b] == -1 ? -v : C(L::M[a][b]) * v);
+          m = mfirst_ ? term : m + term;
+          mfirst_ = false;
         }
       }
+      C t = 0;
+      bool tfirst_ = true;
+      if (E_::T.A[a] != 0) {
+        t = C(E_::T.A[a] / 36.0);
+        tfirst_ = false;
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (E_::T.B[a][d] != 0) {
+          const C term = C(3.0 * E_::T.B[a][d] / 36.0) * uv[d];
+          t = tfirst_ ? term : t + term;
+          tfirst_ = false;
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 6; ++d) {
+        if (E_::T.Cq[a][d] != 0) {
+          const C term = C(E_::T.Cq[a][d] / 2592.0) * uq[d];
+          t = tfirst_ ? term : t + term;
+          tfirst_ = false;
+        }
+      }
+      const C me = rho * t;
       mp[a] = m - C(mrt_s[a]) * (m - me);
     }
-    const C* minv;
-    if constexpr (sizeof(C) == 8) minv = (const C*)minv_d;
-    else minv = (const C*)minv_f;
+    if constexpr (L::Q == 9) {
+      // D2Q9's rows are orthogonal: M^-1 = M^T D^-1 with D = diag(M M^T),
+      // exact in closed form (the reference inverts M9 numerically; the two
+      // agree to 2e-16)
+      C r[Q];
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      C acc = 0;
+      for (int a = 0; a < Q; ++a) r[a] = mp[a] * C(1.0 / E_::T.D[a]);
+      // out_i = E_i + O_i and out_opp(i) = E_i - O_i over the even / odd rows
 #pragma unroll
-      for (int a = 0; a < Q; ++a) acc += minv[i * Q + a] * mp[a];
-      out[i] = acc;
+      for (int i = 0; i < Q; ++i) {
+        if (i != 0 && i > L::OPP[i]) continue;
+        C ev = 0, od = 0;
+        bool efirst_ = true, ofirst_ = true;
+#pragma unroll
+        for (int a = 0; a < Q; ++a) {
+          if (L::M[a][i] != 0) {
+            const C term = L::M[a][i] == 1 ? r[a] : (L::M[a][i] == -1 ? -r[a] : C(L::M[a][i]) * r[a]);
+            if (E_::T.even[a]) {
+              ev = efirst_ ? term : ev + term;
+              efirst_ = false;
+            } else {
+              od = ofirst_ ? term : od + term;
+              ofirst_ = false;
+            }
+          }
+        }
+        if (i == 0) {
+          out[0] = ev;
+        } else {
+          out[i] = ev + od;
+          out[L::OPP[i]] = ev - od;
+        }
+      }
+    } else {
+      // The host's pseudo-inverse P, skipping its entries that are zero up
+      // to rounding (D3Q19::PINV_NZ).  P keeps the rows' symmetry
+      // (P[opp i][a] = +-P[i][a] for even / odd rows, to 2e-15 in the
+      // float64 pseudo-inverse), so row i gives both out_i = E_i + O_i and
+      // out_opp(i) = E_i - O_i.
+      const C* minv;
+      if constexpr (sizeof(C) == 8) minv = (const C*)minv_d;
+      else minv = (const C*)minv_f;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        if (i != 0 && i > L::OPP[i]) continue;
+        C ev = 0, od = 0;
+        bool efirst_ = true, ofirst_ = true;
+#pragma unroll
+        for (int a = 0; a < Q; ++a) {
+          if (L::PINV_NZ[i][a]) {
+            const C term = minv[i * Q + a] * mp[a];
+            if (E_::T.even[a]) {
+              ev = efirst_ ? term : ev + term;
+              efirst_ = false;
+            } else {
+              od = ofirst_ ? term : od + term;
+              ofirst_ = false;
+            }
+          }
+        }
+        if (i == 0) {
+          out[0] = ev;
+        } else {
+          out[i] = ev + od;
+          out[L::OPP[i]] = ev - od;
+        }
+      }
     }
   }
 }

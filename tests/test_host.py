@@ -185,3 +185,38 @@ def test_solver_argument_errors_match_reference():
         Solver(8, 16, solid, 1.0, 0.05, 4.0, les_model="dynamic")
     with pytest.raises(ValueError, match="streamwise_bc"):
         Solver3D(4, 8, 16, np.zeros((4, 8, 16), bool), 1.0, 0.05, 4.0, streamwise_bc="inflow")
+
+
+def _header_table(lattice_struct, name):
+    """A constexpr integer / bool table of a lattice struct in lbm_device.h."""
+    import pathlib
+    import re
+    src = (pathlib.Path(__file__).resolve().parents[1] / "aero_amd" / "csrc" / "lbm_device.h").read_text()
+    seg = src[src.index(f"struct {lattice_struct} {{"):]
+    j = seg.index(f" {name}[")
+    body = seg[seg.index("=", j):seg.index("};", j)]
+    return np.array([int(v) for v in re.findall(r"-?\d+", body)])
+
+
+def test_mrt_tables_the_kernels_rely_on():
+    """The MRT collision uses (a) D3Q19::PINV_NZ, the entries of the
+    pseudo-inverse of the reference's rank-16 M19 that are nonzero beyond
+    rounding, (b) that every row of M9 / M19 is even or odd under
+    i <-> opp(i), with the pseudo-inverse keeping that symmetry, and (c) that
+    M9's rows are orthogonal, so inv(M9) = M9^T diag(M9 M9^T)^-1."""
+    from aero_amd import lattice
+    P = lattice.M19_inv
+    nz = _header_table("D3Q19", "PINV_NZ").reshape(19, 19).astype(bool)
+    np.testing.assert_array_equal(nz, np.abs(P) > 1e-12)
+    assert np.abs(P[~nz]).max() < 1e-14 and np.abs(P[nz]).min() > 1e-3
+    np.testing.assert_array_equal(_header_table("D3Q19", "M").reshape(19, 19), lattice.M19)
+    np.testing.assert_array_equal(_header_table("D2Q9", "M").reshape(9, 9), lattice.M9)
+    for M, opp, Pi in ((lattice.M9, lattice.OPP, lattice.M9_inv), (lattice.M19, lattice.OPP3, P)):
+        even = np.array([np.array_equal(r, r[opp]) for r in M])
+        odd = np.array([np.array_equal(r, -r[opp]) for r in M])
+        assert (even ^ odd).all()
+        sign = np.where(even, 1.0, -1.0)
+        assert np.abs(Pi[opp] - Pi * sign[None, :]).max() < 1e-13
+    G9 = lattice.M9 @ lattice.M9.T
+    assert np.abs(G9 - np.diag(np.diag(G9))).max() == 0
+    np.testing.assert_allclose(lattice.M9.T / np.diag(G9), lattice.M9_inv, atol=1e-15)
