@@ -1014,11 +1014,14 @@ __device__ __forceinline__ void reduce_forces_block(const ForceReduce& r) {
 // Occupancy target of the step kernel: the 4- and 2-byte BGK/TRT forms ask
 // the register allocator for 7 waves per SIMD (at most 72 VGPRs; the common
 // pull + collide path needs ~50, the rare link / outlet / reduction paths
-// would otherwise raise the whole kernel to ~85).  FP64 and MRT keep the
-// compiler's choice.
+// would otherwise raise the whole kernel to ~85).  MRT in the 2-byte forms
+// targets 6 (the compiler's own choice, 5, measured 10% slower in D3Q19);
+// FP64 and FP32 MRT keep the compiler's choice.
 template <class L, int COLL, int ST>
 struct StepWaves {
-  static constexpr int value = (ST != ST_F64 && COLL != COLL_MRT) ? 7 : 1;
+  static constexpr int value = (ST != ST_F64 && COLL != COLL_MRT)                   ? 7
+                              : (COLL == COLL_MRT && (ST == ST_F16S || ST == ST_BF16S)) ? 6
+                                                                                        : 1;
 };
 
 // The step of one cell (or, in block 0, the previous step's force
