@@ -143,6 +143,11 @@ CASES_3D = {
         geom="sphere"),
     "bgk_ibm": dict(kw=dict(collision="bgk", ibm_enabled=True), geom="sphere"),
     "trt_ibm_les": dict(kw=dict(collision="trt", ibm_enabled=True, les=True, les_cs=0.2), geom="box", omega=1.3),
+    # the 512 x 256 x 256 MRT + WALE LES + scalar + Boussinesq + SEM inlet configuration, small
+    "mrt_les_wale_thermal_sem": dict(
+        kw=dict(collision="mrt", les=True, les_model="wale", les_cs=0.3, bouzidi=True, thermal=True,
+                alpha_T=0.01, buoyancy=True, g_gravity=0.01, beta=0.5, T_ref=0.5, sem_inlet=True,
+                sem_Tu=0.05, sem_L_int=2.5, sem_N=20), geom="box", omega=1.3),
 }
 
 NY2, NX2 = 20, 40
