@@ -56,6 +56,8 @@ _SIGS = {
     "aero_set_geometry": ([_P, _P, ctypes.c_int64, _P, _P], ctypes.c_int),
     "aero_set_damping": ([_P, _P], ctypes.c_int),
     "aero_set_ibm": ([_P, _P], ctypes.c_int),
+    "aero_set_sem": ([_P, ctypes.c_int32, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+    "aero_sem_planes": ([_P, ctypes.c_int32, _P, _P], ctypes.c_int),
     "aero_set_f": ([_P, _P], ctypes.c_int),
     "aero_get_f": ([_P, _P], ctypes.c_int),
     "aero_set_f_equilibrium": ([_P, _P, _P, _P, _P], ctypes.c_int),

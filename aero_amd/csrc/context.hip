@@ -194,6 +194,7 @@ struct aero_ctx {
   // double-buffered device coefficient rows and pinned staging, so the
   // device never waits on the host between chunks and memory stays bounded
   DevBuf planes[2], recyc, hist_dev[2], last_plane_buf;
+  SemGen* sem = nullptr;  // device SEM inlet (aero_set_sem): `planes` arguments carry eddy snapshots
   double* hist_stage[2] = {nullptr, nullptr};
   hipEvent_t chunk_ev[2] = {nullptr, nullptr};
   const double* last_plane = nullptr;
@@ -219,6 +220,19 @@ struct aero_ctx {
 
   bool les_split() const { return d.les != AERO_LES_OFF && d.collision != AERO_MRT; }
   bool need_planes() const { return d.inlet == AERO_IN_FIELD || d.inlet == AERO_IN_SEM; }
+  // Inlet planes of steps s0 .. s0+n-1 of a call into planes[b]: uploaded
+  // from the host planes, or, with the device SEM, generated from the host's
+  // per-step eddy snapshots.
+  void fill_planes(int b, const double* host, int s0, int n, hipStream_t s) {
+    if (sem) {
+      sem_planes(sem, host + (size_t)6 * sem_n * s0, n, planes[b].as<double>(), s);
+      return;
+    }
+    const size_t plane_cnt = 3 * (size_t)nrow;
+    ck(hipMemcpyAsync(planes[b].p, host + plane_cnt * s0, sizeof(double) * plane_cnt * n, hipMemcpyHostToDevice, s),
+       "upload planes");
+  }
+  int sem_n = 0;
   // inlet planes are 3 x nrow doubles per step: smaller chunks bound their buffers
   int chunk_steps() const { return need_planes() ? STEP_CHUNK / 4 : STEP_CHUNK; }
   bool convective() const { return d.outlet == AERO_OUT_CONVECTIVE; }
@@ -536,9 +550,7 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
       const double* pdev = nullptr;
       if (c->need_planes()) {
         const size_t plane_cnt = 3 * (size_t)c->nrow;
-        ck(hipMemcpyAsync(c->planes[b].p, planes[i] + plane_cnt * s0, sizeof(double) * plane_cnt * nk,
-                          hipMemcpyHostToDevice, stream),
-           "upload planes");
+        c->fill_planes(b, planes[i], s0, nk, stream);
         pdev = c->planes[b].as<double>();
         last_planes[i] = pdev + plane_cnt * (nk - 1);
       }
@@ -719,6 +731,8 @@ int aero_destroy(aero_ctx* c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  sem_destroy(c->sem);
+  c->sem = nullptr;
   DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags, &c->wave_any,
                     &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
                     &c->damp, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
@@ -964,6 +978,38 @@ struct ScopedStream {
   ~ScopedStream() { (void)hipStreamDestroy(s); }
 };
 
+int aero_set_sem(aero_ctx* c, int32_t n_eddies, double sigma, double Tu) {
+  return guarded([&] {
+    if (c->d.q != 19 || c->d.inlet != AERO_IN_SEM) throw std::invalid_argument("aero_set_sem: not a 3D SEM inlet");
+    if (n_eddies < 0) throw std::invalid_argument("aero_set_sem: negative eddy count");
+    c->set_device();
+    ck(hipStreamSynchronize(c->stream), "SEM sync");
+    sem_destroy(c->sem);
+    c->sem = nullptr;
+    c->sem = sem_create(c->d.ny, c->d.nz, n_eddies, sigma, Tu, c->d.u0, c->chunk_steps());
+    c->sem_n = n_eddies;
+  });
+}
+
+int aero_sem_planes(aero_ctx* c, int32_t n, const double* eddies, double* planes_out) {
+  return guarded([&] {
+    if (!c->sem) throw std::invalid_argument("aero_sem_planes: call aero_set_sem first");
+    c->set_device();
+    const size_t plane_cnt = 3 * (size_t)c->nrow;
+    const int CH = c->chunk_steps();
+    ScopedBuf out;
+    out.alloc(sizeof(double) * plane_cnt * (size_t)std::min<int>(std::max<int>(n, 1), CH));
+    for (int s0 = 0; s0 < n; s0 += CH) {
+      const int k = std::min(CH, n - s0);
+      sem_planes(c->sem, eddies + (size_t)6 * c->sem_n * s0, k, out.as<double>(), c->stream);
+      ck(hipMemcpyAsync(planes_out + plane_cnt * s0, out.p, sizeof(double) * plane_cnt * k, hipMemcpyDeviceToHost,
+                        c->stream),
+         "download SEM planes");
+      ck(hipStreamSynchronize(c->stream), "SEM planes sync");
+    }
+  });
+}
+
 int aero_mesh_inside(const double* tris, int64_t ntri, const double* ray, const double* points, int64_t npts,
                      int32_t nx, int32_t ny, int32_t nz, uint8_t* inside) {
   return guarded([&] {
@@ -1140,10 +1186,7 @@ int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, 
       const int b = ci & 1;
       drain(b);
       double* rows = c->hist_dev[b].as<double>();
-      if (need_planes)
-        ck(hipMemcpyAsync(c->planes[b].p, planes + plane_cnt * s0, sizeof(double) * plane_cnt * n,
-                          hipMemcpyHostToDevice, c->stream),
-           "upload planes");
+      if (need_planes) c->fill_planes(b, planes, s0, n, c->stream);
       for (int k = 0; k < n; ++k) {
         const long long step_no = step0 + s0 + k + 1;
         const int rd = c->side, wr = c->side ^ 1;

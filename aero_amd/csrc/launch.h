@@ -29,6 +29,15 @@ AERO_DECL_SPLIT(d3q19_trt)
 #undef AERO_DECL_BATCH
 #undef AERO_DECL_SPLIT
 
+// Synthetic-eddy-method inlet planes on the device (sem.hip): a generator for
+// one ny x nz inlet holding up to max_planes planes per call.
+struct SemGen;
+SemGen* sem_create(int ny, int nz, int n_eddies, double sigma, double Tu, double u0, int max_planes);
+void sem_destroy(SemGen* g);
+// nk planes (nk x 3 x nz*ny doubles, device) from nk host eddy snapshots
+// (nk x 6 x n_eddies doubles: x, y, z, then the three signs)
+void sem_planes(SemGen* g, const double* eddies_host, int nk, double* planes_dev, hipStream_t s);
+
 // Step k of a chunk for a group of cases sharing lattice, collision, storage
 // and scalar transport: one launch over `ntiles` packed blocks.  Events, when
 // given, are recorded at the kernel's start and end.

@@ -147,6 +147,13 @@ class Solver3D(DeviceSolver):
             # Guo IBM: links stop bouncing back; forcing only where phi is given
             # (the reference's `ibm_enabled and phi is not None`)
             self._set_ibm(self.phi)
+        self._sem_device = False
+        if self._sem is not None and d.inlet == _abi.IN_SEM:
+            # the SEM planes are formed on the device from the eddy states the
+            # host advances (SEMInlet.step keeps the reference's random draws)
+            self._ctx.call("aero_set_sem", ctypes.c_int32(self._sem.N), ctypes.c_double(self._sem.sigma),
+                           ctypes.c_double(self._sem.Tu))
+            self._sem_device = True
 
         sh = (Nz, Ny, Nx)
         # f = feq(rho0, u0, 0) (solver3d.py:239-247), evaluated on the device;
@@ -173,6 +180,14 @@ class Solver3D(DeviceSolver):
     def _planes(self, n):
         if self.streamwise_bc != "open" or self.inlet_bc != "velocity":
             return None
+        if self._sem is not None and self._sem_device:
+            # per step: the eddy state after that step's advance (aero_set_sem)
+            snaps = np.empty((n, 6, self._sem.N))
+            for k in range(n):
+                self._sem.step()
+                snaps[k, 0], snaps[k, 1], snaps[k, 2] = self._sem.xk, self._sem.yk, self._sem.zk
+                snaps[k, 3:] = self._sem.eps
+            return snaps
         if self._sem is not None:
             planes = np.empty((n, 3, self.Nz, self.Ny))
             for k in range(n):

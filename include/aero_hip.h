@@ -147,6 +147,18 @@ int aero_set_damping(aero_ctx* ctx, const double* damp);
  * after aero_set_geometry and before the state is set. */
 int aero_set_ibm(aero_ctx* ctx, const double* phi);
 
+/* Synthetic-eddy-method inlet planes formed on the device: replaces the
+ * per-step SEMInlet.fluctuation() (aero/lbm/sem_inlet.py:71-136) of an
+ * n_eddies-eddy SEM with eddy size sigma (L_int) and intensity Tu.  The host
+ * keeps SEMInlet.step() (sem_inlet.py:58-69), whose random regeneration is
+ * the reference's.  Once enabled, the `planes` argument of aero_step and
+ * aero_batch_step carries, per step, the eddy state after that step's
+ * advance -- 6 x n_eddies doubles: x, y, z, then the u, v, w signs --
+ * instead of 3 x nz x ny plane values.  aero_sem_planes returns the planes
+ * that n such snapshots give (n x 3 x nz x ny doubles), for checking. */
+int aero_set_sem(aero_ctx* ctx, int32_t n_eddies, double sigma, double Tu);
+int aero_sem_planes(aero_ctx* ctx, int32_t n, const double* eddies, double* planes);
+
 /* State: the reference's self.f (post-boundary, pre-collision), self.g and
  * self._f_outlet_prev, read and written in the reference layout. */
 int aero_set_f(aero_ctx* ctx, const double* f);
