@@ -631,19 +631,30 @@ __device__ __forceinline__ void scalar_step(const StepParams& p, const StepDyn& 
   const C inv = rho > C(0) ? recip(rho) : C(0);
   const C ux = inv * mx, uy = inv * my, uz = inv * mz;
   if (p.buoyancy) {
+    // directions with e_y = 0 receive exactly zero: skipped
     const C buoy = (C)p.gbeta * (T - (C)p.T_ref);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) fn[i] += C(L::W[i]) * C(3) * rho * buoy * C(L::EY[i]);
+    for (int i = 0; i < Q; ++i)
+      if (L::EY[i] != 0) fn[i] += C(L::W[i]) * C(3) * rho * buoy * C(L::EY[i]);
   }
   if (write_g) {
     const C om = (C)p.omega_T;
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      const C eu = edot<L>(i, ux, uy, uz);
-      const C geq = C(L::W[i]) * T * (C(1) + C(3) * eu);
-      const C gp = solid ? C(L::W[i]) * T : g[i] - om * (g[i] - geq);
+    auto put = [&](int i, C gp) {
       S::st((typename S::T*)scalar_base((AERO_GLOBAL typename S::T*)p.gout + (size_t)i * N), (unsigned)cell, gp,
             C(L::W[i]) * (C)p.gshift);
+    };
+    // geq_i = w T (1 + 3 e_i.u): a pair (i, opp i) shares w T and +-3 w T e_i.u
+    put(0, solid ? C(L::W[0]) * T : g[0] - om * (g[0] - C(L::W[0]) * T));
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const int j = L::OPP[i];
+      if (i > j) continue;
+      const C wt = C(L::W[i]) * T;
+      const C eu = edot<L>(i, ux, uy, uz);
+      const C geq_i = wt * (C(1) + C(3) * eu);
+      const C geq_j = wt * (C(1) - C(3) * eu);
+      put(i, solid ? wt : g[i] - om * (g[i] - geq_i));
+      put(j, solid ? wt : g[j] - om * (g[j] - geq_j));
     }
   }
 }
@@ -1015,13 +1026,14 @@ __device__ __forceinline__ void reduce_forces_block(const ForceReduce& r) {
 // the register allocator for 7 waves per SIMD (at most 72 VGPRs; the common
 // pull + collide path needs ~50, the rare link / outlet / reduction paths
 // would otherwise raise the whole kernel to ~85).  MRT in the 2-byte forms
-// targets 6 (the compiler's own choice, 5, measured 10% slower in D3Q19);
-// FP64 and FP32 MRT keep the compiler's choice.
-template <class L, int COLL, int ST>
+// targets 6 (the compiler's own choice, 5, measured 10% slower in D3Q19)
+// unless it carries the scalar too (6 waves spill there: 16% slower); FP64,
+// FP32 MRT and MRT with the scalar keep the compiler's choice.
+template <class L, int COLL, int ST, bool THERMAL>
 struct StepWaves {
-  static constexpr int value = (ST != ST_F64 && COLL != COLL_MRT)                   ? 7
-                              : (COLL == COLL_MRT && (ST == ST_F16S || ST == ST_BF16S)) ? 6
-                                                                                        : 1;
+  static constexpr int value = (ST != ST_F64 && COLL != COLL_MRT) ? 7
+                              : (COLL == COLL_MRT && (ST == ST_F16S || ST == ST_BF16S) && !THERMAL) ? 6
+                                                                                                    : 1;
 };
 
 // The step of one cell (or, in block 0, the previous step's force
@@ -1095,7 +1107,7 @@ struct StepWaves {
 
 // One case, parameters by value (split LES step, recycling inlet).
 template <class L, int COLL, int ST, bool THERMAL, int KMODE>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(StepWaves<L, COLL, ST>::value)))
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(StepWaves<L, COLL, ST, THERMAL>::value)))
 void step_kernel(StepParams p) {
   const StepDyn dyn = dyn_of(p);
   const int blk = blockIdx.x;
@@ -1109,7 +1121,7 @@ void step_kernel(StepParams p) {
 // A single case runs through this same kernel as a batch of one, so a case
 // computes the same bits alone or among others.
 template <class L, int COLL, int ST, bool THERMAL>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(StepWaves<L, COLL, ST>::value)))
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(StepWaves<L, COLL, ST, THERMAL>::value)))
 void batch_step_kernel(const BatchCase* __restrict__ cases, const int2* __restrict__ tiles, int k) {
   const int2 t = tiles[blockIdx.x];
   const BatchCase& bc = cases[t.x];
