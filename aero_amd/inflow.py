@@ -82,6 +82,23 @@ class SEMInlet:
         self.zk = self._rng.uniform(0.0, float(self.Nz), self.N)
         self.eps = self._rng.choice(np.array([-1.0, 1.0]), size=(3, self.N)).astype(np.float64)
 
+    def state(self) -> dict:
+        """Everything the next planes depend on: the eddies and the random
+        generator's bit state (not in the reference's checkpoints; kept so a
+        resumed run continues the same turbulence)."""
+        return dict(Ny=self.Ny, Nz=self.Nz, u0=self.u0, Tu=self.Tu, L_int=self.sigma, N=self.N,
+                    xk=self.xk.tolist(), yk=self.yk.tolist(), zk=self.zk.tolist(), eps=self.eps.tolist(),
+                    rng=self._rng.bit_generator.state)
+
+    def set_state(self, st: dict) -> None:
+        if (int(st["Ny"]), int(st["Nz"]), int(st["N"])) != (self.Ny, self.Nz, self.N):
+            raise ValueError("SEM state of a different inlet")
+        self.xk = np.array(st["xk"], dtype=np.float64)
+        self.yk = np.array(st["yk"], dtype=np.float64)
+        self.zk = np.array(st["zk"], dtype=np.float64)
+        self.eps = np.array(st["eps"], dtype=np.float64).reshape(3, self.N)
+        self._rng.bit_generator.state = st["rng"]
+
     def step(self, dt: float = 1.0) -> None:
         self.xk += self.u0 * dt
         wrap = self.xk > self._r

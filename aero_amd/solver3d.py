@@ -341,6 +341,10 @@ class Solver3D(DeviceSolver):
             kw["phi"] = self.phi
         if self.thermal:
             kw["g"] = self.g
+        if self._sem is not None:
+            # not a reference key (its loader ignores it): the SEM eddies and
+            # random state, so a resumed run continues the same inlet turbulence
+            kw["sem_state_json"] = np.array(json.dumps(self._sem.state()))
         np.savez_compressed(path, **kw)
 
     def load_checkpoint(self, path: str) -> None:
@@ -365,6 +369,8 @@ class Solver3D(DeviceSolver):
         self.step_count = int(data["step_count"])
         if "g" in data and self.thermal:
             self.g = data["g"]
+        if "sem_state_json" in data and self._sem is not None:
+            self._sem.set_state(json.loads(str(data["sem_state_json"])))
 
     @classmethod
     def from_checkpoint(cls, path: str, backend: Optional[str] = None) -> "Solver3D":
@@ -373,6 +379,9 @@ class Solver3D(DeviceSolver):
         params = json.loads(str(data["params_json"]))
         if backend is not None:
             params["backend"] = backend
+        if "sem_state_json" in data:
+            st = json.loads(str(data["sem_state_json"]))
+            params.update(sem_inlet=True, sem_Tu=st["Tu"], sem_L_int=st["L_int"], sem_N=st["N"])
         solver = cls(solid=data["solid"], phi=data["phi"] if "phi" in data else None, **params)
         solver.load_checkpoint(path)
         return solver
