@@ -23,6 +23,19 @@ class SyntheticInflow2D:
         self.rng = np.random.default_rng(int(seed))
         self.field = np.zeros(ny, dtype=np.float64)
 
+    def state(self) -> dict:
+        """The filtered field and the random generator's bit state (not in the
+        reference's checkpoints; kept so a resumed run continues the same
+        inflow)."""
+        return dict(field=self.field.tolist(), rng=self.rng.bit_generator.state)
+
+    def set_state(self, st: dict) -> None:
+        field = np.array(st["field"], dtype=np.float64)
+        if field.shape != self.field.shape:
+            raise ValueError("inflow state of a different inlet")
+        self.field = field
+        self.rng.bit_generator.state = st["rng"]
+
     def next(self) -> np.ndarray:
         noise = self.rng.standard_normal(self.ny)
         filtered = (noise + np.roll(noise, 1) + np.roll(noise, -1)) / 3.0
@@ -44,6 +57,18 @@ class SyntheticInflow3D:
         self.rng = np.random.default_rng(int(seed))
         self.fy = np.zeros(self.shape)
         self.fz = np.zeros(self.shape)
+
+    def state(self) -> dict:
+        """The filtered planes and the random generator's bit state (see
+        SyntheticInflow2D.state)."""
+        return dict(fy=self.fy.tolist(), fz=self.fz.tolist(), rng=self.rng.bit_generator.state)
+
+    def set_state(self, st: dict) -> None:
+        fy, fz = np.array(st["fy"], dtype=np.float64), np.array(st["fz"], dtype=np.float64)
+        if fy.shape != self.shape or fz.shape != self.shape:
+            raise ValueError("inflow state of a different inlet")
+        self.fy, self.fz = fy, fz
+        self.rng.bit_generator.state = st["rng"]
 
     @staticmethod
     def _filt(a):

@@ -305,6 +305,10 @@ class Solver(DeviceSolver):
             kw["phi"] = self.phi
         if self.thermal:
             kw["g"] = self.g
+        if self.synthetic_inflow:
+            # not a reference key (its loader ignores it): the inflow generator's
+            # state, so a resumed run continues the same inflow
+            kw["inflow_state_json"] = np.array(json.dumps(self._inflow.state()))
         np.savez_compressed(path, **kw)
 
     def load_checkpoint(self, path: str) -> None:
@@ -329,6 +333,8 @@ class Solver(DeviceSolver):
         self.step_count = int(data["step_count"])
         if "g" in data and self.thermal:
             self.g = data["g"]
+        if "inflow_state_json" in data and self.synthetic_inflow:
+            self._inflow.set_state(json.loads(str(data["inflow_state_json"])))
 
     @classmethod
     def from_checkpoint(cls, path: str) -> "Solver":

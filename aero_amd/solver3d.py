@@ -345,6 +345,10 @@ class Solver3D(DeviceSolver):
             # not a reference key (its loader ignores it): the SEM eddies and
             # random state, so a resumed run continues the same inlet turbulence
             kw["sem_state_json"] = np.array(json.dumps(self._sem.state()))
+        if self.synthetic_inflow:
+            # not a reference key (its loader ignores it): the inflow generator's
+            # state, so a resumed run continues the same inflow
+            kw["inflow_state_json"] = np.array(json.dumps(self._inflow.state()))
         np.savez_compressed(path, **kw)
 
     def load_checkpoint(self, path: str) -> None:
@@ -371,6 +375,8 @@ class Solver3D(DeviceSolver):
             self.g = data["g"]
         if "sem_state_json" in data and self._sem is not None:
             self._sem.set_state(json.loads(str(data["sem_state_json"])))
+        if "inflow_state_json" in data and self.synthetic_inflow:
+            self._inflow.set_state(json.loads(str(data["inflow_state_json"])))
 
     @classmethod
     def from_checkpoint(cls, path: str, backend: Optional[str] = None) -> "Solver3D":

@@ -240,3 +240,20 @@ def test_sem_state_round_trip_continues_the_same_planes():
     for pa, pb in zip(a.fluctuation(), b.fluctuation()):
         np.testing.assert_array_equal(pa, pb)
     assert not np.array_equal(a.yk, SEMInlet(24, 18, 0.05, 0.05, 3.0, 40, rng_seed=5).yk)  # eddies did regenerate
+
+
+def test_synthetic_inflow_state_round_trip():
+    """SyntheticInflow2D/3D.state() through JSON into a generator with another
+    seed: the next planes are those of the uninterrupted generator."""
+    import json
+    for make in (lambda seed: SyntheticInflow2D(40, 0.05, 0.05, seed),
+                 lambda seed: SyntheticInflow3D(12, 20, 0.05, 0.05, seed)):
+        a = make(7)
+        for _ in range(9):
+            a.next()
+        b = make(99)
+        b.set_state(json.loads(json.dumps(a.state())))
+        for _ in range(5):
+            pa, pb = a.next(), b.next()
+            for x, y in zip(pa if isinstance(pa, tuple) else (pa,), pb if isinstance(pb, tuple) else (pb,)):
+                np.testing.assert_array_equal(x, y)
