@@ -11,6 +11,7 @@ from __future__ import annotations
 import ctypes
 import time
 import warnings
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -28,6 +29,50 @@ def resolve_backend(backend: str) -> str:
     if backend not in VALID_BACKENDS:
         raise ValueError(f"Unknown backend '{backend}'. Choose 'auto', 'numpy', 'numba', 'cupy' or 'hip'.")
     return "hip"
+
+
+class _WriteThrough:
+    """Shared by a state array handed out by DeviceSolver.f / .g and every
+    view of it: the solver, the attribute, the whole array and the state
+    generation it was read at."""
+
+    __slots__ = ("owner", "name", "root", "gen")
+
+    def __init__(self, owner, name, root, gen):
+        self.owner, self.name, self.root, self.gen = owner, name, root, gen
+
+
+class LiveField(np.ndarray):
+    """The reference keeps self.f / self.g as numpy arrays, so scripts edit
+    them in place (``s.f[:, y, x] += 0.05``).  Here the state lives on the
+    device and f / g return host copies; an element assignment into such a
+    copy, or into any view of it, writes the whole array back while the
+    solver's state is still the one the copy was read from.  After a step or
+    another write the copy is detached, as an array the reference replaced
+    would be.  Arrays computed from it (``s.f * 2``) are plain copies."""
+
+    def __array_finalize__(self, obj):
+        # only views share the state array's memory (a ufunc result is a view
+        # of a fresh buffer: no overlap)
+        wt = getattr(obj, "_wt", None)
+        self._wt = wt if wt is not None and np.may_share_memory(self, wt.root) else None
+
+    def __setitem__(self, key, value):
+        super().__setitem__(key, value)
+        wt = getattr(self, "_wt", None)
+        if wt is None:
+            return
+        owner = wt.owner()
+        if owner is None or owner._state_gen != wt.gen:
+            return
+        setattr(owner, wt.name, np.asarray(wt.root))
+        wt.gen = owner._state_gen
+
+
+def _live(owner, name, arr):
+    out = arr.view(LiveField)
+    out._wt = _WriteThrough(weakref.ref(owner), name, out, owner._state_gen)
+    return out
 
 
 class DeviceSolver:
@@ -66,7 +111,7 @@ class DeviceSolver:
         """The reference's self.f: post-boundary, pre-collision populations."""
         out = np.empty(self._ref_shape(), dtype=np.float64)
         self._ctx.call("aero_get_f", _abi.ptr(out))
-        return out
+        return _live(self, "f", out)
 
     @f.setter
     def f(self, value) -> None:
@@ -74,6 +119,14 @@ class DeviceSolver:
         if arr.shape != self._ref_shape():
             raise ValueError(f"f must have shape {self._ref_shape()}, got {arr.shape}")
         self._ctx.call("aero_set_f", _abi.ptr(arr))
+        self._bump()
+
+    # generation of the device state: every step and every write bumps it,
+    # which detaches host copies handed out before (LiveField)
+    _state_gen = 0
+
+    def _bump(self) -> None:
+        self._state_gen += 1
 
     def set_equilibrium(self, rho=None, ux=None, uy=None, uz=None) -> None:
         """self.f = compute_feq(rho, ux, uy[, uz]) evaluated on the device
@@ -93,6 +146,7 @@ class DeviceSolver:
                 raise ValueError(f"{name} must have shape {shape}, got {a.shape}")
             args.append(a)
         self._ctx.call("aero_set_f_equilibrium", *[_abi.ptr(a) for a in args])
+        self._bump()
 
     @property
     def g(self) -> Optional[np.ndarray]:
@@ -100,7 +154,7 @@ class DeviceSolver:
             return None
         out = np.empty(self._ref_shape(), dtype=np.float64)
         self._ctx.call("aero_get_g", _abi.ptr(out))
-        return out
+        return _live(self, "g", out)
 
     @g.setter
     def g(self, value) -> None:
@@ -112,6 +166,7 @@ class DeviceSolver:
         if arr.shape != self._ref_shape():
             raise ValueError(f"g must have shape {self._ref_shape()}, got {arr.shape}")
         self._ctx.call("aero_set_g", _abi.ptr(arr))
+        self._bump()
 
     def _prev_shape(self):
         s = self._ref_shape()
@@ -129,6 +184,7 @@ class DeviceSolver:
         if arr.shape != self._prev_shape():
             raise ValueError(f"f_outlet_prev must have shape {self._prev_shape()}, got {arr.shape}")
         self._ctx.call("aero_set_outlet_prev", _abi.ptr(arr))
+        self._bump()
 
     def _macroscopic(self):
         n = self._N
@@ -169,6 +225,7 @@ class DeviceSolver:
         self._ctx.call("aero_step", ctypes.c_int32(n), ctypes.c_int64(self.step_count),
                        _abi.ptr(planes), _abi.ptr(hist))
         self.step_count += n
+        self._bump()
         return hist
 
     def close(self) -> None:

@@ -45,6 +45,7 @@ class Batch:
                                               step0s.ctypes.data, pptr, hptr))
         for s, rows in zip(solvers, hists):
             s.step_count += n
+            s._bump()
             if record:
                 s._record(rows)
         return hists

@@ -257,3 +257,40 @@ def test_synthetic_inflow_state_round_trip():
             pa, pb = a.next(), b.next()
             for x, y in zip(pa if isinstance(pa, tuple) else (pa,), pb if isinstance(pb, tuple) else (pb,)):
                 np.testing.assert_array_equal(x, y)
+
+
+def test_state_arrays_write_through_while_current():
+    """DeviceSolver.f / .g hand out LiveField copies: an element assignment
+    into one, or into a view of it, writes the array back while the state is
+    the one it was read from; after a step (a new state generation) it is a
+    detached copy, and arrays computed from it are plain copies."""
+    from aero_amd._base import _live
+
+    class Owner:  # the DeviceSolver attribute protocol, without a device
+        _state_gen = 0
+
+        def __init__(self):
+            self.data, self.writes = np.zeros((3, 4, 5)), 0
+
+        @property
+        def f(self):
+            return _live(self, "f", self.data.copy())
+
+        @f.setter
+        def f(self, v):
+            self.data, self.writes = np.array(v), self.writes + 1
+            self._state_gen += 1
+
+    o = Owner()
+    o.f[:, 1, 2] += 0.5  # the reference scripts' in-place edit
+    assert o.data[0, 1, 2] == 0.5 and o.writes == 1
+    a = o.f
+    a[0][3, 4] = 7.0  # through a view
+    a[1, 0, 0] = 2.0  # the same copy stays current after its own write-back
+    assert o.data[0, 3, 4] == 7.0 and o.data[1, 0, 0] == 2.0 and o.writes == 3
+    o._state_gen += 1  # a step
+    a[2, 0, 0] = 9.0
+    assert o.data[2, 0, 0] == 0.0 and o.writes == 3
+    b = o.f * 2
+    b[0, 0, 0] = 5.0
+    assert o.writes == 3 and type(np.asarray(o.f)) is np.ndarray
