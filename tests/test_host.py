@@ -294,3 +294,29 @@ def test_state_arrays_write_through_while_current():
     b = o.f * 2
     b[0, 0, 0] = 5.0
     assert o.writes == 3 and type(np.asarray(o.f)) is np.ndarray
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_lattice_tables_and_equilibrium(dim):
+    """tests/test_d2q9.py:21-110 (and the same properties for D3Q19): weights
+    sum to 1, opposite directions are mutual and reverse e, c_s^2 = 1/3, the
+    equilibrium carries the density and momentum it is built from, stays
+    positive at low Mach number, and compute_macroscopic inverts it."""
+    from aero_amd import lattice as lat
+    E, W, OPP = (lat.E, lat.W, lat.OPP) if dim == 2 else (lat.E3, lat.W3, lat.OPP3)
+    assert abs(W.sum() - 1.0) < 1e-14 and abs(lat.CS2 - 1.0 / 3.0) < 1e-15
+    for i in range(len(W)):
+        assert OPP[OPP[i]] == i and (E[OPP[i]] == -E[i]).all()
+    rng = np.random.default_rng(4)
+    shape = (10, 20) if dim == 2 else (6, 8, 10)
+    rho = 1.0 + 0.01 * rng.standard_normal(shape)
+    u = [0.05 + 0.01 * rng.standard_normal(shape)] + [0.01 * rng.standard_normal(shape) for _ in range(dim - 1)]
+    feq = lat.compute_feq(rho, *u) if dim == 2 else lat.compute_feq_3d(rho, *u)
+    np.testing.assert_allclose(feq.sum(axis=0), rho, rtol=0, atol=1e-13)
+    for d in range(dim):
+        np.testing.assert_allclose(np.tensordot(E[:, d].astype(float), feq, axes=1), rho * u[d], rtol=0, atol=1e-14)
+    assert feq.min() >= 0.0
+    mac = lat.compute_macroscopic(feq) if dim == 2 else lat.compute_macroscopic_3d(feq)
+    np.testing.assert_allclose(mac[0], rho, rtol=0, atol=1e-13)
+    for d in range(dim):
+        np.testing.assert_allclose(mac[1 + d], u[d], rtol=0, atol=1e-13)
