@@ -416,7 +416,7 @@ __device__ __forceinline__ void apply_inlet(const StepParams& p, const StepDyn& 
 template <class L, class C>
 __device__ __forceinline__ void sponge_and_walls(const StepParams& p, int x, int y, C fn[L::Q]) {
   constexpr int Q = L::Q;
-  if (p.sigma != nullptr) {  // sponge.py:24-70
+  if (p.sigma != nullptr && x >= p.sponge_x0) {  // sponge.py:24-70; sigma is 0 before sponge_x0
     const double s = p.sigma[x];
     if (s > 0.0) {
       const C cs = (C)s;
