@@ -155,6 +155,34 @@ def run_form(name, steps, warmup):
 
 
 LES_FORMS = {"mrt_les_scalar_512_f32": "f32", "mrt_les_scalar_512_f16": "f16"}
+CYL2D_FORMS = {"cylinder_2d_400x200_f64": "f64", "cylinder_2d_400x200_f32": "f32",
+               "cylinder_2d_400x200_f16": "f16"}
+
+
+def run_cylinder_2d_form(name, steps, warmup):
+    """The reference's CPU-runnable case on the device: D2Q9 BGK cylinder,
+    400x200, radius 20 at (100, 100), u0 = 0.05, Re = 100, velocity inlet,
+    zero-gradient outlet with a 40-column sponge, free-slip walls, seeded
+    1e-4 transverse noise.  One case alone: 80,000 cells fill about 1,250
+    wavefronts, so the step is launch-latency-bound."""
+    import aero_amd
+    from aero_amd.geometry import Cylinder
+    from aero_amd.lattice import compute_feq
+    ny, nx = 200, 400
+    st = CYL2D_FORMS[name]
+    solid = Cylinder(radius=20.0, cx_frac=0.25, cy_frac=0.5).mark_solid(ny, nx)
+    u0, D = 0.05, 40.0
+    s = aero_amd.Solver(Ny=ny, Nx=nx, solid=solid, omega=1.0 / (3.0 * u0 * D / 100.0 + 0.5), u0=u0, D=D,
+                        collision="bgk", inlet_bc="velocity", outlet_bc="zerogradient", sponge_thickness=40,
+                        wall_bc="slip", storage=st)
+    rng = np.random.default_rng(0)
+    s.f = compute_feq(np.ones((ny, nx)), np.full((ny, nx), u0), rng.uniform(-1e-4, 1e-4, (ny, nx)))
+    w, k, rows = timed(s, steps, warmup)
+    cells = nx * ny
+    bpl = 2 * 9 * {"f64": 8, "f32": 4, "f16": 2}[st] * cells
+    s.close()
+    return {"form": name, "MLUPS": round(cells * steps / w / 1e6, 2), "kernel_ms": round(k, 5),
+            "roofline_frac": round(bpl / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "cd_last": float(rows[-1, 0])}
 
 
 def run_les_form(name, steps, warmup):
@@ -316,7 +344,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving / batched-solves leg")
     ap.add_argument("--serve-full", action="store_true",
                     help="serving leg on the full 256-case workload and the full arrival-rate sweep")
-    ap.add_argument("--form", choices=sorted(FORMS) + sorted(LES_FORMS), default=None,
+    ap.add_argument("--form", choices=sorted(FORMS) + sorted(LES_FORMS) + sorted(CYL2D_FORMS), default=None,
                     help="only time one storage form of the TRT sphere (prints one JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (calibrates the HBM byte counters)")
@@ -329,6 +357,9 @@ def main():
               "has no multi-GPU path (a large grid would split into x-slabs with one face exchange "
               "per step).", file=sys.stderr)
         sys.exit(2)
+    if a.form in CYL2D_FORMS:
+        print(json.dumps(run_cylinder_2d_form(a.form, a.steps, a.warmup)), flush=True)
+        return
     if a.form in LES_FORMS:
         print(json.dumps(run_les_form(a.form, a.steps, a.warmup)), flush=True)
         return
