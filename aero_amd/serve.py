@@ -288,7 +288,9 @@ class Server:
         chunk j run while the device advances chunk j+1 (a worker thread
         holds the device call).  A case found done at chunk j is retired with
         its history cut at the end of chunk j, exactly what it would have
-        been alone; its speculative chunk is discarded."""
+        been alone; its speculative chunk is discarded.  A case whose history
+        has reached max_steps is decided before the next chunk instead (it
+        stops whatever the check finds), so it costs no speculative chunk."""
         from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
         if any(r[1].shape == "polygon" for r in requests):
             import matplotlib.path  # noqa: F401  (imported once here, not raced by the builder threads)
@@ -324,6 +326,16 @@ class Server:
                         wait([b[0] for b in building], return_when=FIRST_COMPLETED)
                     elif waiting:
                         sleep(max(0.0, waiting[0][0] - (clock() - t0)))
+                    continue
+                # A case whose history has reached max_steps stops at this check
+                # whatever it finds, so it is decided now rather than stepped for
+                # a speculative chunk: same reason, same history cut, same result.
+                now = clock() - t0
+                for lv in [lv for lv in live if len(lv.solver.Cd_history) >= lv.spec.max_steps]:
+                    done.append(self._result(lv, self._done(lv), now, len(lv.solver.Cd_history)))
+                    lv.solver.close()
+                    live.remove(lv)
+                if not live:
                     continue
                 n = self.chunk
                 batch = self._batch_obj()
