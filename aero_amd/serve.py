@@ -226,7 +226,7 @@ class Server:
     def __init__(self, chunk: int = 500, max_live: int = 512, mem_reserve: int = 8 << 30,
                  convergence_window: int = 5000, convergence_tol: float = 0.01, strouhal_window: int = 4096,
                  strouhal_tol: float = 0.05, batch=None, free_memory: Optional[Callable[[], int]] = None,
-                 keep_history: bool = False, build_workers: int = 4):
+                 keep_history: bool = False, build_workers: int = 8):
         self.chunk = int(chunk)
         self.build_workers = max(1, int(build_workers))
         self.max_live = int(max_live)
