@@ -57,6 +57,11 @@ class LiveField(np.ndarray):
         wt = getattr(obj, "_wt", None)
         self._wt = wt if wt is not None and np.may_share_memory(self, wt.root) else None
 
+    def __array_wrap__(self, obj, context=None, return_scalar=False):
+        # results of ufuncs and reductions are plain arrays (or scalars)
+        out = obj.view(np.ndarray) if isinstance(obj, np.ndarray) else obj
+        return out[()] if return_scalar else out
+
     def __setitem__(self, key, value):
         super().__setitem__(key, value)
         wt = getattr(self, "_wt", None)

@@ -294,6 +294,7 @@ def test_state_arrays_write_through_while_current():
     b = o.f * 2
     b[0, 0, 0] = 5.0
     assert o.writes == 3 and type(np.asarray(o.f)) is np.ndarray
+    assert type(b) is np.ndarray and type(o.f.sum(axis=0)) is np.ndarray and isinstance(o.f.sum(), float)
 
 
 @pytest.mark.parametrize("dim", [2, 3])
