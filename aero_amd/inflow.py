@@ -37,15 +37,25 @@ class SyntheticInflow2D:
         self.rng.bit_generator.state = st["rng"]
 
     def next(self) -> np.ndarray:
-        noise = self.rng.standard_normal(self.ny)
-        filtered = (noise + np.roll(noise, 1) + np.roll(noise, -1)) / 3.0
-        self.field = 0.75 * self.field + 0.25 * filtered
-        self.field -= float(np.mean(self.field))
-        rms = float(np.sqrt(np.mean(self.field ** 2)))
+        return self.next_n(1)[0]
+
+    def next_n(self, n: int) -> np.ndarray:
+        """(n, ny): the next n planes, equal to n calls of next().  The noise
+        of all n steps is drawn in one block (numpy's generator gives the same
+        values as n successive draws) and filtered at once; only the
+        recursion runs per step."""
+        noise = self.rng.standard_normal((n, self.ny))
+        filtered = (noise + np.roll(noise, 1, axis=1) + np.roll(noise, -1, axis=1)) / 3.0
+        out = np.empty((n, self.ny))
         target = self.intensity * self.u0
-        if rms > 1e-12 and target > 0.0:
-            self.field *= target / rms
-        return self.field.copy()
+        for k in range(n):
+            self.field = 0.75 * self.field + 0.25 * filtered[k]
+            self.field -= float(np.mean(self.field))
+            rms = float(np.sqrt(np.mean(self.field ** 2)))
+            if rms > 1e-12 and target > 0.0:
+                self.field *= target / rms
+            out[k] = self.field
+        return out
 
 
 class SyntheticInflow3D:
@@ -70,25 +80,33 @@ class SyntheticInflow3D:
         self.fy, self.fz = fy, fz
         self.rng.bit_generator.state = st["rng"]
 
-    @staticmethod
-    def _filt(a):
-        return (a + np.roll(a, 1, axis=0) + np.roll(a, -1, axis=0)
-                + np.roll(a, 1, axis=1) + np.roll(a, -1, axis=1)) / 5.0
-
     def next(self) -> Tuple[np.ndarray, np.ndarray]:
-        ny_ = self.rng.standard_normal(self.shape)
-        nz_ = self.rng.standard_normal(self.shape)
-        self.fy = 0.8 * self.fy + 0.2 * self._filt(ny_)
-        self.fz = 0.8 * self.fz + 0.2 * self._filt(nz_)
-        self.fy -= float(np.mean(self.fy))
-        self.fz -= float(np.mean(self.fz))
+        fy, fz = self.next_n(1)
+        return fy[0], fz[0]
+
+    def next_n(self, n: int) -> Tuple[np.ndarray, np.ndarray]:
+        """(n, nz, ny) each: the next n (uy, uz) planes, equal to n calls of
+        next().  The noise of all n steps is drawn in one block (the
+        generator gives the same values in the same order); each step's two
+        planes are filtered together."""
+        noise = self.rng.standard_normal((n, 2) + self.shape)
+        oy, oz = np.empty((n,) + self.shape), np.empty((n,) + self.shape)
         target = self.intensity * self.u0
-        rms = float(np.sqrt(np.mean(self.fy ** 2 + self.fz ** 2)))
-        if rms > 1e-12 and target > 0.0:
-            s = target / rms
-            self.fy *= s
-            self.fz *= s
-        return self.fy.copy(), self.fz.copy()
+        for k in range(n):
+            a = noise[k]  # both planes of step k, filtered together
+            filt = (a + np.roll(a, 1, axis=1) + np.roll(a, -1, axis=1) + np.roll(a, 1, axis=2)
+                    + np.roll(a, -1, axis=2)) / 5.0
+            self.fy = 0.8 * self.fy + 0.2 * filt[0]
+            self.fz = 0.8 * self.fz + 0.2 * filt[1]
+            self.fy -= float(np.mean(self.fy))
+            self.fz -= float(np.mean(self.fz))
+            rms = float(np.sqrt(np.mean(self.fy ** 2 + self.fz ** 2)))
+            if rms > 1e-12 and target > 0.0:
+                s = target / rms
+                self.fy *= s
+                self.fz *= s
+            oy[k], oz[k] = self.fy, self.fz
+        return oy, oz
 
 
 class SEMInlet:

@@ -175,9 +175,8 @@ class Solver(DeviceSolver):
         if not (self.synthetic_inflow and self.inlet_bc == "velocity"):
             return None
         planes = np.zeros((n, 3, self.Ny))
-        for k in range(n):
-            planes[k, 0] = self.u0
-            planes[k, 1] = self._inflow.next()
+        planes[:, 0] = self.u0
+        planes[:, 1] = self._inflow.next_n(n)
         return planes
 
     # ---------------------------------------------------------------- step

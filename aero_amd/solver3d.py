@@ -196,11 +196,8 @@ class Solver3D(DeviceSolver):
             return planes
         if self.synthetic_inflow:
             planes = np.empty((n, 3, self.Nz, self.Ny))
-            for k in range(n):
-                uy, uz = self._inflow.next()
-                planes[k, 0] = self.u0
-                planes[k, 1] = uy
-                planes[k, 2] = uz
+            planes[:, 0] = self.u0
+            planes[:, 1], planes[:, 2] = self._inflow.next_n(n)
             return planes
         return None
 
