@@ -321,3 +321,17 @@ def test_lattice_tables_and_equilibrium(dim):
     np.testing.assert_allclose(mac[0], rho, rtol=0, atol=1e-13)
     for d in range(dim):
         np.testing.assert_allclose(mac[1 + d], u[d], rtol=0, atol=1e-13)
+
+
+def test_synthetic_inflow_chunks_equal_single_steps():
+    """next_n(n) draws a chunk's noise in one block; its planes must be
+    bitwise those of n successive next() calls, also across chunk borders."""
+    a, b = SyntheticInflow2D(37, 0.05, 0.03, 11), SyntheticInflow2D(37, 0.05, 0.03, 11)
+    ref = np.array([a.next() for _ in range(30)])
+    np.testing.assert_array_equal(np.concatenate([b.next_n(7), b.next_n(1), b.next_n(22)]), ref)
+    a3, b3 = SyntheticInflow3D(9, 14, 0.05, 0.03, 12), SyntheticInflow3D(9, 14, 0.05, 0.03, 12)
+    steps = [a3.next() for _ in range(20)]
+    y1, z1 = b3.next_n(13)
+    y2, z2 = b3.next_n(7)
+    np.testing.assert_array_equal(np.concatenate([y1, y2]), np.array([s[0] for s in steps]))
+    np.testing.assert_array_equal(np.concatenate([z1, z2]), np.array([s[1] for s in steps]))
