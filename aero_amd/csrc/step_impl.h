@@ -1025,13 +1025,17 @@ __device__ __forceinline__ void reduce_forces_block(const ForceReduce& r) {
 // Occupancy target of the step kernel: the 4- and 2-byte BGK/TRT forms ask
 // the register allocator for 7 waves per SIMD (at most 72 VGPRs; the common
 // pull + collide path needs ~50, the rare link / outlet / reduction paths
-// would otherwise raise the whole kernel to ~85).  MRT in the 2-byte forms
+// would otherwise raise the whole kernel to ~85).  The 2-byte D2Q9 forms ask
+// for 8 (64 VGPRs, a 3-VGPR spill on a rare path): a D2Q9 wave keeps only 9
+// two-byte loads in flight, so the extra wave per SIMD raised their batches
+// 5%.  MRT in the 2-byte forms
 // targets 6 (the compiler's own choice, 5, measured 10% slower in D3Q19)
 // unless it carries the scalar too (6 waves spill there: 16% slower); FP64,
 // FP32 MRT and MRT with the scalar keep the compiler's choice.
 template <class L, int COLL, int ST, bool THERMAL>
 struct StepWaves {
-  static constexpr int value = (ST != ST_F64 && COLL != COLL_MRT) ? 7
+  static constexpr int value = (ST != ST_F64 && COLL != COLL_MRT)
+                                  ? ((L::Q == 9 && (ST == ST_F16S || ST == ST_BF16S)) ? 8 : 7)
                               : (COLL == COLL_MRT && (ST == ST_F16S || ST == ST_BF16S) && !THERMAL) ? 6
                                                                                                     : 1;
 };
