@@ -1028,15 +1028,15 @@ __device__ __forceinline__ void reduce_forces_block(const ForceReduce& r) {
 // would otherwise raise the whole kernel to ~85).  The 2-byte D2Q9 forms ask
 // for 8 (64 VGPRs, a 3-VGPR spill on a rare path): a D2Q9 wave keeps only 9
 // two-byte loads in flight, so the extra wave per SIMD raised their batches
-// 5%.  MRT in the 2-byte forms
-// targets 6 (the compiler's own choice, 5, measured 10% slower in D3Q19)
+// 5%; likewise D2Q9 MRT (+5%).  D3Q19 MRT in the 2-byte forms
+// targets 6 (the compiler's own choice, 5, measured 10% slower)
 // unless it carries the scalar too (6 waves spill there: 16% slower); FP64,
 // FP32 MRT and MRT with the scalar keep the compiler's choice.
 template <class L, int COLL, int ST, bool THERMAL>
 struct StepWaves {
   static constexpr int value = (ST != ST_F64 && COLL != COLL_MRT)
                                   ? ((L::Q == 9 && (ST == ST_F16S || ST == ST_BF16S)) ? 8 : 7)
-                              : (COLL == COLL_MRT && (ST == ST_F16S || ST == ST_BF16S) && !THERMAL) ? 6
+                              : (COLL == COLL_MRT && (ST == ST_F16S || ST == ST_BF16S) && !THERMAL) ? (L::Q == 9 ? 8 : 6)
                                                                                                     : 1;
 };
 
