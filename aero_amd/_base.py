@@ -9,6 +9,7 @@ with the host only at check points (aero/lbm/solver.py:470-632).
 from __future__ import annotations
 
 import ctypes
+import json
 import time
 import warnings
 import weakref
@@ -29,6 +30,29 @@ def resolve_backend(backend: str) -> str:
     if backend not in VALID_BACKENDS:
         raise ValueError(f"Unknown backend '{backend}'. Choose 'auto', 'numpy', 'numba', 'cupy' or 'hip'.")
     return "hip"
+
+
+# Constructor arguments this build adds to the reference's (storage format and
+# the labelled operator variants).  Checkpoints keep them under a key of their
+# own, so ``params_json`` holds only the reference's constructor arguments and
+# the reference's from_checkpoint (aero/lbm/solver.py:711-727,
+# solver3d.py:816-838) can rebuild a solver from a checkpoint written here.
+EXTRA_PARAMS_KEY = "aero_amd_params_json"
+EXTRA_PARAM_NAMES = ("storage", "trt_variant", "bouzidi_variant")
+
+
+def extra_params_entry(extra: dict) -> dict:
+    return {EXTRA_PARAMS_KEY: np.array(json.dumps(extra))}
+
+
+def checkpoint_params(data) -> dict:
+    """Constructor arguments of a checkpoint: the reference's params_json plus,
+    when present, this build's own entries (also accepted inside params_json,
+    as older checkpoints of this build wrote them)."""
+    params = json.loads(str(data["params_json"]))
+    if EXTRA_PARAMS_KEY in data:
+        params.update(json.loads(str(data[EXTRA_PARAMS_KEY])))
+    return params
 
 
 class _WriteThrough:

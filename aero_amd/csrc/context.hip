@@ -1,6 +1,7 @@
 // Host side of the C ABI: context state, geometry records, the step loop and
 // state materialisation.  See include/aero_hip.h for the contract.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -121,7 +122,10 @@ struct StepRunner {
   double* stage[2] = {nullptr, nullptr};
   size_t ncase_cap = 0, rows_cap = 0;
   hipEvent_t ev[2] = {nullptr, nullptr};
-  std::vector<std::pair<const void*, int>> tile_key;  // case list the tile table was built for
+  // creation serials of the case list the tile table was built for: a serial
+  // is never reused, so a context created at a freed context's address never
+  // matches a stale table
+  std::vector<unsigned long long> tile_key;
 
   template <class T>
   static void host_grow(T*& p, size_t n) {
@@ -169,8 +173,11 @@ struct StepRunner {
   }
 };
 
+static std::atomic<unsigned long long> g_ctx_serial{0};
+
 struct aero_ctx {
   aero_desc d;
+  unsigned long long serial = ++g_ctx_serial;  // unique per context for the process lifetime
   int device = 0;
   hipStream_t stream = nullptr;
   int Q = 0;
@@ -493,8 +500,8 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
     }
     members[g].push_back(i);
   }
-  std::vector<std::pair<const void*, int>> key;
-  for (aero_ctx* c : cs) key.emplace_back((const void*)c, c->nblocks);
+  std::vector<unsigned long long> key;
+  for (aero_ctx* c : cs) key.push_back(c->serial);
   {
     std::vector<int2> tiles, nat;
     for (size_t g = 0; g < groups.size(); ++g) {
@@ -868,6 +875,10 @@ static void set_geometry_impl(aero_ctx* c, const uint8_t* solid, int64_t nlinks,
       fl[kv.first] |= FLAG_LINKS;
     }
     c->nrec = (int)recs.size();
+    // new flags carry no IBM band: immersed-boundary mode ends until
+    // aero_set_ibm marks the band again
+    c->ibm_on = 0;
+    c->ibm_coeff = 0.0;
     c->rec_cells.clear();
     c->rec_masks.clear();
     for (auto& r : recs) {

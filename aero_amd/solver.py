@@ -16,7 +16,8 @@ from typing import Callable, Optional
 import numpy as np
 
 from . import _abi
-from ._base import DeviceSolver, resolve_backend, run_loop, warn_hdf5
+from ._base import (DeviceSolver, checkpoint_params, extra_params_entry, resolve_backend, run_loop,
+                    warn_hdf5)
 from .diagnostics import coefficient_spectrum, detect_statistical_stationarity
 from .inflow import SyntheticInflow2D
 from .lattice import M9_inv, build_s_vec, compute_feq, base_nu_from_omega
@@ -289,8 +290,10 @@ class Solver(DeviceSolver):
             synthetic_inflow=self.synthetic_inflow, synthetic_inflow_intensity=self.synthetic_inflow_intensity,
             thermal=self.thermal, T_hot=self.T_hot, T_cold=self.T_cold, alpha_T=self.alpha_T,
             buoyancy=self.buoyancy, g_gravity=self.g_gravity, beta=self.beta, T_ref=self.T_ref,
-            collision=self.collision, backend=self.backend, storage=self.storage, trt_variant=self.trt_variant,
-            bouzidi_variant=self.bouzidi_variant)
+            collision=self.collision, backend="auto")
+
+    def _extra_params(self) -> dict:
+        return dict(storage=self.storage, trt_variant=self.trt_variant, bouzidi_variant=self.bouzidi_variant)
 
     def save_checkpoint(self, path: str) -> None:
         """Same keys and layout as the reference (solver.py:638-686)."""
@@ -299,7 +302,8 @@ class Solver(DeviceSolver):
                   Cl_history=np.array(self.Cl_history, dtype=np.float64),
                   Cm_history=np.array(self.Cm_history, dtype=np.float64),
                   step_count=np.array(self.step_count, dtype=np.int64), surface_links=self.surface_links,
-                  q_vals=self.q_vals, params_json=np.array(json.dumps(self._params())))
+                  q_vals=self.q_vals, params_json=np.array(json.dumps(self._params())),
+                  **extra_params_entry(self._extra_params()))
         if self.phi is not None:
             kw["phi"] = self.phi
         if self.thermal:
@@ -323,6 +327,9 @@ class Solver(DeviceSolver):
             import ctypes
             self._ctx.call("aero_set_geometry", _abi.ptr(sol), ctypes.c_int64(len(l4)), _abi.ptr(l4),
                            _abi.ptr(np.ascontiguousarray(q, dtype=np.float32)))
+            if self.ibm_enabled:
+                # a new geometry ends immersed-boundary mode: mark the band again
+                self._set_ibm(self.phi)
         self.f = data["f"]
         self._f_outlet_prev = data["f_outlet_prev"]
         self.Cd_history = list(data["Cd_history"])
@@ -339,7 +346,7 @@ class Solver(DeviceSolver):
     def from_checkpoint(cls, path: str) -> "Solver":
         """solver.py:711-727"""
         data = np.load(path, allow_pickle=False)
-        params = json.loads(str(data["params_json"]))
+        params = checkpoint_params(data)
         solver = cls(solid=data["solid"], phi=data["phi"] if "phi" in data else None, **params)
         solver.load_checkpoint(path)
         return solver

@@ -15,7 +15,8 @@ from typing import Callable, List, Optional
 import numpy as np
 
 from . import _abi
-from ._base import DeviceSolver, resolve_backend, run_loop, warn_hdf5
+from ._base import (DeviceSolver, checkpoint_params, extra_params_entry, resolve_backend, run_loop,
+                    warn_hdf5)
 from .diagnostics import coefficient_spectrum, detect_statistical_stationarity
 from .inflow import SEMInlet, SyntheticInflow3D
 from .lattice import M19_inv, W3, build_s3_vec, compute_feq_3d, base_nu_from_omega
@@ -318,9 +319,10 @@ class Solver3D(DeviceSolver):
             wall_velocity_top=self.wall_velocity_top, wall_velocity_bottom=self.wall_velocity_bottom,
             synthetic_inflow=self.synthetic_inflow, synthetic_inflow_intensity=self.synthetic_inflow_intensity,
             thermal=self.thermal, T_hot=self.T_hot, T_cold=self.T_cold, alpha_T=self.alpha_T,
-            buoyancy=self.buoyancy, g_gravity=self.g_gravity, beta=self.beta, T_ref=self.T_ref,
-            storage=self.storage, trt_variant=self.trt_variant,
-            bouzidi_variant=self.bouzidi_variant)
+            buoyancy=self.buoyancy, g_gravity=self.g_gravity, beta=self.beta, T_ref=self.T_ref)
+
+    def _extra_params(self) -> dict:
+        return dict(storage=self.storage, trt_variant=self.trt_variant, bouzidi_variant=self.bouzidi_variant)
 
     def save_checkpoint(self, path: str) -> None:
         """Same keys and layout as the reference (solver3d.py:727-783)."""
@@ -333,7 +335,8 @@ class Solver3D(DeviceSolver):
                   Cmy_history=np.array(self.Cmy_history, dtype=np.float64),
                   Cmz_history=np.array(self.Cmz_history, dtype=np.float64),
                   step_count=np.array(self.step_count, dtype=np.int64),
-                  params_json=np.array(json.dumps(self._params())))
+                  params_json=np.array(json.dumps(self._params())),
+                  **extra_params_entry(self._extra_params()))
         if self.phi is not None:
             kw["phi"] = self.phi
         if self.thermal:
@@ -359,6 +362,9 @@ class Solver3D(DeviceSolver):
             l4 = np.ascontiguousarray(links, dtype=np.int32)
             self._ctx.call("aero_set_geometry", _abi.ptr(sol), ctypes.c_int64(len(l4)), _abi.ptr(l4),
                            _abi.ptr(np.ascontiguousarray(q, dtype=np.float32)))
+            if self.ibm_enabled:
+                # a new geometry ends immersed-boundary mode: mark the band again
+                self._set_ibm(self.phi)
         self.f = data["f"]
         self._f_outlet_prev = data["f_outlet_prev"]
         self.Cd_history = list(data["Cd_history"])
@@ -379,7 +385,7 @@ class Solver3D(DeviceSolver):
     def from_checkpoint(cls, path: str, backend: Optional[str] = None) -> "Solver3D":
         """solver3d.py:816-838"""
         data = np.load(path, allow_pickle=False)
-        params = json.loads(str(data["params_json"]))
+        params = checkpoint_params(data)
         if backend is not None:
             params["backend"] = backend
         if "sem_state_json" in data:

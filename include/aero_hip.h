@@ -134,7 +134,9 @@ int aero_create(const aero_desc* desc, aero_ctx** out);
 int aero_destroy(aero_ctx* ctx);
 
 /* Geometry: solid mask and the surface links + q values built by
- * build_surface_links(_3d) (boundary.py:232-280, boundary3d.py:279-330). */
+ * build_surface_links(_3d) (boundary.py:232-280, boundary3d.py:279-330).
+ * Replaces the whole geometry, so it also leaves immersed-boundary mode:
+ * call aero_set_ibm again afterwards to keep it. */
 int aero_set_geometry(aero_ctx* ctx, const uint8_t* solid, int64_t nlinks, const int32_t* links,
                       const float* q_vals);
 /* Van Driest factor (1 - exp(-|phi|/A))^2 per cell (les.py:512-514), or NULL. */

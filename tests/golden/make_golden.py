@@ -5,6 +5,9 @@ Run once, in a container that holds the reference checkout at
 
     python tests/golden/make_golden.py          # every fixture
     python tests/golden/make_golden.py ibm      # only the step cases named *ibm*
+    python tests/golden/make_golden.py ckpt     # reference-written checkpoints
+    python tests/golden/make_golden.py signatures
+    python tests/golden/make_golden.py c1_long_run   # 20000 reference steps (~20 min)
 
 The reference is pure Python.  Numba is absent here, so every call takes the
 reference's NumPy fallback path, which its own tests hold equal to the Numba
@@ -294,11 +297,104 @@ def sem_fixture(R):
     return dict(planes=np.array(planes), xk=sem.xk, yk=sem.yk, zk=sem.zk, eps=sem.eps)
 
 
+def signatures_fixture(R):
+    """Parameter names of the reference constructors (solver.py:75-116,
+    solver3d.py:71-119): a checkpoint's params_json must use only these."""
+    import inspect
+    out = {}
+    for cls in ("Solver", "Solver3D"):
+        out[cls] = [n for n in inspect.signature(R[cls].__init__).parameters if n != "self"]
+    return out
+
+
+# Checkpoints written by the reference itself: the reference runs CKPT_AT
+# steps from the seeded state, saves with its own save_checkpoint, then runs
+# CKPT_MORE steps more; the fixture pair holds the checkpoint and the state
+# the continued run reaches.
+CKPT_CASES_2D = ("bgk_ibm", "trt_bouzidi_conv_slip")
+CKPT_CASES_3D = ("trt_ibm_les", "bgk_bouzidi_conv_slip")
+CKPT_AT, CKPT_MORE = 12, 9
+
+
+def ckpt_fixtures(R):
+    for dim, names, table in ((2, CKPT_CASES_2D, CASES_2D), (3, CKPT_CASES_3D, CASES_3D)):
+        for name in names:
+            spec = table[name]
+            seed = 300 + list(table).index(name)
+            kw = dict(spec["kw"])
+            if dim == 2:
+                solid, phi, D = geom2d(R, spec["geom"])
+                omega = spec.get("omega", 1.0 / (3.0 * 0.05 * D / 60.0 + 0.5))
+                use_phi = kw.get("bouzidi", False) or kw.get("ibm_enabled", False)
+                s = R["Solver"](Ny=NY2, Nx=NX2, solid=solid, omega=omega, u0=0.05, D=D, backend="numpy",
+                                phi=phi if use_phi else None, **kw)
+                f0, _ = noisy_feq_2d(R, NY2, NX2, 0.05, seed)
+            else:
+                solid, phi, D = geom3d(R, spec["geom"])
+                omega = spec.get("omega", 1.0 / (3.0 * 0.05 * D / 40.0 + 0.5))
+                use_phi = kw.get("bouzidi", False) or kw.get("ibm_enabled", False)
+                s = R["Solver3D"](Nz=NZ3, Ny=NY3, Nx=NX3, solid=solid, omega=omega, u0=0.05, D=D,
+                                  backend="numpy", phi=phi if use_phi else None, **kw)
+                f0, _, _ = noisy_feq_3d(R, NZ3, NY3, NX3, 0.05, seed)
+            s.f = f0.copy()
+            s._f_outlet_prev = f0[..., -1].copy()
+            for _ in range(CKPT_AT):
+                s._step()
+            ck = OUT / f"ckpt_ref{dim}d_{name}.npz"
+            s.save_checkpoint(str(ck))
+            cd = []
+            for _ in range(CKPT_MORE):
+                cd.append(s._step()[0])
+            np.savez_compressed(OUT / f"ckpt_ref{dim}d_{name}_cont.npz", f=np.array(s.f, dtype=np.float64),
+                                Cd_hist=np.array(cd), steps=np.int64(CKPT_MORE))
+            print("ckpt", dim, name, flush=True)
+
+
+def c1_long_run(R, steps=20000):
+    """The reference's CPU-runnable configuration run to its end: D2Q9 BGK
+    cylinder 400x200, r = 20 at (100, 100), u0 = 0.05, Re = 100, velocity
+    inlet, zero-gradient outlet with a 40-column sponge, free-slip walls,
+    equilibrium plus seeded 1e-4 transverse noise (seed 0), float64.
+    Observables as cli.py:429-439 reports them."""
+    from aero.diagnostics import compute_strouhal
+    ny, nx, u0, D = 200, 400, 0.05, 40.0
+    solid = R["Cylinder"](radius=20.0, cx_frac=0.25, cy_frac=0.5).mark_solid(ny, nx)
+    s = R["Solver"](Ny=ny, Nx=nx, solid=solid, omega=1.0 / (3.0 * u0 * D / 100.0 + 0.5), u0=u0, D=D,
+                    backend="numpy", collision="bgk", inlet_bc="velocity", outlet_bc="zerogradient",
+                    sponge_thickness=40, wall_bc="slip")
+    rng = np.random.default_rng(0)
+    uy = rng.uniform(-1e-4, 1e-4, (ny, nx))
+    f0 = R["compute_feq"](np.ones((ny, nx)), np.full((ny, nx), u0), uy)
+    s.f = f0.copy()
+    s._f_outlet_prev = f0[:, :, -1].copy()
+    cd, cl = [], []
+    for k in range(steps):
+        a, b = s._step()
+        cd.append(a)
+        cl.append(b)
+        if (k + 1) % 1000 == 0:
+            print("c1 step", k + 1, a, b, flush=True)
+    cd, cl = np.array(cd), np.array(cl)
+    st = compute_strouhal(list(cl), D, u0)
+    np.savez_compressed(OUT / "c1_long_run.npz", Cd_hist=cd, Cl_hist=cl, steps=np.int64(steps),
+                        Cd_mean_last4000=np.float64(cd[-4000:].mean()), Cl_std_last4000=np.float64(cl[-4000:].std()),
+                        St=np.float64(st if st is not None else np.nan), f_end=np.array(s.f, dtype=np.float64)[:, ::8, ::8])
+
+
 def main(only=None):
     """Write every fixture, or with `only` (a substring) just the matching
     step cases; a case's seed is its position in its table, so adding cases at
     the end leaves the existing fixtures' inputs unchanged."""
     R = _import_reference()
+    if only == "c1_long_run":
+        c1_long_run(R)
+        return
+    if only in (None, "signatures"):
+        (OUT / "ref_signatures.json").write_text(json.dumps(signatures_fixture(R), indent=1))
+    if only in (None, "ckpt"):
+        ckpt_fixtures(R)
+    if only in ("signatures", "ckpt"):
+        return
     if only is None:
         np.savez_compressed(OUT / "mrt_inverses.npz", M9_inv=R["M9_inv"], M19_inv=R["M19_inv"])
     for k, (name, spec) in enumerate(CASES_2D.items()):
