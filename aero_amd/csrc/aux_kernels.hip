@@ -198,8 +198,10 @@ struct Weights {
   double w[19];
 };
 
+// A reference-layout FP64 field (q x n, n = nrow * nx) into the storage
+// layout (row-interleaved, rows of q runs of xp values).
 template <int ST>
-__global__ __launch_bounds__(BLOCK) void store_from_f64_kernel(long long n, int q, const double* src,
+__global__ __launch_bounds__(BLOCK) void store_from_f64_kernel(long long n, int q, int nx, int xp, const double* src,
                                                                typename Storage<ST>::T* dst, Weights w,
                                                                double shift_scale) {
   using S = Storage<ST>;
@@ -207,22 +209,29 @@ __global__ __launch_bounds__(BLOCK) void store_from_f64_kernel(long long n, int 
   const long long k = (long long)blockIdx.x * BLOCK + threadIdx.x;
   if (k >= n * q) return;
   const int i = (int)(k / n);
-  S::store(dst, (size_t)k, (C)src[k], (C)(w.w[i] * shift_scale));
+  const long long c = k - (long long)i * n;
+  const long long row = c / nx;
+  const long long x = c - row * nx;
+  S::store(dst, (size_t)((row * q + i) * xp + x), (C)src[k], (C)(w.w[i] * shift_scale));
 }
 
-hipError_t launch_store_from_f64(int st, int q, long long n, const double* src, void* dst, double shift_scale,
-                                 hipStream_t s) {
+hipError_t launch_store_from_f64(int st, int q, long long n, int nx, int xp, const double* src, void* dst,
+                                 double shift_scale, hipStream_t s) {
   const dim3 g((unsigned)((n * q + BLOCK - 1) / BLOCK));
   Weights w{};
   for (int i = 0; i < q; ++i) w.w[i] = q == 9 ? D2Q9::W[i] : D3Q19::W[i];
   switch (st) {
-    case ST_F64: store_from_f64_kernel<ST_F64><<<g, BLOCK, 0, s>>>(n, q, src, (double*)dst, w, shift_scale); break;
-    case ST_F32: store_from_f64_kernel<ST_F32><<<g, BLOCK, 0, s>>>(n, q, src, (float*)dst, w, shift_scale); break;
+    case ST_F64:
+      store_from_f64_kernel<ST_F64><<<g, BLOCK, 0, s>>>(n, q, nx, xp, src, (double*)dst, w, shift_scale);
+      break;
+    case ST_F32:
+      store_from_f64_kernel<ST_F32><<<g, BLOCK, 0, s>>>(n, q, nx, xp, src, (float*)dst, w, shift_scale);
+      break;
     case ST_F16S:
-      store_from_f64_kernel<ST_F16S><<<g, BLOCK, 0, s>>>(n, q, src, (__half*)dst, w, shift_scale);
+      store_from_f64_kernel<ST_F16S><<<g, BLOCK, 0, s>>>(n, q, nx, xp, src, (__half*)dst, w, shift_scale);
       break;
     case ST_BF16S:
-      store_from_f64_kernel<ST_BF16S><<<g, BLOCK, 0, s>>>(n, q, src, (__hip_bfloat16*)dst, w, shift_scale);
+      store_from_f64_kernel<ST_BF16S><<<g, BLOCK, 0, s>>>(n, q, nx, xp, src, (__hip_bfloat16*)dst, w, shift_scale);
       break;
     default: return hipErrorInvalidValue;
   }

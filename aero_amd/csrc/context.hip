@@ -183,7 +183,9 @@ struct aero_ctx {
   int Q = 0;
   long long N = 0;
   int nrow = 0;
-  int nblocks = 0;
+  int nblocks = 0;  // blocks over the N cells (per-cell kernels)
+  int xp = 0;       // padded row length (storage rows and the step's thread grid)
+  int sblocks = 0;  // blocks over the nrow * xp step threads
   size_t esz = 8;  // storage element size
   size_t csz = 8;  // compute element size
   // state: ping-pong sides, `side` is the one the next step reads
@@ -225,6 +227,8 @@ struct aero_ctx {
   double timed_ms = 0.0;
   long long timed_launches = 0;
 
+  // bytes of one population buffer in the storage layout
+  size_t state_bytes() const { return esz * Q * (size_t)nrow * xp; }
   bool les_split() const { return d.les != AERO_LES_OFF && d.collision != AERO_MRT; }
   bool need_planes() const { return d.inlet == AERO_IN_FIELD || d.inlet == AERO_IN_SEM; }
   // Inlet planes of steps s0 .. s0+n-1 of a call into planes[b]: uploaded
@@ -253,6 +257,10 @@ struct aero_ctx {
     p.ny = d.ny;
     p.nz = d.nz;
     p.nrow = nrow;
+    p.xp = xp;
+    p.qxp = Q * xp;
+    p.div_xp = make_div_magic((unsigned)xp);
+    p.sblocks = sblocks;
     p.ibm_on = ibm_on;
     p.ibm_coeff = ibm_coeff;
     p.sponge_x0 = sponge_x0;
@@ -371,6 +379,8 @@ struct aero_ctx {
     cp.ny = d.ny;
     cp.nz = d.nz;
     cp.dim = Q == 19 ? 3 : 2;
+    cp.xp = xp;
+    cp.qxp = Q * xp;
     cp.ncell = N;
     cp.nblocks = nblocks;
     cp.F = F;
@@ -509,10 +519,10 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
       nat.clear();
       size_t zreuse = 0;  // planes z-1, z, z+1 of every population, on all XCDs at once
       for (int i : members[g]) {
-        for (int b = 0; b <= cs[i]->nblocks; ++b) nat.push_back(make_int2(i, b));
+        for (int b = 0; b <= cs[i]->sblocks; ++b) nat.push_back(make_int2(i, b));
         const aero_ctx* c = cs[i];
         if (c->d.nz > 1)
-          zreuse = std::max(zreuse, (size_t)XCDS * 3 * c->Q * (size_t)c->d.nx * c->d.ny * c->esz);
+          zreuse = std::max(zreuse, (size_t)XCDS * 3 * c->Q * (size_t)c->xp * c->d.ny * c->esz);
       }
       const int run = zreuse > ZREUSE_BYTES ? XCD_RUN_LARGE : ((int)nat.size() + XCDS - 1) / XCDS;
       xcd_order(nat, std::max(run, 1), tiles);
@@ -541,7 +551,7 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
   // most of MI355X's 256 MB last-level cache a stored line is evicted before
   // the next step reads it, so the stores bypass the caches.
   size_t ws = 0;
-  for (aero_ctx* c : cs) ws += 2 * (size_t)c->Q * (size_t)c->N * c->esz * (c->d.thermal ? 2 : 1);
+  for (aero_ctx* c : cs) ws += 2 * c->state_bytes() * (c->d.thermal ? 2 : 1);
   const bool nt = ws > NT_STORE_BYTES;
   bool any_hist = false;
   for (int i = 0; i < n; ++i) any_hist |= hists[i] != nullptr;
@@ -661,17 +671,19 @@ int aero_create(const aero_desc* desc, aero_ctx** out) {
     if (!(d.omega > 0.0)) throw std::invalid_argument("omega must be positive");
     const long long N = (long long)d.nx * d.ny * d.nz;
     if (N * d.q >= (1ll << 31)) throw std::invalid_argument("grid too large for 32-bit cell indexing");
-    // kernels address one direction plane with a 32-bit byte offset
-    if (N * 8 >= (1ll << 32)) throw std::invalid_argument("grid too large: one population plane must stay under 4 GiB");
+    const long long nthread = (long long)padded_row(d.nx) * d.ny * d.nz;
+    if (nthread >= (1ll << 31)) throw std::invalid_argument("grid too large for 32-bit step thread indexing");
     c->Q = d.q;
     c->N = N;
     c->nrow = d.nz * d.ny;
+    c->xp = padded_row(d.nx);
     c->nblocks = (int)((N + BLOCK - 1) / BLOCK);
+    c->sblocks = (int)((nthread + BLOCK - 1) / BLOCK);
     c->esz = storage_bytes(d.storage);
     c->csz = d.storage == ST_F64 ? 8 : 4;
     ck(hipGetDevice(&c->device), "hipGetDevice");
     ck(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
-    const size_t fbytes = c->esz * d.q * (size_t)N;
+    const size_t fbytes = c->state_bytes();
     c->B[0].alloc(fbytes);
     c->B[1].alloc(fbytes);
     if (d.thermal) {
@@ -831,10 +843,15 @@ int aero_batch_step(aero_batch* b, int32_t n, aero_ctx* const* ctxs, int32_t nst
 // scalar load that every cell it covers is plain fluid.
 static void upload_flags(aero_ctx* c, const std::vector<uint8_t>& fl) {
   ck(hipMemcpy(c->flags.p, fl.data(), (size_t)c->N, hipMemcpyHostToDevice), "copy flags");
-  const size_t nwave = ((size_t)c->N + 63) / 64;
+  // waves of the step's thread grid: cell (row, x) is step thread row * xp + x
+  const size_t nwave = ((size_t)c->nrow * c->xp + 63) / 64;
   std::vector<uint32_t> words((nwave + 31) / 32, 0u);
+  const int nx = c->d.nx;
   for (size_t k = 0; k < (size_t)c->N; ++k)
-    if (fl[k]) words[(k >> 6) >> 5] |= 1u << ((k >> 6) & 31);
+    if (fl[k]) {
+      const size_t w = ((k / nx) * c->xp + k % nx) >> 6;
+      words[w >> 5] |= 1u << (w & 31);
+    }
   c->wave_any.alloc(sizeof(uint32_t) * words.size());
   ck(hipMemcpy(c->wave_any.p, words.data(), sizeof(uint32_t) * words.size(), hipMemcpyHostToDevice),
      "copy wave flags");
@@ -1129,7 +1146,8 @@ int aero_set_g(aero_ctx* c, const double* g) {
     c->tmp.alloc(sizeof(double) * cnt);
     ck(hipMemcpy(c->tmp.p, inv.data(), sizeof(double) * cnt, hipMemcpyHostToDevice), "upload g");
     const double shift = (c->d.storage == ST_F16S || c->d.storage == ST_BF16S) ? c->d.T_ref : 0.0;
-    ck(launch_store_from_f64(c->d.storage, c->Q, c->N, c->tmp.as<double>(), c->G[c->side].p, shift, c->stream),
+    ck(launch_store_from_f64(c->d.storage, c->Q, c->N, c->d.nx, c->xp, c->tmp.as<double>(), c->G[c->side].p, shift,
+                             c->stream),
        "store g");
     ck(hipStreamSynchronize(c->stream), "sync");
     c->tmp.release();

@@ -51,8 +51,8 @@ hipError_t launch_materialize(int q, int st, bool thermal, const StepParams& p, 
 hipError_t launch_collide(int q, int coll, int st, const CollideParams& p, hipStream_t s);
 hipError_t launch_recycling_plane(int st, const StepParams& p, double* out, hipStream_t s);
 hipError_t launch_materialize_g(int q, int st, const StepParams& p, double* out, hipStream_t s);
-hipError_t launch_store_from_f64(int st, int q, long long n, const double* src, void* dst, double shift_scale,
-                                 hipStream_t s);
+hipError_t launch_store_from_f64(int st, int q, long long n, int nx, int xp, const double* src, void* dst,
+                                 double shift_scale, hipStream_t s);
 hipError_t launch_compute_from_f64(int st, long long count, const double* src, void* dst, hipStream_t s);
 hipError_t launch_ibm_field(int q, int st, long long n, void* F, const uint8_t* flags, double coeff, hipStream_t s);
 // STL geometry (mesh.hip).

@@ -220,9 +220,10 @@ __host__ __device__ __forceinline__ C fshift(int i, C u0) {
 // shifted value f - w_i*s (s = 1 for f, T_ref for the scalar g) so the 10/7
 // mantissa bits resolve the deviation from rest equilibrium, not its bulk.
 template <int ST> struct Storage;
-// ld/st address one direction plane with a 32-bit byte offset (the host
-// keeps every plane under 4 GiB), so the compiler forms scalar-base +
-// 32-bit-offset addresses: one VGPR per access instead of a 64-bit pair.
+// ld/st address one population run of a storage row with a 32-bit byte
+// offset from the run's base (an x offset, below the padded row length), so
+// the compiler forms scalar-base + 32-bit-offset addresses: one VGPR per
+// access instead of a 64-bit pair.
 #define AERO_PLANE_ACCESS                                                                    \
   static __device__ __forceinline__ C ld(const T* plane, unsigned k, C s) {                 \
     return load((const T*)((const char*)plane + k * (unsigned)sizeof(T)), 0, s);            \
@@ -309,13 +310,31 @@ constexpr int BLOCK = 256;
 #define AERO_GLOBAL
 #endif
 
+// Population storage (f and the scalar's g, in the storage type): grid row
+// r = z ny + y holds its cells' Q populations as Q runs of xp values, with
+// xp = nx rounded up to 64, population i of cell (x, y, z) at element
+// (r Q + i) xp + x.  The fused step runs one thread per padded cell
+// (nrow * xp threads), so a 64-lane wavefront always covers 64 aligned
+// cells of one row: its y and z, every neighbour row's base address and the
+// wall / inlet row tests are wave-uniform (scalar registers), and each lane
+// adds only its 32-bit x offset.  A wave touches Q contiguous runs of one
+// row instead of Q planes spread over the whole buffer, which keeps far
+// fewer HBM pages open at once than plane-major [q][z][y][x] storage.
+// Per-cell tables (flags, link records, output fields) stay compact,
+// indexed by cell = r nx + x.
+constexpr int ROW_ALIGN = 64;
+__host__ __device__ constexpr int padded_row(int nx) { return (nx + ROW_ALIGN - 1) / ROW_ALIGN * ROW_ALIGN; }
+
 // Everything one step launch needs; passed by value as the kernel argument.
 struct StepParams {
   int nx, ny, nz;
   int nrow;  // nz * ny
-  DivMagic div_nx, div_ny;  // cell -> (x, y, z) without integer division
+  int xp;    // padded row length of the population storage and of the step's thread grid
+  int qxp;   // Q * xp: elements of one storage row
+  DivMagic div_nx, div_ny, div_xp;  // cell -> (x, y, z) and step thread -> row without integer division
   long long ncell;
-  int nblocks;  // blocks covering ncell (the step launch adds one reduction block)
+  int nblocks;  // blocks covering ncell (per-cell kernels)
+  int sblocks;  // blocks covering nrow * xp step threads (the step launch adds one reduction block)
   int nt_store;  // population stores bypass the caches (launch working set beyond the last-level cache)
   int ibm_on;      // immersed-boundary mode: no link bounce-back; Guo forcing of FLAG_IBM cells
   int sponge_x0;   // first column with sponge sigma > 0 (nx when there is none)
@@ -327,7 +346,7 @@ struct StepParams {
   AERO_GLOBAL void* fsplit;  // split mode: post-BC populations, compute type, Q x N
   AERO_GLOBAL void* usplit;  // split mode: ux, uy, uz, compute type, 3 x N
   const AERO_GLOBAL uint8_t* flags;
-  const AERO_GLOBAL uint32_t* wave_any;  // bit w: some cell of [64 w, 64 w + 64) is flagged
+  const AERO_GLOBAL uint32_t* wave_any;  // bit w: some cell of step threads [64 w, 64 w + 64) is flagged
   const AERO_GLOBAL LinkCell* lrec;
   const AERO_GLOBAL int* rec_of;   // per cell: its link record (read only where FLAG_LINKS is set)
   const AERO_GLOBAL float* qdir;   // Q-1 wall fractions per link record

@@ -93,6 +93,14 @@ __device__ __forceinline__ C feq_i(C w, C rho, C eu, C usq) {
   return w * rho * (C(1) + C(3) * eu + C(4.5) * eu * eu - C(1.5) * usq);
 }
 
+// Storage row of grid row (y, z) in buffer B: its Q runs of xp values
+// (population i of the row's cell x at element i * xp + x).  Wave-uniform in
+// the fused step, where a wave covers one row.
+template <class T>
+__device__ __forceinline__ T* srow(const StepParams& p, T* B, int y, int z) {
+  return B + (size_t)((unsigned)z * (unsigned)p.ny + (unsigned)y) * (unsigned)p.qxp;
+}
+
 // Pull gather of all Q populations of cell (x, y, z) from buffer B: the
 // populations f (KIND 0, shifted by the free-stream equilibrium in the half
 // forms) or the scalar's g (KIND 1, shifted by w_i * T_ref).
@@ -102,7 +110,6 @@ __device__ __forceinline__ void gather(const StepParams& p, const typename Stora
                                        typename Storage<ST>::C fn[L::Q]) {
   using S = Storage<ST>;
   using C = typename S::C;
-  const size_t N = (size_t)p.ncell;
   const int xm = x == 0 ? p.nx - 1 : x - 1, xp = x == p.nx - 1 ? 0 : x + 1;
   const int ym = y == 0 ? p.ny - 1 : y - 1, yp = y == p.ny - 1 ? 0 : y + 1;
   int zm = z, zp = z;
@@ -115,8 +122,8 @@ __device__ __forceinline__ void gather(const StepParams& p, const typename Stora
     const int xs = L::EX[i] == 0 ? x : (L::EX[i] > 0 ? xm : xp);
     const int ys = L::EY[i] == 0 ? y : (L::EY[i] > 0 ? ym : yp);
     const int zs = L::EZ[i] == 0 ? z : (L::EZ[i] > 0 ? zm : zp);
-    const unsigned src = ((unsigned)zs * (unsigned)p.ny + (unsigned)ys) * (unsigned)p.nx + (unsigned)xs;
-    fn[i] = S::ld(B + (size_t)i * N, src, KIND == 0 ? (C)p.fsh[fsh_class<L>(i)] : C(L::W[i]) * shift_scale);
+    fn[i] = S::ld(srow(p, B, ys, zs) + (unsigned)(i * p.xp), (unsigned)xs,
+                  KIND == 0 ? (C)p.fsh[fsh_class<L>(i)] : C(L::W[i]) * shift_scale);
   }
 }
 
@@ -149,17 +156,17 @@ __device__ __forceinline__ int link_record(const StepParams& p, int cell) { retu
 // memory round trip instead of one per link.
 template <class L, int ST>
 __device__ __forceinline__ void apply_links_light(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
-                                                  int rec, int cell, int x, int y, int z,
-                                                  typename Storage<ST>::C fn[L::Q]) {
+                                                  int rec, int x, int y, int z, typename Storage<ST>::C fn[L::Q]) {
   using S = Storage<ST>;
   using C = typename S::C;
+  using T = typename S::T;
   constexpr int Q = L::Q;
-  const size_t N = (size_t)p.ncell;
   const uint32_t mask = p.lrec[rec].mask;
+  const T* own = srow(p, B, y, z);
   if (!p.bouzidi) {
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
-      const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, (C)p.fsh[fsh_class<L>(i)]);
+      const C fi = S::ld(own + (unsigned)(i * p.xp), (unsigned)x, (C)p.fsh[fsh_class<L>(i)]);
       fn[L::OPP[i]] = ((mask >> i) & 1u) ? fi : fn[L::OPP[i]];
     }
     return;
@@ -178,10 +185,10 @@ __device__ __forceinline__ void apply_links_light(const StepParams& p, const typ
     const int xn = clampi(x + L::EX[o], 0, p.nx - 1);
     const int yn = clampi(y + L::EY[o], 0, p.ny - 1);
     const int zn = clampi(z + L::EZ[o], 0, p.nz - 1);
-    const unsigned nb = ((unsigned)zn * p.ny + yn) * p.nx + xn;
-    const C fi = S::ld(B + (size_t)i * N, (unsigned)cell, (C)p.fsh[fsh_class<L>(i)]);
-    const C f_up = S::ld(B + (size_t)i * N, nb, (C)p.fsh[fsh_class<L>(i)]);          // f_pre[i](x - e_i)
-    const C f_op = S::ld(B + (size_t)o * N, (unsigned)cell, (C)p.fsh[fsh_class<L>(o)]);  // f_pre[OPP[i]](x)
+    const C fi = S::ld(own + (unsigned)(i * p.xp), (unsigned)x, (C)p.fsh[fsh_class<L>(i)]);
+    const C f_up = S::ld(srow(p, B, yn, zn) + (unsigned)(i * p.xp), (unsigned)xn,
+                         (C)p.fsh[fsh_class<L>(i)]);                                        // f_pre[i](x - e_i)
+    const C f_op = S::ld(own + (unsigned)(o * p.xp), (unsigned)x, (C)p.fsh[fsh_class<L>(o)]);  // f_pre[OPP[i]](x)
     const C partner = upstream ? f_up : f_op;
     const C a = lo ? C(2) * q : C(1) / (C(2) * q);
     const C v = a * fi + (C(1) - a) * partner;
@@ -201,15 +208,15 @@ __device__ __forceinline__ void apply_links_light(const StepParams& p, const typ
 // in registers across the step.  The reduction combines them per cell.
 template <class L, int ST>
 __device__ __forceinline__ void link_pre_terms(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
-                                               int rec, int cell) {
+                                               int rec, int x, int y, int z) {
   using St = Storage<ST>;
   using C = typename St::C;
   constexpr int Q = L::Q;
-  const size_t N = (size_t)p.ncell;
   const uint32_t mask = p.lrec[rec].mask;
+  const typename St::T* own = srow(p, B, y, z);
   C fp[Q];
 #pragma unroll
-  for (int i = 0; i < Q; ++i) fp[i] = St::ld(B + (size_t)i * N, (unsigned)cell, (C)p.fsh[fsh_class<L>(i)]);
+  for (int i = 0; i < Q; ++i) fp[i] = St::ld(own + (unsigned)(i * p.xp), (unsigned)x, (C)p.fsh[fsh_class<L>(i)]);
   double rho = 0, mx = 0, my = 0, mz = 0, fx = 0, fy = 0, fz = 0;
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
@@ -517,7 +524,7 @@ __device__ __forceinline__ int pre_outlet(const StepParams& p, const StepDyn& dy
   int rec = -1;
   if (flag & FLAG_LINKS) {
     rec = link_record(p, cell);
-    if (!p.ibm_on) apply_links_light<L, ST>(p, B, rec, cell, x, y, z, fn);  // IBM: solver.py:379-383
+    if (!p.ibm_on) apply_links_light<L, ST>(p, B, rec, x, y, z, fn);  // IBM: solver.py:379-383
   }
   if (x == 0) apply_inlet<L, C>(p, dyn, y, z, fn);
   return rec;
@@ -610,7 +617,6 @@ __device__ __forceinline__ void scalar_step(const StepParams& p, const StepDyn& 
   using S = Storage<ST>;
   using C = typename S::C;
   constexpr int Q = L::Q;
-  const size_t N = (size_t)p.ncell;
   C g[Q];
   gather<L, ST, 1>(p, (const typename S::T*)p.gin, x, y, z, (C)p.gshift, g);
   if (!dyn.g_bc_skip) {
@@ -639,9 +645,9 @@ __device__ __forceinline__ void scalar_step(const StepParams& p, const StepDyn& 
   }
   if (write_g) {
     const C om = (C)p.omega_T;
+    AERO_GLOBAL typename S::T* grow = srow(p, (AERO_GLOBAL typename S::T*)p.gout, y, z);
     auto put = [&](int i, C gp) {
-      S::st((typename S::T*)scalar_base((AERO_GLOBAL typename S::T*)p.gout + (size_t)i * N), (unsigned)cell, gp,
-            C(L::W[i]) * (C)p.gshift);
+      S::st((typename S::T*)(grow + (unsigned)(i * p.xp)), (unsigned)x, gp, C(L::W[i]) * (C)p.gshift);
     };
     // geq_i = w T (1 + 3 e_i.u): a pair (i, opp i) shares w T and +-3 w T e_i.u
     put(0, solid ? C(L::W[0]) * T : g[0] - om * (g[0] - C(L::W[0]) * T));
@@ -908,8 +914,10 @@ __device__ __forceinline__ void collide(const double* mrt_s, const double* minv_
 // Tail of a step for one cell: collide into fout (KMODE 0), or -- split step
 // for the LES finite-difference pass -- write the post-boundary populations
 // and velocity (KMODE 1).
+// `sb` is the storage element of the wave's first cell: its row's base plus
+// the wave's first x (wave-uniform), so a lane's cell sits at sb + lane.
 template <class L, int COLL, int ST, int KMODE>
-__device__ __forceinline__ void finish_cell(const StepParams& p, int cell, unsigned wcell, bool solid,
+__device__ __forceinline__ void finish_cell(const StepParams& p, int cell, size_t sb, bool solid,
                                             const typename Storage<ST>::C fn[L::Q]) {
   using S = Storage<ST>;
   using C = typename S::C;
@@ -919,19 +927,18 @@ __device__ __forceinline__ void finish_cell(const StepParams& p, int cell, unsig
   if constexpr (KMODE == 0) {
     C out[Q];
     collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, (C)p.omega, (C)p.trt_sm, fn, out);
-    // cell = wcell (the wave's first cell, scalar) + lane: the store offset is
-    // rebuilt here rather than held in a register across the whole step
+    // the store offset is the lane index, rebuilt here rather than held in a
+    // register across the whole step; the run bases are scalar
     const unsigned lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    AERO_GLOBAL T* wbase = (AERO_GLOBAL T*)p.fout + sb;
     if (p.nt_store) {
 #pragma unroll
       for (int i = 0; i < Q; ++i)
-        S::st_nt((T*)scalar_base((AERO_GLOBAL T*)p.fout + (size_t)i * N + wcell), lane, out[i],
-                 (C)p.fsh[fsh_class<L>(i)]);
+        S::st_nt((T*)scalar_base(wbase + (unsigned)(i * p.xp)), lane, out[i], (C)p.fsh[fsh_class<L>(i)]);
     } else {
 #pragma unroll
       for (int i = 0; i < Q; ++i)
-        S::st((T*)scalar_base((AERO_GLOBAL T*)p.fout + (size_t)i * N + wcell), lane, out[i],
-              (C)p.fsh[fsh_class<L>(i)]);
+        S::st((T*)scalar_base(wbase + (unsigned)(i * p.xp)), lane, out[i], (C)p.fsh[fsh_class<L>(i)]);
     }
   } else {
     C* F = (C*)p.fsplit;
@@ -1059,13 +1066,19 @@ struct StepWaves {
       }                                                                                                    \
       return;                                                                                              \
     }                                                                                                      \
-    /* cell indices fit in 31 bits (checked when the context is created) */                                \
-    const unsigned cu_ = (unsigned)(blk - 1) * (unsigned)BLOCK + threadIdx.x;                              \
-    const unsigned wcell_ = (unsigned)(blk - 1) * (unsigned)BLOCK + __builtin_amdgcn_readfirstlane(threadIdx.x);\
-    if (cu_ >= (unsigned)p.ncell) return;                                                                  \
-    const int cell = (int)cu_;                                                                             \
-    int x, y, z;                                                                                           \
-    cell_xyz(p, cu_, x, y, z);                                                                             \
+    /* one thread per padded cell; thread indices fit in 31 bits (checked when the context is */          \
+    /* created).  xp is a multiple of 64, so a wave covers 64 aligned cells of one row: t0_, row, y, */    \
+    /* z and the storage base sb_ of the wave's first cell are wave-uniform (scalar registers). */          \
+    const unsigned t0_ = (unsigned)(blk - 1) * (unsigned)BLOCK + __builtin_amdgcn_readfirstlane(threadIdx.x);\
+    const unsigned row_ = udiv(t0_, p.div_xp);                                                             \
+    if (row_ >= (unsigned)p.nrow) return;                                                                  \
+    const unsigned xw_ = t0_ - row_ * (unsigned)p.xp;                                                      \
+    const int x = (int)(xw_ + __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)));          \
+    if (x >= p.nx) return; /* row padding */                                                               \
+    const int z = (int)udiv(row_, p.div_ny);                                                               \
+    const int y = (int)row_ - z * p.ny;                                                                    \
+    const int cell = (int)(row_ * (unsigned)p.nx) + x;                                                     \
+    const size_t sb_ = (size_t)row_ * (unsigned)p.qxp + xw_;                                               \
     const T_* B = (const T_*)p.fin;                                                                        \
     /* Bulk waves: no cell of the wave flagged (solid, link, IBM; one bit per 64 cells, a scalar */        \
     /* load) and no lane at a domain edge or in the sponge -> pull, [body force,] collide, store, */      \
@@ -1076,14 +1089,14 @@ struct StepWaves {
     /* the flag load in unflagged waves, measured as fast (and faster on small grids), and MRT */         \
     /* kernels are register-bound. */                                                                     \
     if constexpr (!THERMAL && COLL != COLL_MRT && L::DIM == 2) {                                           \
-      const unsigned w_ = wcell_ >> 6;                                                                     \
+      const unsigned w_ = t0_ >> 6;                                                                        \
       const bool edge_ = x == 0 || x == p.nx - 1 || x >= p.sponge_x0 ||                                    \
                          (p.wall_mode != WALL_NONE && (y == 0 || y == p.ny - 1));                          \
       if (!((p.wave_any[w_ >> 5] >> (w_ & 31)) & 1u) && __builtin_amdgcn_ballot_w64(edge_) == 0) {        \
         C_ fq_[L::Q];                                                                                      \
         gather<L, ST>(p, B, x, y, z, (C_)p.u0, fq_);                                                       \
         if (L::DIM == 3 && p.body_force_on) body_force<L, C_>(p, fq_);                                     \
-        finish_cell<L, COLL, ST, KMODE>(p, cell, wcell_, false, fq_);                                      \
+        finish_cell<L, COLL, ST, KMODE>(p, cell, sb_, false, fq_);                                         \
         break;                                                                                             \
       }                                                                                                    \
     }                                                                                                      \
@@ -1091,13 +1104,13 @@ struct StepWaves {
     /* x = nx-2 source, the wave before) holds a flagged cell, and a convective outlet row's */            \
     /* previous state is fetched before the pull, so neither adds a dependent memory round trip. */       \
     const OutletRow<L, ST> orow_(p, x, y, z);                                                              \
-    const unsigned wv_ = wcell_ >> 6;                                                                      \
+    const unsigned wv_ = t0_ >> 6;                                                                         \
     const unsigned wp_ = wv_ > 0 ? wv_ - 1 : 0;                                                            \
     const bool wflag_ = (p.wave_any[wv_ >> 5] >> (wv_ & 31)) & 1u;                                         \
     const bool pflag_ = wflag_ || ((p.wave_any[wp_ >> 5] >> (wp_ & 31)) & 1u);                             \
     const uint8_t flag = wflag_ ? p.flags[cell] : (uint8_t)0;                                              \
     const bool solid = flag & FLAG_SOLID;                                                                  \
-    if (flag & FLAG_LINKS) link_pre_terms<L, ST>(p, B, link_record(p, cell), cell);                       \
+    if (flag & FLAG_LINKS) link_pre_terms<L, ST>(p, B, link_record(p, cell), x, y, z);                    \
     __builtin_amdgcn_sched_barrier(0);                                                                     \
     C_ fn[L::Q];                                                                                           \
     const int rec = post_bc<L, ST>(p, dyn, B, cell, x, y, z, flag, true, fn, pflag_, orow_.pv, orow_.coop,  \
@@ -1106,7 +1119,7 @@ struct StepWaves {
     if (rec >= 0) link_post_sum<L, C_>(p, rec, fn);                                                        \
     /* the next step's IBM forcing acts on this post-boundary state (solver.py:280-285) */                 \
     if (p.ibm_on && (flag & FLAG_IBM)) ibm_force<L, C_>((C_)p.ibm_coeff, fn);                             \
-    finish_cell<L, COLL, ST, KMODE>(p, cell, wcell_, solid, fn);                                           \
+    finish_cell<L, COLL, ST, KMODE>(p, cell, sb_, solid, fn);                                              \
   } while (0)
 
 // One case, parameters by value (split LES step, recycling inlet).
@@ -1153,7 +1166,7 @@ hipError_t launch_batch(const BatchCase* cases, const int2* tiles, int ntiles, i
 // are recorded by the packet processor at the kernel's start and end.
 template <class L, int COLL, int ST, bool THERMAL, int KMODE>
 hipError_t launch_step(const StepParams& p, hipStream_t s) {
-  const dim3 grid((unsigned)(p.nblocks + 1));
+  const dim3 grid((unsigned)(p.sblocks + 1));
   if (p.ev_start)
     hipExtLaunchKernelGGL(step_kernel<L, COLL, ST, THERMAL, KMODE>, grid, dim3(BLOCK), 0, s,
                           (hipEvent_t)p.ev_start, (hipEvent_t)p.ev_stop, 0, p);
@@ -1257,6 +1270,7 @@ __global__ __launch_bounds__(BLOCK) void materialize_kernel(StepParams p) {
 // model on U (les.py:273-344).
 struct CollideParams {
   int nx, ny, nz, dim;
+  int xp, qxp;  // storage row layout of fout (as StepParams)
   long long ncell;
   int nblocks;
   const void* F;
@@ -1375,9 +1389,11 @@ __global__ __launch_bounds__(BLOCK) void collide_kernel(CollideParams p) {
     }
   }
   collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, om, sm, fn, out);
-  T* Bo = (T*)p.fout;
+  const unsigned row = (unsigned)cell / (unsigned)p.nx;
+  T* Bo = (T*)p.fout + (size_t)row * (unsigned)p.qxp;
+  const unsigned x = (unsigned)cell - row * (unsigned)p.nx;
 #pragma unroll
-  for (int i = 0; i < Q; ++i) S::st(Bo + (size_t)i * N, (unsigned)cell, out[i], (C)p.fsh[fsh_class<L>(i)]);
+  for (int i = 0; i < Q; ++i) S::st(Bo + (unsigned)(i * p.xp), x, out[i], (C)p.fsh[fsh_class<L>(i)]);
 }
 
 // IBM forcing applied in place to a compute-precision field (Q x N), for
