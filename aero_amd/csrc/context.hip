@@ -126,6 +126,12 @@ struct StepRunner {
   // is never reused, so a context created at a freed context's address never
   // matches a stale table
   std::vector<unsigned long long> tile_key;
+  // kernel groups of that case list: lattice, collision, storage, scalar
+  // transport, and the group's slice of the tile table
+  struct Group {
+    int q, coll, st, th, toff, ntiles;
+  };
+  std::vector<Group> groups;
 
   template <class T>
   static void host_grow(T*& p, size_t n) {
@@ -170,6 +176,7 @@ struct StepRunner {
     tiles.release();
     ncase_cap = rows_cap = 0;
     tile_key.clear();
+    groups.clear();
   }
 };
 
@@ -491,28 +498,26 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
   int CH = STEP_CHUNK;
   for (aero_ctx* c : cs) CH = std::min(CH, c->chunk_steps());
   R.reserve(n, CH);
-  // groups and the packed tile list (rebuilt only when the case list changes)
-  struct Group {
-    int q, coll, st, th, toff, ntiles;
-  };
-  std::vector<Group> groups;
-  std::vector<std::vector<int>> members;
-  for (int i = 0; i < n; ++i) {
-    const aero_desc& d = cs[i]->d;
-    int g = 0;
-    for (; g < (int)groups.size(); ++g)
-      if (groups[g].q == d.q && groups[g].coll == d.collision && groups[g].st == d.storage &&
-          groups[g].th == (d.thermal != 0))
-        break;
-    if (g == (int)groups.size()) {
-      groups.push_back(Group{d.q, d.collision, d.storage, d.thermal != 0, 0, 0});
-      members.emplace_back();
-    }
-    members[g].push_back(i);
-  }
+  // groups and the packed tile list, rebuilt only when the case list changes
   std::vector<unsigned long long> key;
   for (aero_ctx* c : cs) key.push_back(c->serial);
-  {
+  if (key != R.tile_key) {
+    std::vector<StepRunner::Group>& groups = R.groups;
+    groups.clear();
+    std::vector<std::vector<int>> members;
+    for (int i = 0; i < n; ++i) {
+      const aero_desc& d = cs[i]->d;
+      int g = 0;
+      for (; g < (int)groups.size(); ++g)
+        if (groups[g].q == d.q && groups[g].coll == d.collision && groups[g].st == d.storage &&
+            groups[g].th == (d.thermal != 0))
+          break;
+      if (g == (int)groups.size()) {
+        groups.push_back(StepRunner::Group{d.q, d.collision, d.storage, d.thermal != 0, 0, 0});
+        members.emplace_back();
+      }
+      members[g].push_back(i);
+    }
     std::vector<int2> tiles, nat;
     for (size_t g = 0; g < groups.size(); ++g) {
       groups[g].toff = (int)tiles.size();
@@ -528,14 +533,13 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
       xcd_order(nat, std::max(run, 1), tiles);
       groups[g].ntiles = (int)tiles.size() - groups[g].toff;
     }
-    if (key != R.tile_key) {
-      R.tiles.alloc(sizeof(int2) * tiles.size());
-      ck(hipMemcpyAsync(R.tiles.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, stream),
-         "upload tiles");
-      ck(hipStreamSynchronize(stream), "tiles sync");
-      R.tile_key = key;
-    }
+    R.tiles.alloc(sizeof(int2) * tiles.size());
+    ck(hipMemcpyAsync(R.tiles.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, stream),
+       "upload tiles");
+    ck(hipStreamSynchronize(stream), "tiles sync");
+    R.tile_key = key;
   }
+  const std::vector<StepRunner::Group>& groups = R.groups;
   const bool timed = n == 1 && cs[0]->timing;
   int pend_off[2] = {-1, -1}, pend_n[2] = {0, 0};
   auto drain = [&](int b) {
@@ -581,7 +585,7 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
     for (int k = 0; k < nk; ++k) {
       for (aero_ctx* c : cs)
         if (c->d.inlet == AERO_IN_RECYCLING) c->recycling_prepass((c->side + k) & 1, stream);
-      for (const Group& g : groups)
+      for (const StepRunner::Group& g : groups)
         ck(launch_batch_step(g.q, g.coll, g.st, g.th, cases, R.tiles.as<int2>() + g.toff, g.ntiles, k, stream,
                              timed ? cs[0]->ev_pool[2 * k] : nullptr, timed ? cs[0]->ev_pool[2 * k + 1] : nullptr),
            "step launch");

@@ -1,21 +1,24 @@
 """Throughput benchmark of the MI355X lattice-Boltzmann step.
 
 Workload (one step = one lattice update of every cell, solid included):
-3D D3Q19 TRT sphere, 128 x 64 x 64 (Nx x Ny x Nz), radius 8 centred at
-(32, 32, 32), Bouzidi links, u0 = 0.05, Re = 100 (tau = 0.524), slip walls,
-convective outlet, equilibrium start plus seeded 1e-4 transverse velocity
-noise, float32 population storage.  At tau = 0.524 the reference's TRT rates
-are linearly unstable (s- = -18) and its interpolated bounce-back diverges
-within ~100 steps even in float64, so the timed run uses the standard
-magic-parameter TRT (Lambda = 3/16) and the linear Bouzidi rule with link
-fractions, labelled "trt_variant": "lambda", "bouzidi_variant": "linear".
+3D D3Q19 TRT sphere, 512 x 256 x 256 (Nx x Ny x Nz), radius 32 centred at
+(128, 128, 128), Bouzidi links, u0 = 0.05, Re = 100, slip walls, convective
+outlet, equilibrium start plus seeded 1e-4 transverse velocity noise, float32
+population storage: the configuration the FP32 roofline target is quoted on.
+At this tau the reference's TRT rates are linearly unstable (s- < 0 for tau
+below ~0.76) and its interpolated bounce-back diverges even in float64, so the
+timed run uses the standard magic-parameter TRT (Lambda = 3/16) and the linear
+Bouzidi rule with link fractions, labelled "trt_variant": "lambda",
+"bouzidi_variant": "linear".
 
 Prints ONE JSON line.  ``value`` is whole-job MLUPS over the timed steps
 (all launches included, populations resident in HBM).  ``roofline`` is the
 fused step kernel's algorithmic HBM rate (2 Q sizeof(float) bytes per cell)
 over its average HIP-event duration.  ``cpu_baseline`` times the float64 C
-oracle (OpenMP, all host cores) on a bounded number of full steps of the same
-workload.
+oracle (OpenMP) on a bounded sample of full steps of the same workload, with
+all host cores and with one.  ``latency`` gives the wall time of one
+``_step()`` call (how the reference's own tests drive the solver) and the
+host overhead of a multi-step call.
 """
 
 from __future__ import annotations
@@ -83,28 +86,76 @@ def cell_counts(solver):
     return int(tot.value), int(links.value)
 
 
-def cpu_baseline(case, nx, ny, nz, budget_s=15.0):
-    """Float64 C oracle (OpenMP) on the same workload, full reference steps."""
+def cpu_baseline(case, nx, ny, nz, budget_s=8.0):
+    """Float64 C oracle (OpenMP) on the same workload: a bounded sample of
+    full reference steps (collide, copy, stream, Bouzidi, inlet, outlet,
+    walls, forces) and of kernel-only steps (collide, copy, stream, timed
+    inside the C library), with all host cores and with one.  The per-step
+    rate of the sample is the figure; the run is not carried to the end."""
     from oracle import oracle as orc
     o = orc.OracleSolver(case["solid"], omega=case["omega"], u0=case["u0"], D=case["D"], phi=case["phi"],
                          bouzidi=True, collision="trt", trt_lambda=0.1875, trt_variant="lambda",
                          bouzidi_variant="linear")
     o.set_state(case["f0"])
-    o.step()  # warm-up
-    t0 = time.perf_counter()
-    n = 0
-    while time.perf_counter() - t0 < budget_s:
-        o.step()
-        n += 1
-    dt = time.perf_counter() - t0
     cells = nx * ny * nz
-    kt = orc.kernel_only_seconds(o, max(1, n // 2))
-    return {"value": round(cells * n / dt / 1e6, 3), "unit": "MLUPS", "cores": orc.lib().orc_num_threads(),
-            "kind": "port",
-            "sample": f"{n} full float64 steps (collide, copy, stream, Bouzidi, inlet, outlet, walls, forces) "
-                      f"of the {nx}x{ny}x{nz} D3Q19 TRT sphere; kernel-only (collide+copy+stream) "
-                      f"{cells * max(1, n // 2) / kt / 1e6:.3f} MLUPS",
+    cores = len(os.sched_getaffinity(0))
+    orc.set_threads(cores)
+
+    def full(budget, min_steps=1, warm=True):
+        if warm:
+            o.step()  # first touch of the step's arrays
+        t0 = time.perf_counter()
+        n = 0
+        while n < min_steps or time.perf_counter() - t0 < budget:
+            o.step()
+            n += 1
+        return n, time.perf_counter() - t0
+
+    n, dt = full(budget_s)
+    kn = max(2, n // 2)
+    kt = orc.kernel_only_seconds(o, kn)
+    orc.set_threads(1)
+    n1, dt1 = full(0.0, 1, warm=False)
+    kt1 = orc.kernel_only_seconds(o, 1)
+    orc.set_threads(cores)
+    return {"value": round(cells * n / dt / 1e6, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
+            "kernel_only_MLUPS": round(cells * kn / kt / 1e6, 3),
+            "one_core_MLUPS": round(cells * n1 / dt1 / 1e6, 3),
+            "one_core_kernel_only_MLUPS": round(cells / kt1 / 1e6, 3),
+            "sample": f"{n} full float64 steps of the {nx}x{ny}x{nz} D3Q19 TRT sphere on {cores} threads "
+                      f"(collide, copy, stream, Bouzidi, inlet, outlet, walls, forces; kernel-only = collide, "
+                      f"copy, stream over {kn} steps); one thread: {n1} full step, 1 kernel-only step",
             "cpu_model": _cpu_model()}
+
+
+def latency(solver, calls=100):
+    """Wall time of single ``_step()`` calls (the reference's drive pattern:
+    one call per step, coefficients returned to the host each time), and of
+    20-step calls beside the sum of their 20 kernel durations (HIP events):
+    the difference is the per-call host and launch overhead."""
+    import ctypes
+    lat = []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        solver._step()
+        lat.append((time.perf_counter() - t0) * 1e3)
+    walls20 = []
+    for _ in range(5):
+        solver._ctx.call("aero_synchronize")
+        t0 = time.perf_counter()
+        solver._advance(20)
+        walls20.append((time.perf_counter() - t0) * 1e3)
+    solver._ctx.call("aero_set_timing", ctypes.c_int32(1))
+    solver._advance(20)
+    ms = ctypes.c_double(0.0)
+    n = ctypes.c_int64(0)
+    solver._ctx.call("aero_get_timing", ctypes.byref(ms), ctypes.byref(n))
+    solver._ctx.call("aero_set_timing", ctypes.c_int32(0))
+    w20 = float(np.median(walls20))
+    return {"step_call_ms_median": round(float(np.median(lat)), 4),
+            "step_call_ms_p99": round(float(np.percentile(lat, 99)), 4),
+            "calls": calls, "steps20_call_ms_median": round(w20, 4),
+            "steps20_kernel_ms": round(ms.value, 4), "steps20_overhead_ms": round(w20 - ms.value, 4)}
 
 
 def _cpu_model():
@@ -130,6 +181,8 @@ def traffic_from_profiles(name):
 FORMS = {  # name: (nx, ny, nz, sphere radius, centre x, storage)
     "f32_128x64x64": (128, 64, 64, 8.0, 32.0, "f32"),
     "f16_128x64x64": (128, 64, 64, 8.0, 32.0, "f16"),
+    "bf16_128x64x64": (128, 64, 64, 8.0, 32.0, "bf16"),
+    "f64_128x64x64": (128, 64, 64, 8.0, 32.0, "f64"),
     "f32_512x256x256": (512, 256, 256, 32.0, 128.0, "f32"),
     "f16_512x256x256": (512, 256, 256, 32.0, 128.0, "f16"),
     "bf16_512x256x256": (512, 256, 256, 32.0, 128.0, "bf16"),
@@ -367,9 +420,9 @@ def main():
         print(json.dumps(run_form(a.form, a.steps, a.warmup)), flush=True)
         return
 
-    nx, ny, nz = 128, 64, 64
+    nx, ny, nz = 512, 256, 256
     cells = nx * ny * nz
-    s, case = make_case(nx, ny, nz, 8.0, 32.0, "f32")
+    s, case = make_case(nx, ny, nz, 32.0, 128.0, "f32")
     wall, kern_ms, rows = timed(s, a.steps, a.warmup)
     if not np.all(np.isfinite(rows[:, 0])):
         raise RuntimeError("non-finite drag in the timed run")
@@ -390,32 +443,30 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: analytic sphere, equilibrium + seeded 1e-4 noise",
-        "config": {"workload": "D3Q19 TRT sphere 128x64x64 r=8 Bouzidi, u0=0.05 Re=100, fp32 storage",
+        "config": {"workload": "D3Q19 TRT sphere 512x256x256 r=32 Bouzidi, u0=0.05 Re=100, fp32 storage",
                    "nx": nx, "ny": ny, "nz": nz, "collision": "trt", "trt_variant": "lambda",
                    "bouzidi_variant": "linear",
                    "trt_lambda": 0.1875, "storage": "f32", "links": int(len(s.surface_links))},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles("batch_step_kernel_d3q19_trt_f32_128x64x64"),
+                     "traffic": traffic_from_profiles("batch_step_kernel_d3q19_trt_f32_512x256x256"),
                      "kernel": "aero::batch_step_kernel<D3Q19, TRT, F32>", "kernel_ms": round(kern_ms, 5),
                      "algorithmic_bytes_per_launch": bytes_per_launch, "cells": ncell, "link_cells": nlink,
                      "bytes_per_cell": 2 * 19 * 4},
         "cd_last": float(rows[-1, 0]),
     }
     if not a.no_extra:
+        out["latency"] = latency(s)
+    s.close()
+    if not a.no_extra:
         extra = {}
-        for st in ("f16", "bf16", "f64"):
-            s2, _ = make_case(nx, ny, nz, 8.0, 32.0, st)
-            w2, k2, r2 = timed(s2, max(200, a.steps // 2), a.warmup)
-            n2 = max(200, a.steps // 2)
-            bpl = 2 * 19 * {"f16": 2, "bf16": 2, "f64": 8}[st] * cells
-            extra[st] = {"MLUPS": round(cells * n2 / w2 / 1e6, 2), "kernel_ms": round(k2, 5),
-                         "roofline_frac": round(bpl / (k2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                         "cd_last": float(r2[-1, 0])}
-            s2.close()
-        # the 512 x 256 x 256 sphere (radius 32) the throughput target is quoted on
-        for st in ("f32", "f16"):
-            extra[f"{st}_512x256x256"] = run_form(f"{st}_512x256x256", 40, 10)
+        # the mixed-precision forms of the same workload, then the 128 x 64 x 64 sphere (radius 8)
+        for name in ("f16_512x256x256", "bf16_512x256x256", "f32_128x64x64", "f16_128x64x64", "bf16_128x64x64",
+                     "f64_128x64x64"):
+            n2 = max(200, a.steps) if "128x" in name else max(100, a.steps)
+            extra[name] = run_form(name, n2, max(a.warmup, 20))
+        extra["f16_over_f32_MLUPS"] = round(extra["f16_512x256x256"]["MLUPS"] / out["value"], 3)
+        extra["f16_over_f32_kernel_time"] = round(kern_ms / extra["f16_512x256x256"]["kernel_ms"], 3)
         out["forms"] = extra
         out["stl_geometry"] = stl_geometry()
     if not a.no_serve:

@@ -696,3 +696,42 @@ int orc_num_threads(void) {
     return 1;
 #endif
 }
+
+/* Thread count of the following parallel loops (the CPU baseline's 1-core
+   and all-core figures). */
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    extern void omp_set_num_threads(int);
+    omp_set_num_threads(n > 0 ? n : 1);
+#endif
+    return 0;
+}
+
+/* The reference's kernel part of a step, `steps` times, timed inside the
+   library: collide f -> a (kernels.py:42-92 and its TRT / MRT variants),
+   the post-collision copy a -> b (solver.py:348, f_pre = f_post.copy()),
+   and the periodic push stream b -> f (kernels.py:94-115).  a and b are
+   caller-owned work arrays of f's size.  Returns the wall seconds in *sec. */
+int orc_kernel_only(int Q, int nz, int ny, int nx, double *f, double *a, double *b, const uint8_t *solid,
+                    const int *ex, const int *ey, const int *ez, const double *w, const int *opp, int coll,
+                    double omega, double trt_lambda, const double *M, const double *Minv, const double *s,
+                    int trt_variant, int steps, double *sec) {
+#ifdef _OPENMP
+    extern double omp_get_wtime(void);
+    const double t0 = omp_get_wtime();
+#endif
+    const long long n = (long long)Q * nz * ny * nx;
+    for (int k = 0; k < steps; ++k) {
+        orc_collide(Q, nz, ny, nx, f, a, solid, ex, ey, ez, w, opp, coll, omega, NULL, trt_lambda, M, Minv, s,
+                    trt_variant);
+#pragma omp parallel for schedule(static)
+        for (long long j = 0; j < n; ++j) b[j] = a[j];
+        orc_stream(Q, nz, ny, nx, b, f, ex, ey, ez);
+    }
+#ifdef _OPENMP
+    *sec = omp_get_wtime() - t0;
+#else
+    *sec = 0.0;
+#endif
+    return 0;
+}

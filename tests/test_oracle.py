@@ -274,3 +274,32 @@ def test_ibm_forcing_closed_forms():
             m1 = np.tensordot(E[:, a].astype(float), g, axes=(0, 0))
             F = (1.0 - 0.5 / tau) * (0.0 - m0 / r0)
             np.testing.assert_allclose((m1 - m0)[band], F[band] / 3.0, rtol=1e-12, atol=1e-16)
+
+
+def test_kernel_only_loop_matches_collide_copy_stream():
+    """orc_kernel_only (the CPU baseline's timed kernel leg) performs exactly
+    collide -> copy -> stream, as the composed step's first two stages do,
+    and leaves the solver's own state unchanged."""
+    from oracle import oracle as orc
+    rng = np.random.default_rng(3)
+    nz, ny, nx = 5, 6, 9
+    solid = np.zeros((nz, ny, nx), dtype=bool)
+    solid[2, 3, 4] = True
+    o = orc.OracleSolver(solid, omega=1.3, u0=0.04, D=2.0, collision="trt", trt_lambda=0.1875)
+    f0 = orc.feq(orc.E19, orc.W19, np.ones((nz, ny, nx)), np.full((nz, ny, nx), 0.04),
+                 rng.uniform(-1e-3, 1e-3, (nz, ny, nx)), rng.uniform(-1e-3, 1e-3, (nz, ny, nx)))
+    o.set_state(f0)
+    expect = o.stream(o.collide(o.stream(o.collide(o.f))))
+    Q, _, _, _ = o._dims()
+    f = o.f.copy()
+    a, b = np.empty_like(f), np.empty_like(f)
+    import ctypes
+    sec = ctypes.c_double(-1.0)
+    orc.lib().orc_kernel_only(Q, nz, ny, nx, orc._p(f), orc._p(a), orc._p(b), orc._p(o.solid_u8), orc._p(o.ex),
+                              orc._p(o.ey), orc._p(o.ez), orc._p(o.w), orc._p(o.opp), o.collision,
+                              ctypes.c_double(o.omega), ctypes.c_double(o.trt_lambda), orc._p(o.M), orc._p(o.Minv),
+                              orc._p(o.s), o.trt_variant, 2, ctypes.byref(sec))
+    np.testing.assert_array_equal(f, expect)
+    assert sec.value >= 0.0
+    np.testing.assert_array_equal(o.f, f0)
+    assert orc.kernel_only_seconds(o, 1) >= 0.0
