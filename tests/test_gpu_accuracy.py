@@ -10,9 +10,13 @@ and in float32 / shifted-half storage from the same seeded state:
   linear Bouzidi, u0 = 0.05, Re = 100; 5000 steps.
 
 Stated tolerances: mean Cd over the last 20% of the run within 0.5% of the
-FP64 run (FP32 and FP16-shifted storage; the BF16-shifted form, with an
-8-bit significand, within 2%); 2D Strouhal number within 0.5% when the FP64
-run sheds (a peak in the physical band 0.1-0.3).  MI355X only.
+FP64 run for FP32 and FP16-shifted storage, the forms this build claims.
+The BF16-shifted form (8-bit significand) is kept as an unclaimed option:
+its bound is 2%, and it measures about 1%.  2D Strouhal number within 0.5%
+when the FP64 run sheds (a peak in the physical band 0.1-0.3).  For the 3D
+sphere the final density and velocity fields are also compared with the
+FP64 run's (max and RMS, relative to the field's largest magnitude).
+MI355X only.
 """
 
 import numpy as np
@@ -21,6 +25,11 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 CD_TOL = {"f32": 5e-3, "f16": 5e-3, "bf16": 2e-2}  # |Cd - Cd_f64| / Cd_f64, mean over the last 20%
+# final rho / u fields of the 3D sphere vs FP64: (max, rms) relative to the
+# largest |rho| and |u| of the FP64 field
+# (measured on MI355X, max / rms over rho, ux, uy, uz: FP32 9.7e-5 / 3.9e-5,
+# FP16 2.9e-3 / 7.9e-5, BF16 8.2e-2 / 8.4e-3)
+C2_FIELD_TOL = {"f32": (3e-4, 1.2e-4), "f16": (9e-3, 2.5e-4), "bf16": (2.5e-1, 2.5e-2)}
 
 
 def _c1(storage, steps=20000):
@@ -62,7 +71,16 @@ def _c2(storage, steps=5000):
     s._f_outlet_prev = s.f[..., -1].copy()
     res = s.run(steps=steps, check_every=1000, verbose=False)
     s.close()
-    return res["Cd_mean"]
+    return res["Cd_mean"], {"rho": res["rho"], "ux": res["ux"], "uy": res["uy"], "uz": res["uz"]}
+
+
+_C2_F64 = {}
+
+
+def _c2_f64():
+    if not _C2_F64:
+        _C2_F64["run"] = _c2("f64")
+    return _C2_F64["run"]
 
 
 def test_c1_fp32_cd_and_strouhal_within_half_percent_of_fp64():
@@ -76,9 +94,17 @@ def test_c1_fp32_cd_and_strouhal_within_half_percent_of_fp64():
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
-def test_c2_reduced_storage_cd_within_half_percent_of_fp64(storage):
-    cd64 = _c2("f64")
-    cd = _c2(storage)
+def test_c2_reduced_storage_cd_and_fields_against_fp64(storage):
+    cd64, f64 = _c2_f64()
+    cd, fields = _c2(storage)
     print(f"[gpu] C2 Cd f64 {cd64:.5f} {storage} {cd:.5f} ({(cd - cd64) / cd64:+.2e})", flush=True)
     assert 0.85 < cd64 < 1.25  # the reference's sphere band at Re = 100 (aero/benchmarks.py:88-89)
     assert abs(cd - cd64) / abs(cd64) < CD_TOL[storage]
+    u_scale = max(float(np.max(np.abs(f64[k]))) for k in ("ux", "uy", "uz"))
+    tmax, trms = C2_FIELD_TOL[storage]
+    for k, a in fields.items():
+        d = a - f64[k]
+        scale = u_scale if k != "rho" else float(np.max(np.abs(f64["rho"])))
+        em, er = float(np.max(np.abs(d)) / scale), float(np.sqrt(np.mean(d * d)) / scale)
+        print(f"[gpu] C2 {storage} {k}: max {em:.2e} rms {er:.2e}", flush=True)
+        assert em < tmax and er < trms, (storage, k, em, er)
