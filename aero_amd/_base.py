@@ -222,10 +222,27 @@ class DeviceSolver:
         sh = self._ref_shape()[1:]
         return rho.reshape(sh), ux.reshape(sh), uy.reshape(sh), uz.reshape(sh)
 
-    def _device_check(self, step: int) -> None:
+    def _device_stats(self) -> np.ndarray:
+        """check_stability's statistics of the current state, reduced on the
+        device (3D: the z mid-plane, as solver3d.py:608-615)."""
         stats = np.empty(_abi.NCHECK)
         self._ctx.call("aero_check", _abi.ptr(stats))
-        check_stability_stats(stats, step)
+        return stats
+
+    def _device_check(self, step: int) -> None:
+        check_stability_stats(self._device_stats(), step)
+
+    def lift_history(self, window: int = 4096) -> list:
+        """The lift history the auto-stop and Strouhal tests read: Cl in 2D;
+        in 3D Cly, or Clz when its standard deviation over the last `window`
+        steps is larger (aero/lbm/solver3d.py:634-638)."""
+        if not self._is3d:
+            return self.Cl_history
+        w = window
+        if np.std(self.Clz_history[-min(len(self.Clz_history), w):]) > np.std(
+                self.Cly_history[-min(len(self.Cly_history), w):]):
+            return self.Clz_history
+        return self.Cly_history
 
     def _link_forces(self):
         n = ctypes.c_int32(0)

@@ -37,25 +37,35 @@ class StrouhalReport:
     relative_drift: Optional[float] = None
 
 
-def check_stability_stats(stats: np.ndarray, step: int) -> None:
-    """check_stability (aero/diagnostics.py:32-71) from device statistics."""
+def stability_findings(stats: np.ndarray, step: int):
+    """The outcome of check_stability (aero/diagnostics.py:32-71) for device
+    statistics, without acting on it: (error message or None, warning
+    messages).  The error is the one the reference raises first."""
     nan_any, nonpos_any, umax, rho_sum, count = (float(v) for v in stats)
     if nan_any > 0.0:
-        raise RuntimeError(
-            f"[step {step}] NaN detected in f — simulation has diverged. "
-            "Check tau (must be > 0.5), inlet velocity, and grid resolution.")
+        return (f"[step {step}] NaN detected in f — simulation has diverged. "
+                "Check tau (must be > 0.5), inlet velocity, and grid resolution."), []
     if nonpos_any > 0.0:
-        raise RuntimeError(f"[step {step}] Non-positive density detected — simulation has diverged.")
+        return f"[step {step}] Non-positive density detected — simulation has diverged.", []
+    warns = []
     if umax > 0.3:
-        warnings.warn(
-            f"[step {step}] Max velocity = {umax:.4f} lattice units (Ma ≈ {umax * (3 ** 0.5):.3f}). "
-            "Compressibility artifacts likely. Reduce u0 or increase grid size.",
-            RuntimeWarning, stacklevel=2)
+        warns.append(f"[step {step}] Max velocity = {umax:.4f} lattice units (Ma ≈ {umax * (3 ** 0.5):.3f}). "
+                     "Compressibility artifacts likely. Reduce u0 or increase grid size.")
     rho_mean = rho_sum / max(count, 1.0)
     if abs(rho_mean - 1.0) > 0.05:
-        warnings.warn(
-            f"[step {step}] Mean density drifted to {rho_mean:.4f} (>5% from 1.0). "
-            "Check outlet BC or total mass conservation.", RuntimeWarning, stacklevel=2)
+        warns.append(f"[step {step}] Mean density drifted to {rho_mean:.4f} (>5% from 1.0). "
+                     "Check outlet BC or total mass conservation.")
+    return None, warns
+
+
+def check_stability_stats(stats: np.ndarray, step: int) -> None:
+    """check_stability (aero/diagnostics.py:32-71) from device statistics:
+    RuntimeError on divergence, RuntimeWarning for the soft limits."""
+    err, warns = stability_findings(stats, step)
+    if err is not None:
+        raise RuntimeError(err)
+    for w in warns:
+        warnings.warn(w, RuntimeWarning, stacklevel=2)
 
 
 def check_stability(f, rho, ux, uy, step: int) -> None:

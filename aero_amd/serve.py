@@ -7,11 +7,15 @@ advances all live cases together, ``chunk`` steps at a time, through
 ``Batch.advance``: one kernel launch per step for each group of cases that
 share a kernel.  Between chunks it
 
+* runs the reference's stability check on every live case
+  (check_stability, aero/diagnostics.py:32-71, on device statistics): a case
+  it would stop with an error is retired as diverged, and its warnings are
+  counted;
 * retires every case that met the reference's auto-stop criterion
-  (``detect_statistical_stationarity``, aero/diagnostics.py:197-228, checked
-  every ``chunk`` = check_every steps as in the reference run loop,
-  aero/lbm/solver.py:517-570), reached its step limit, or diverged (a
-  non-finite drag coefficient);
+  (``detect_statistical_stationarity``, aero/diagnostics.py:197-228, on the
+  drag and the lift history the reference picks, aero/lbm/solver3d.py:634-638,
+  checked every ``chunk`` = check_every steps as in the reference run loop,
+  aero/lbm/solver.py:517-570) or reached its step limit;
 * admits waiting requests whose arrival time has passed while the live set
   stays under ``max_live`` cases and the device keeps ``mem_reserve`` bytes
   free after the new case's estimated footprint; admitted cases are built
@@ -34,7 +38,7 @@ from typing import Callable, List, Optional, Sequence
 
 import numpy as np
 
-from .diagnostics import detect_statistical_stationarity
+from .diagnostics import detect_statistical_stationarity, stability_findings
 
 
 # ------------------------------------------------------------------ cases
@@ -204,6 +208,7 @@ class Result:
     Cd_mean: float
     Cl_std: float
     strouhal: Optional[float]
+    warnings: int = 0  # stability checks that warned (Ma, density drift)
     Cd_history: list = field(repr=False, default_factory=list)
 
     @property
@@ -218,6 +223,8 @@ class _Live:
     solver: object
     arrival: float
     admitted: float
+    warnings: int = 0
+    error: Optional[str] = None  # set when check_stability would have raised
 
 
 class Server:
@@ -256,10 +263,10 @@ class Server:
 
     def _done(self, lv: _Live) -> Optional[str]:
         s = lv.solver
-        if not np.isfinite(s.Cd_history[-1]):
+        if lv.error is not None or not np.isfinite(s.Cd_history[-1]):
             return "diverged"
-        _, _, stop = detect_statistical_stationarity(s.Cd_history, s.Cl_history if hasattr(s, "Cl_history")
-                                                     else s.Cly_history, s.D, s.u0, **self.stop_kw)
+        _, _, stop = detect_statistical_stationarity(s.Cd_history, s.lift_history(self.stop_kw["strouhal_window"]),
+                                                     s.D, s.u0, **self.stop_kw)
         if stop:
             return "converged"
         if len(s.Cd_history) >= lv.spec.max_steps:
@@ -269,13 +276,21 @@ class Server:
     def _result(self, lv: _Live, reason: str, now: float, upto: int) -> Result:
         s = lv.solver
         cd = np.asarray(s.Cd_history[:upto])
-        cl = np.asarray((s.Cl_history if hasattr(s, "Cl_history") else s.Cly_history)[:upto])
+        cl = np.asarray(s.lift_history(self.stop_kw["strouhal_window"])[:upto])
         w = max(1, len(cd) // 5)
         from .diagnostics import compute_strouhal
         st = compute_strouhal(cl.tolist(), s.D, s.u0) if reason != "diverged" else None
         return Result(lv.rid, lv.spec, lv.arrival, lv.admitted, now, upto, reason,
-                      float(np.mean(cd[-w:])), float(np.std(cl[-w:])), st,
+                      float(np.mean(cd[-w:])), float(np.std(cl[-w:])), st, lv.warnings,
                       cd.tolist() if self.keep_history else [])
+
+    def _stability(self, lv: _Live) -> None:
+        """check_stability on the case's state at the end of the last chunk,
+        as the reference's run loop does every check_every steps."""
+        err, warns = stability_findings(lv.solver._device_stats(), len(lv.solver.Cd_history))
+        lv.warnings += len(warns) > 0
+        if err is not None:
+            lv.error = err
 
     def serve(self, requests: Sequence[tuple], clock: Callable[[], float] = time.perf_counter,
               sleep: Callable[[float], None] = time.sleep) -> List[Result]:
@@ -363,6 +378,14 @@ class Server:
                         lv.solver._record(r)
                         self.steps_done += n * lv.spec.cells
                 live = [lv for lv in live if id(lv) not in gone]
+                # the stability check at this chunk boundary; a case it stops
+                # is retired now, with its history through this chunk
+                for lv in list(live):
+                    self._stability(lv)
+                    if lv.error is not None:
+                        done.append(self._result(lv, "diverged", clock() - t0, len(lv.solver.Cd_history)))
+                        lv.solver.close()
+                        live.remove(lv)
         finally:
             builders.shutdown(wait=True)
             pool.shutdown(wait=True)
@@ -370,14 +393,19 @@ class Server:
 
 
 def summarize(results: Sequence[Result], wall: float, updates: int, stepping_seconds: Optional[float] = None) -> dict:
-    """p50 / p99 time-to-result, solves per hour and aggregate MLUPS (over
-    the whole wall time, and over the time spent inside batched stepping)."""
-    lat = np.array([r.latency for r in results]) if results else np.zeros(1)
+    """Solves per hour and p50 / p99 time-to-result over the requests that
+    finished without diverging (converged or at their step limit); diverged
+    requests and requests whose stability checks warned are counted apart.
+    Aggregate MLUPS over the whole wall time and over the time spent inside
+    batched stepping count every lattice update, diverged cases' included."""
+    solved = [r for r in results if r.stop_reason != "diverged"]
+    lat = np.array([r.latency for r in solved]) if solved else np.zeros(1)
     reasons = {}
     for r in results:
         reasons[r.stop_reason] = reasons.get(r.stop_reason, 0) + 1
-    return {"requests": len(results), "wall_s": round(wall, 3),
-            "solves_per_hour": round(len(results) / wall * 3600.0, 1) if wall > 0 else None,
+    return {"requests": len(results), "solves": len(solved), "diverged": len(results) - len(solved),
+            "warned": sum(1 for r in results if r.warnings), "wall_s": round(wall, 3),
+            "solves_per_hour": round(len(solved) / wall * 3600.0, 1) if wall > 0 else None,
             "p50_s": round(float(np.percentile(lat, 50)), 3), "p99_s": round(float(np.percentile(lat, 99)), 3),
             "MLUPS": round(updates / wall / 1e6, 1) if wall > 0 else None,
             "stepping_MLUPS": round(updates / stepping_seconds / 1e6, 1) if stepping_seconds else None,

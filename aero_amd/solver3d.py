@@ -258,12 +258,9 @@ class Solver3D(DeviceSolver):
                 callback(step, self.Cd_history[-1], self.Cly_history[-1], self.Clz_history[-1])
 
         def stop_fn():
-            lift = self.Cly_history
-            if np.std(self.Clz_history[-min(len(self.Clz_history), strouhal_window):]) > np.std(
-                    self.Cly_history[-min(len(self.Cly_history), strouhal_window):]):
-                lift = self.Clz_history
             return detect_statistical_stationarity(
-                self.Cd_history, lift, self.D, self.u0, convergence_window=convergence_window,
+                self.Cd_history, self.lift_history(strouhal_window), self.D, self.u0,
+                convergence_window=convergence_window,
                 convergence_tol=convergence_tol, strouhal_window=strouhal_window, strouhal_tol=strouhal_tol)
 
         stop_reason, last_conv, last_st = run_loop(

@@ -313,20 +313,25 @@ def copy_calibration(mib=1024, reps=4):
 
 def _sequential(cases, chunk=500):
     """Each case alone through the plain single-case API, one after another:
-    Solver.run with the reference's auto-stop checked every `chunk` steps."""
+    Solver.run with the reference's stability check and auto-stop every
+    `chunk` steps.  Returns (stop reason, steps, Cd_mean, warned) per case."""
+    import warnings
     out = []
     t0 = time.perf_counter()
     updates = 0
     for c in cases:
         s = c.build()
-        try:
-            r = s.run(steps=c.max_steps, check_every=chunk, verbose=False, auto_stop=True)
-            reason, cd = r["stop_reason"], np.asarray(r["Cd_history"])
-        except RuntimeError:
-            reason, cd = "diverged", np.asarray(s.Cd_history)
+        with warnings.catch_warnings(record=True) as caught:
+            warnings.simplefilter("always")
+            try:
+                r = s.run(steps=c.max_steps, check_every=chunk, verbose=False, auto_stop=True)
+                reason, cd = r["stop_reason"], np.asarray(r["Cd_history"])
+            except RuntimeError:
+                reason, cd = "diverged", np.asarray(s.Cd_history)
+        warned = any(issubclass(w.category, RuntimeWarning) for w in caught)
         updates += s.step_count * c.cells
         w = max(1, len(cd) // 5)
-        out.append((reason, len(cd), float(np.mean(cd[-w:])) if len(cd) else float("nan")))
+        out.append((reason, len(cd), float(np.mean(cd[-w:])) if len(cd) else float("nan"), warned))
         s.close()
     return out, time.perf_counter() - t0, updates
 
@@ -363,8 +368,10 @@ def serving(full=False):
     reasons = {}
     for r in seq:
         reasons[r[0]] = reasons.get(r[0], 0) + 1
-    out["sequential_2d"] = {"requests": len(seq), "wall_s": round(wall_s, 3),
-                            "solves_per_hour": round(len(seq) / wall_s * 3600.0, 1),
+    solved = sum(1 for r in seq if r[0] != "diverged")
+    out["sequential_2d"] = {"requests": len(seq), "solves": solved, "diverged": len(seq) - solved,
+                            "warned": sum(1 for r in seq if r[3]), "wall_s": round(wall_s, 3),
+                            "solves_per_hour": round(solved / wall_s * 3600.0, 1),
                             "MLUPS": round(upd_s / wall_s / 1e6, 1), "stop_reasons": reasons}
     # batched and sequential runs stop at the same step with the same drag
     by = {r.rid: r for r in res}

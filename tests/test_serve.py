@@ -71,6 +71,15 @@ class _FakeSolver:
         self.Cd_history.extend(rows[:, 0].tolist())
         self.Cl_history.extend(rows[:, 1].tolist())
 
+    def lift_history(self, window=4096):
+        return self.Cl_history
+
+    def _device_stats(self):
+        # nan flag, non-positive density flag, max |u|, sum and count of density
+        if self.diverge:
+            return np.array([1.0, 0.0, 0.0, 1.0, 1.0])
+        return np.array([0.0, 0.0, 0.4 if self.stop_at == 7 else 0.1, 1.0, 1.0])
+
     def close(self):
         self.closed = True
 
@@ -119,6 +128,27 @@ def test_server_admits_retires_and_times_requests(monkeypatch):
     assert by[2].admitted >= 0.005
     s = serve.summarize(res, wall=clock[0], updates=srv.steps_done)
     assert s["requests"] == 5 and s["p99_s"] >= s["p50_s"]
+    # the diverged request is not a solve
+    assert s["solves"] == 4 and s["diverged"] == 1 and s["warned"] == 0
+    assert s["solves_per_hour"] == pytest.approx(4 / clock[0] * 3600.0, rel=1e-3)
+
+
+def test_server_counts_stability_warnings_and_stops_on_errors():
+    """check_stability at every chunk boundary: a warning (Ma above 0.3 here)
+    is counted on the result; the error the reference raises retires the
+    case as diverged with its history through that chunk."""
+    clock = [0.0]
+    batch = _FakeBatch(clock)
+    srv = serve.Server(chunk=100, max_live=4, mem_reserve=0, batch=batch, free_memory=lambda: 1 << 40)
+    reqs = [(0.0, _FakeSpec(seed=7, max_steps=300)), (0.0, _FakeSpec(seed=0, max_steps=300)),
+            (0.0, _FakeSpec(seed=0, max_steps=300, Re=-1.0))]
+    res = srv.serve(reqs, clock=lambda: clock[0], sleep=lambda dt: clock.__setitem__(0, clock[0] + dt))
+    by = {r.rid: r for r in res}
+    assert by[0].warnings == 3 and by[0].stop_reason == "max_steps"
+    assert by[1].warnings == 0
+    assert by[2].stop_reason == "diverged" and by[2].steps == 100
+    s = serve.summarize(res, wall=clock[0], updates=srv.steps_done)
+    assert (s["solves"], s["diverged"], s["warned"]) == (2, 1, 1)
 
 
 class _CountedSpec(_FakeSpec):
