@@ -66,6 +66,7 @@ _SIGS = {
     "aero_set_outlet_prev": ([_P, _P], ctypes.c_int),
     "aero_get_outlet_prev": ([_P, _P], ctypes.c_int),
     "aero_step": ([_P, ctypes.c_int32, ctypes.c_int64, _P, _P], ctypes.c_int),
+    "aero_step_noalloc": ([_P, ctypes.c_int32, ctypes.c_int64, _P], ctypes.c_int),
     "aero_macroscopic": ([_P, _P, _P, _P, _P], ctypes.c_int),
     "aero_check": ([_P, _P], ctypes.c_int),
     "aero_link_forces": ([_P, _P, _P, _P], ctypes.c_int),

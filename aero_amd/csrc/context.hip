@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1286,6 +1287,137 @@ int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, 
     ck(hipStreamSynchronize(c->stream), "step sync");
   });
 }
+
+// ------------------------------------------------ allocation-free stepping
+// aero_step_noalloc keeps every buffer the fused step needs beyond the
+// context's own state -- the batched kernel's case table, its tile list, the
+// chunk's coefficient rows and the last step's reduction record -- in this
+// file-scope device array instead of buffers the library allocates.  Each
+// device holds its own copy for the life of the process.  In host code the
+// array's name is not a device address: it is reached only through the
+// per-device address hipGetSymbolAddress returns (noalloc_scratch()).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)4 << 20;
+static __device__ __align__(256) unsigned char g_noalloc_scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of the scratch array on `device` (current), looked up once
+// per device.
+void* noalloc_scratch(int device) {
+  if (device < 0 || device >= MAX_DEVICES) throw std::runtime_error("aero_step_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    ck(hipGetSymbolAddress(&ptr, HIP_SYMBOL(g_noalloc_scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr) throw std::runtime_error("hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
+
+// Scratch layout: case tables for the two step parities' chunk, reduction
+// record, coefficient rows of one chunk, then the tile list.
+struct NoallocLayout {
+  size_t cases, red, rows, tiles, end;
+  explicit NoallocLayout(int ntiles) {
+    auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+    cases = 0;
+    red = up(cases + sizeof(BatchCase));
+    rows = up(red + sizeof(ForceReduce));
+    tiles = up(rows + sizeof(double) * NCOEF * STEP_CHUNK);
+    end = tiles + sizeof(int2) * (size_t)ntiles;
+  }
+};
+
+}  // namespace
+
+// The tile list of one case (case index 0, blocks 0..n-1) in the XCD order
+// xcd_order() forms on the host for `run`-tile runs.
+__global__ __launch_bounds__(BLOCK) void noalloc_tiles_kernel(int2* tiles, int n, int run) {
+  const int b = blockIdx.x * BLOCK + threadIdx.x;
+  if (b >= n) return;
+  const int round = XCDS * run;
+  const int r0 = b / round * round;
+  const int j = b - r0;
+  const int m = min(round, n - r0);
+  const int x = j % XCDS;
+  int off = 0;
+  for (int xx = 0; xx < x; ++xx) off += m > xx ? (m - 1 - xx) / XCDS + 1 : 0;
+  tiles[b] = make_int2(0, r0 + off + j / XCDS);
+}
+
+__global__ __launch_bounds__(BLOCK) void noalloc_zero_kernel(double* p, int n) {
+  const int k = blockIdx.x * BLOCK + threadIdx.x;
+  if (k < n) p[k] = 0.0;
+}
+
+extern "C" {
+
+int aero_step_noalloc(aero_ctx* c, int32_t nsteps, int64_t step0, double* hist) {
+  return guarded([&] {
+    if (nsteps <= 0) return;
+    if (!c->has_state) throw std::runtime_error("no state: call aero_set_f first");
+    if (c->les_split()) throw std::invalid_argument("aero_step_noalloc: LES contexts step with aero_step");
+    if (c->need_planes())
+      throw std::invalid_argument("aero_step_noalloc: inlets with per-step planes step with aero_step");
+    c->set_device();
+    const int ntiles = c->sblocks + 1;
+    const NoallocLayout L(ntiles);
+    if (L.end > NOALLOC_SCRATCH_BYTES)
+      throw std::invalid_argument("aero_step_noalloc: grid too large for the " +
+                                  std::to_string(NOALLOC_SCRATCH_BYTES >> 20) + " MiB scratch (" +
+                                  std::to_string(ntiles) + " tiles; at most " +
+                                  std::to_string((NOALLOC_SCRATCH_BYTES - L.tiles) / sizeof(int2)) + ")");
+    unsigned char* base = (unsigned char*)noalloc_scratch(c->device);
+    BatchCase* d_cases = (BatchCase*)(base + L.cases);
+    ForceReduce* d_red = (ForceReduce*)(base + L.red);
+    double* d_rows = (double*)(base + L.rows);
+    int2* d_tiles = (int2*)(base + L.tiles);
+    hipStream_t st = c->stream;
+    // tiles as the allocating path orders them
+    const size_t zreuse = c->d.nz > 1 ? (size_t)XCDS * 3 * c->Q * (size_t)c->xp * c->d.ny * c->esz : 0;
+    const int run = zreuse > ZREUSE_BYTES ? XCD_RUN_LARGE : std::max(1, (ntiles + XCDS - 1) / XCDS);
+    noalloc_tiles_kernel<<<(ntiles + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(d_tiles, ntiles, run);
+    ck(hipGetLastError(), "noalloc tiles");
+    const bool nt = 2 * c->state_bytes() * (c->d.thermal ? 2 : 1) > NT_STORE_BYTES;
+    for (int s0 = 0; s0 < nsteps; s0 += STEP_CHUNK) {
+      const int nk = std::min(STEP_CHUNK, nsteps - s0);
+      noalloc_zero_kernel<<<(NCOEF * STEP_CHUNK + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(d_rows, NCOEF * STEP_CHUNK);
+      ck(hipGetLastError(), "noalloc zero rows");
+      BatchCase bc = c->make_case(step0 + s0, nullptr, d_rows);
+      bc.p[0].nt_store = bc.p[1].nt_store = nt ? 1 : 0;
+      ck(hipMemcpyAsync(d_cases, &bc, sizeof(BatchCase), hipMemcpyHostToDevice, st), "noalloc case table");
+      for (int k = 0; k < nk; ++k) {
+        if (c->d.inlet == AERO_IN_RECYCLING) c->recycling_prepass((c->side + k) & 1, st);
+        ck(launch_batch_step(c->Q, c->d.collision, c->d.storage, c->d.thermal != 0, d_cases, d_tiles, ntiles, k, st,
+                             nullptr, nullptr),
+           "noalloc step launch");
+      }
+      if (c->nrec > 0) {
+        const ForceReduce r = make_force_reduce(c->Q, c->fpart[(c->fpart_side + nk) & 1].as<double>(),
+                                                c->lrec.as<LinkCell>(), c->nrec, c->d.nx, c->d.ny, c->d.center[0],
+                                                c->d.center[1], c->d.center[2], c->force_denom(), c->moment_denom(),
+                                                d_rows + (size_t)NCOEF * (nk - 1));
+        ck(hipMemcpyAsync(d_red, &r, sizeof(ForceReduce), hipMemcpyHostToDevice, st), "noalloc reduction");
+        ck(launch_reduce_forces_many(c->Q, d_red, 1, st), "noalloc force reduction");
+      }
+      if (hist)
+        ck(hipMemcpyAsync(hist + (size_t)NCOEF * s0, d_rows, sizeof(double) * NCOEF * nk, hipMemcpyDeviceToHost, st),
+           "noalloc rows");
+      // the host copies of the case table and reduction record, and the
+      // scratch, are reused by the next chunk
+      ck(hipStreamSynchronize(st), "noalloc chunk sync");
+      c->advance_state(nk, step0 + s0 + nk, nullptr);
+    }
+    c->invalidate();
+  });
+}
+
+}  // extern "C"
 
 int aero_set_timing(aero_ctx* c, int32_t enabled) {
   return guarded([&] {

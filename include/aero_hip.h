@@ -186,6 +186,19 @@ int aero_get_outlet_prev(aero_ctx* ctx, double* prev);
  * hist: nsteps x AERO_NCOEF doubles, or NULL. */
 int aero_step(aero_ctx* ctx, int32_t nsteps, int64_t step0, const double* planes, double* hist);
 
+/* aero_step without any device-memory allocation: no counterpart in the
+ * reference.  For callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as aero_step for a context whose inlet takes no per-step planes and
+ * that has no LES split step (errors otherwise).  The case table, tile list,
+ * coefficient rows and reduction record live in one file-scope device array
+ * of AERO_NOALLOC_SCRATCH_BYTES per GPU, shared by every call on that GPU:
+ * calls that overlap on one GPU race.  A grid needs at most
+ * (AERO_NOALLOC_SCRATCH_BYTES - 64 KiB) / 8 tiles of 256 padded cells (about
+ * 130 M cells); larger grids are rejected. */
+#define AERO_NOALLOC_SCRATCH_BYTES (4u << 20)
+int aero_step_noalloc(aero_ctx* ctx, int32_t nsteps, int64_t step0, double* hist);
+
 /* Observables of the current state (compute_macroscopic, d2q9.py:44-63). */
 int aero_macroscopic(aero_ctx* ctx, double* rho, double* ux, double* uy, double* uz);
 int aero_check(aero_ctx* ctx, double* stats);
