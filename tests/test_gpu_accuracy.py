@@ -50,7 +50,11 @@ def _c1(storage, steps=20000):
     from aero_amd.diagnostics import compute_strouhal
     st = compute_strouhal(r["Cl_history"], D, u0)
     s.close()
+    _C1_CL[storage] = np.asarray(r["Cl_history"])
     return r["Cd_mean"], st, np.asarray(r["Cd_history"])
+
+
+_C1_CL = {}
 
 
 def _c2(storage, steps=5000):
@@ -91,6 +95,39 @@ def test_c1_fp32_cd_and_strouhal_within_half_percent_of_fp64():
     assert abs(cd32 - cd64) / abs(cd64) < 5e-3
     if st64 is not None and 0.1 < st64 < 0.3:
         assert st32 is not None and abs(st32 - st64) / st64 < 5e-3
+
+
+def _fft_bin(st, n, D=40.0, u0=0.05):
+    """FFT bin of a Strouhal number from compute_strouhal over n samples
+    (St = f D / u0, f = bin / n)."""
+    return int(round(st * u0 / D * n))
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_c1_long_run_matches_the_reference(storage):
+    """The reference's own 20000-step run of this configuration
+    (tests/golden/c1_long_run.npz, written by make_golden.py with the
+    reference's NumPy Solver): Cd_mean over the last 4000 steps within 0.5%,
+    the Strouhal number of the whole Cl history (compute_strouhal, as
+    cli.py:429 reports it) in the same FFT bin, and the std of Cl over the
+    last 4000 steps within 5%.  The float64 device run must also follow the
+    reference's drag history step for step."""
+    from tests import _golden as G
+    with np.load(G.GOLDEN / "c1_long_run.npz", allow_pickle=False) as d:
+        ref = {k: d[k] for k in d.files}
+    steps = int(ref["steps"])
+    cd_mean, st, hist = _c1(storage, steps)
+    cd_ref, cl_std_ref, st_ref = float(ref["Cd_mean_last4000"]), float(ref["Cl_std_last4000"]), float(ref["St"])
+    cd4 = float(np.mean(hist[-4000:]))
+    drift = float(np.max(np.abs(hist - ref["Cd_hist"])) / np.max(np.abs(ref["Cd_hist"])))
+    print(f"[gpu] C1 {storage}: Cd_mean(4000) {cd4:.6f} ref {cd_ref:.6f}; St {st} ref {st_ref}; "
+          f"max Cd history deviation {drift:.2e}", flush=True)
+    assert abs(cd4 - cd_ref) / abs(cd_ref) < 5e-3
+    assert st is not None and _fft_bin(st, steps) == _fft_bin(st_ref, steps)
+    cl_std = float(np.std(_C1_CL[storage][-4000:]))
+    assert abs(cl_std - cl_std_ref) / cl_std_ref < 5e-2, (cl_std, cl_std_ref)
+    if storage == "f64":
+        assert drift < 1e-6
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
