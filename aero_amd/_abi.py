@@ -67,6 +67,10 @@ _SIGS = {
     "aero_get_outlet_prev": ([_P, _P], ctypes.c_int),
     "aero_step": ([_P, ctypes.c_int32, ctypes.c_int64, _P, _P], ctypes.c_int),
     "aero_step_noalloc": ([_P, ctypes.c_int32, ctypes.c_int64, _P], ctypes.c_int),
+    "aero_tune_set": ([ctypes.c_int32] * 8, ctypes.c_int),
+    "aero_tune_clear": ([], ctypes.c_int),
+    "aero_tune_grid_class": ([ctypes.c_int64, _P], ctypes.c_int),
+    "aero_step_tiles": ([_P, _P], ctypes.c_int),
     "aero_macroscopic": ([_P, _P, _P, _P, _P], ctypes.c_int),
     "aero_check": ([_P, _P], ctypes.c_int),
     "aero_link_forces": ([_P, _P, _P, _P], ctypes.c_int),
@@ -103,6 +107,10 @@ def lib() -> ctypes.CDLL:
             fn.argtypes = args
             fn.restype = res
         _lib = L
+        # the launch tuning measured on this device class, when a table ships
+        # with the package (aero_amd/autotune.py); it changes schedules only
+        from . import autotune
+        autotune.load_default()
     return _lib
 
 

@@ -1,6 +1,7 @@
 // Host side of the C ABI: context state, geometry records, the step loop and
 // state materialisation.  See include/aero_hip.h for the contract.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -111,6 +112,34 @@ size_t storage_bytes(int st) {
 
 }  // namespace
 
+// Launch tuning of the batched step, set by aero_tune_set (the autotuner's
+// table): per kernel group -- lattice, collision, storage, scalar transport --
+// and grid class (floor(log2) of the group's tile count), how the group's
+// tiles are ordered and launched.  The choices change only the schedule:
+// every cell computes the same bits whatever they are.
+struct TuneVal {
+  int xcd_run = 0;  // 0: built-in rule; -1: natural order; k > 0: runs of k tiles per XCD
+  int nt = -1;      // -1: built-in rule (non-temporal stores past NT_STORE_BYTES); 0 / 1: off / on
+  int graph = 0;    // 1: a chunk's step launches replay from a captured HIP graph
+};
+namespace {
+std::mutex g_tune_mu;
+std::map<std::array<int, 5>, TuneVal> g_tune;
+std::atomic<unsigned> g_tune_gen{1};
+
+int grid_class(long long ntiles) {
+  int c = 0;
+  while ((2ll << c) <= ntiles) ++c;
+  return c;
+}
+
+TuneVal tune_lookup(int q, int coll, int st, int th, int cls) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = g_tune.find({q, coll, st, th, cls});
+  return it == g_tune.end() ? TuneVal{} : it->second;
+}
+}  // namespace
+
 // Buffers for stepping a set of cases in chunks of at most STEP_CHUNK steps:
 // the batched kernel's case table (pinned host copy + device copy, double
 // buffered) and packed tile list, double-buffered coefficient rows with
@@ -127,12 +156,27 @@ struct StepRunner {
   // is never reused, so a context created at a freed context's address never
   // matches a stale table
   std::vector<unsigned long long> tile_key;
+  unsigned tune_gen = 0;  // tuning table generation the tile table was built under
   // kernel groups of that case list: lattice, collision, storage, scalar
-  // transport, and the group's slice of the tile table
+  // transport, the group's slice of the tile table and its launch tuning
   struct Group {
     int q, coll, st, th, toff, ntiles;
+    TuneVal tune;
   };
   std::vector<Group> groups;
+  // captured chunks (graph launch tuning): by buffer parity and chunk length,
+  // valid for the tile table and case-table buffers they were captured with
+  struct Captured {
+    hipGraphExec_t exec = nullptr;
+    const void* cases = nullptr;
+    const void* tiles = nullptr;
+  };
+  std::map<std::pair<int, int>, Captured> graphs;
+  void drop_graphs() {
+    for (auto& kv : graphs)
+      if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    graphs.clear();
+  }
 
   template <class T>
   static void host_grow(T*& p, size_t n) {
@@ -174,6 +218,7 @@ struct StepRunner {
       stage[b] = nullptr;
       ev[b] = nullptr;
     }
+    drop_graphs();
     tiles.release();
     ncase_cap = rows_cap = 0;
     tile_key.clear();
@@ -502,7 +547,9 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
   // groups and the packed tile list, rebuilt only when the case list changes
   std::vector<unsigned long long> key;
   for (aero_ctx* c : cs) key.push_back(c->serial);
-  if (key != R.tile_key) {
+  const unsigned tgen = g_tune_gen.load();
+  if (key != R.tile_key || tgen != R.tune_gen) {
+    R.drop_graphs();
     std::vector<StepRunner::Group>& groups = R.groups;
     groups.clear();
     std::vector<std::vector<int>> members;
@@ -514,7 +561,7 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
             groups[g].th == (d.thermal != 0))
           break;
       if (g == (int)groups.size()) {
-        groups.push_back(StepRunner::Group{d.q, d.collision, d.storage, d.thermal != 0, 0, 0});
+        groups.push_back(StepRunner::Group{d.q, d.collision, d.storage, d.thermal != 0, 0, 0, TuneVal{}});
         members.emplace_back();
       }
       members[g].push_back(i);
@@ -530,8 +577,13 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
         if (c->d.nz > 1)
           zreuse = std::max(zreuse, (size_t)XCDS * 3 * c->Q * (size_t)c->xp * c->d.ny * c->esz);
       }
-      const int run = zreuse > ZREUSE_BYTES ? XCD_RUN_LARGE : ((int)nat.size() + XCDS - 1) / XCDS;
-      xcd_order(nat, std::max(run, 1), tiles);
+      const StepRunner::Group& G = groups[g];
+      const TuneVal tv = tune_lookup(G.q, G.coll, G.st, G.th, grid_class((long long)nat.size()));
+      groups[g].tune = tv;
+      int run = zreuse > ZREUSE_BYTES ? XCD_RUN_LARGE : ((int)nat.size() + XCDS - 1) / XCDS;
+      if (tv.xcd_run > 0) run = tv.xcd_run;
+      if (tv.xcd_run < 0) tiles.insert(tiles.end(), nat.begin(), nat.end());
+      else xcd_order(nat, std::max(run, 1), tiles);
       groups[g].ntiles = (int)tiles.size() - groups[g].toff;
     }
     R.tiles.alloc(sizeof(int2) * tiles.size());
@@ -539,6 +591,7 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
        "upload tiles");
     ck(hipStreamSynchronize(stream), "tiles sync");
     R.tile_key = key;
+    R.tune_gen = tgen;
   }
   const std::vector<StepRunner::Group>& groups = R.groups;
   const bool timed = n == 1 && cs[0]->timing;
@@ -557,7 +610,19 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
   // the next step reads it, so the stores bypass the caches.
   size_t ws = 0;
   for (aero_ctx* c : cs) ws += 2 * c->state_bytes() * (c->d.thermal ? 2 : 1);
-  const bool nt = ws > NT_STORE_BYTES;
+  const bool nt_rule = ws > NT_STORE_BYTES;
+  std::vector<char> nt(n, nt_rule);  // per case, the tuning of its group may override the rule
+  for (int i = 0; i < n; ++i)
+    for (const StepRunner::Group& g : groups)
+      if (g.q == cs[i]->d.q && g.coll == cs[i]->d.collision && g.st == cs[i]->d.storage &&
+          g.th == (cs[i]->d.thermal != 0) && g.tune.nt >= 0)
+        nt[i] = g.tune.nt == 1;
+  // the chunk's step launches replay from a captured graph when every
+  // group's tuning asks for it (not with per-step host work in between:
+  // the recycling prepass, or the timing events)
+  bool use_graph = !timed && !groups.empty();
+  for (const StepRunner::Group& g : groups) use_graph = use_graph && g.tune.graph == 1;
+  for (aero_ctx* c : cs) use_graph = use_graph && c->d.inlet != AERO_IN_RECYCLING;
   bool any_hist = false;
   for (int i = 0; i < n; ++i) any_hist |= hists[i] != nullptr;
   for (int s0 = 0, ci = 0; s0 < nsteps; s0 += CH, ++ci) {
@@ -578,18 +643,45 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
       }
       R.cases_host[b][i] = c->make_case(step0s[i] + s0, pdev, rows + (size_t)i * CH * NCOEF);
     }
-    if (nt)
-      for (int i = 0; i < n; ++i) R.cases_host[b][i].p[0].nt_store = R.cases_host[b][i].p[1].nt_store = 1;
+    for (int i = 0; i < n; ++i)
+      if (nt[i]) R.cases_host[b][i].p[0].nt_store = R.cases_host[b][i].p[1].nt_store = 1;
     ck(hipMemcpyAsync(R.cases[b].p, R.cases_host[b], sizeof(BatchCase) * n, hipMemcpyHostToDevice, stream),
        "upload cases");
     const BatchCase* cases = R.cases[b].as<BatchCase>();
-    for (int k = 0; k < nk; ++k) {
-      for (aero_ctx* c : cs)
-        if (c->d.inlet == AERO_IN_RECYCLING) c->recycling_prepass((c->side + k) & 1, stream);
-      for (const StepRunner::Group& g : groups)
-        ck(launch_batch_step(g.q, g.coll, g.st, g.th, cases, R.tiles.as<int2>() + g.toff, g.ntiles, k, stream,
-                             timed ? cs[0]->ev_pool[2 * k] : nullptr, timed ? cs[0]->ev_pool[2 * k + 1] : nullptr),
-           "step launch");
+    auto launch_steps = [&]() {
+      for (int k = 0; k < nk; ++k) {
+        for (aero_ctx* c : cs)
+          if (c->d.inlet == AERO_IN_RECYCLING) c->recycling_prepass((c->side + k) & 1, stream);
+        for (const StepRunner::Group& g : groups)
+          ck(launch_batch_step(g.q, g.coll, g.st, g.th, cases, R.tiles.as<int2>() + g.toff, g.ntiles, k, stream,
+                               timed ? cs[0]->ev_pool[2 * k] : nullptr, timed ? cs[0]->ev_pool[2 * k + 1] : nullptr),
+             "step launch");
+      }
+    };
+    if (use_graph) {
+      StepRunner::Captured& cap = R.graphs[{b, nk}];
+      if (!cap.exec || cap.cases != R.cases[b].p || cap.tiles != R.tiles.p) {
+        if (cap.exec) (void)hipGraphExecDestroy(cap.exec);
+        cap.exec = nullptr;
+        hipGraph_t graph = nullptr;
+        ck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        try {
+          launch_steps();
+        } catch (...) {
+          (void)hipStreamEndCapture(stream, &graph);
+          if (graph) (void)hipGraphDestroy(graph);
+          throw;
+        }
+        ck(hipStreamEndCapture(stream, &graph), "end capture");
+        const hipError_t e = hipGraphInstantiate(&cap.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        ck(e, "instantiate graph");
+        cap.cases = R.cases[b].p;
+        cap.tiles = R.tiles.p;
+      }
+      ck(hipGraphLaunch(cap.exec, stream), "graph launch");
+    } else {
+      launch_steps();
     }
     // the chunk's last step is reduced here, so every chunk's rows are
     // complete when they are copied out
@@ -1418,6 +1510,37 @@ int aero_step_noalloc(aero_ctx* c, int32_t nsteps, int64_t step0, double* hist) 
 }
 
 }  // extern "C"
+
+int aero_tune_set(int32_t q, int32_t collision, int32_t storage, int32_t thermal, int32_t grid_class_,
+                  int32_t xcd_run, int32_t nt, int32_t graph) {
+  return guarded([&] {
+    if ((q != 9 && q != 19) || collision < 0 || collision > 2 || storage < 0 || storage > 3 || grid_class_ < 0 ||
+        grid_class_ > 40 || nt < -1 || nt > 1 || graph < 0 || graph > 1 || xcd_run < -1)
+      throw std::invalid_argument("aero_tune_set: argument out of range");
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[{q, collision, storage, thermal != 0, grid_class_}] = TuneVal{xcd_run, nt, graph};
+    ++g_tune_gen;
+  });
+}
+
+int aero_tune_clear(void) {
+  return guarded([&] {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune.clear();
+    ++g_tune_gen;
+  });
+}
+
+int aero_tune_grid_class(int64_t ntiles, int32_t* cls) {
+  return guarded([&] {
+    if (ntiles < 1) throw std::invalid_argument("aero_tune_grid_class: ntiles must be positive");
+    *cls = grid_class(ntiles);
+  });
+}
+
+int aero_step_tiles(aero_ctx* c, int64_t* ntiles) {
+  return guarded([&] { *ntiles = (int64_t)c->sblocks + 1; });
+}
 
 int aero_set_timing(aero_ctx* c, int32_t enabled) {
   return guarded([&] {

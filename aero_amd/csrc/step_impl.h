@@ -1064,17 +1064,17 @@ struct StepWaves {
                              p.center[2], p.fdenom, p.mdenom, dyn.hist_prev};                              \
         reduce_forces_block<L>(r_);                                                                        \
       }                                                                                                    \
-      return;                                                                                              \
+      break;                                                                                               \
     }                                                                                                      \
     /* one thread per padded cell; thread indices fit in 31 bits (checked when the context is */          \
     /* created).  xp is a multiple of 64, so a wave covers 64 aligned cells of one row: t0_, row, y, */    \
     /* z and the storage base sb_ of the wave's first cell are wave-uniform (scalar registers). */          \
     const unsigned t0_ = (unsigned)(blk - 1) * (unsigned)BLOCK + __builtin_amdgcn_readfirstlane(threadIdx.x);\
     const unsigned row_ = udiv(t0_, p.div_xp);                                                             \
-    if (row_ >= (unsigned)p.nrow) return;                                                                  \
+    if (row_ >= (unsigned)p.nrow) break;                                                                   \
     const unsigned xw_ = t0_ - row_ * (unsigned)p.xp;                                                      \
     const int x = (int)(xw_ + __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)));          \
-    if (x >= p.nx) return; /* row padding */                                                               \
+    if (x >= p.nx) break; /* row padding */                                                                \
     const int z = (int)udiv(row_, p.div_ny);                                                               \
     const int y = (int)row_ - z * p.ny;                                                                    \
     const int cell = (int)(row_ * (unsigned)p.nx) + x;                                                     \
@@ -1133,6 +1133,8 @@ void step_kernel(StepParams p) {
 
 // Step k of a chunk for every case of a packed tile list: tile b is block
 // `y` of case `x` (block 0 of a case reduces its previous step's forces).
+// One workgroup per tile: a loop over several tiles per workgroup around this
+// body made the register allocator spill (96-108 bytes per lane).
 // Each case's parameters sit in device memory for both parities of k (the
 // ping-pong buffers swap every step); the per-step fields are derived here.
 // A single case runs through this same kernel as a batch of one, so a case
@@ -1151,8 +1153,8 @@ void batch_step_kernel(const BatchCase* __restrict__ cases, const int2* __restri
 }
 
 template <class L, int COLL, int ST, bool THERMAL>
-hipError_t launch_batch(const BatchCase* cases, const int2* tiles, int ntiles, int k, hipStream_t s,
-                        void* ev_start, void* ev_stop) {
+hipError_t launch_batch(const BatchCase* cases, const int2* tiles, int ntiles, int k, hipStream_t s, void* ev_start,
+                        void* ev_stop) {
   const dim3 grid((unsigned)ntiles);
   if (ev_start)
     hipExtLaunchKernelGGL(batch_step_kernel<L, COLL, ST, THERMAL>, grid, dim3(BLOCK), 0, s, (hipEvent_t)ev_start,

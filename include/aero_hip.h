@@ -199,6 +199,21 @@ int aero_step(aero_ctx* ctx, int32_t nsteps, int64_t step0, const double* planes
 #define AERO_NOALLOC_SCRATCH_BYTES (4u << 20)
 int aero_step_noalloc(aero_ctx* ctx, int32_t nsteps, int64_t step0, double* hist);
 
+/* Launch tuning of the batched step (no counterpart in the reference: the
+ * autotuner's table, aero_amd/autotune.py).  One entry per kernel group --
+ * lattice q, collision, storage, scalar transport -- and grid class (floor
+ * of log2 of the group's tile count, aero_tune_grid_class): xcd_run 0 keeps
+ * the built-in tile order, -1 the natural order, k > 0 runs of k tiles per
+ * XCD; nt -1 keeps the built-in non-temporal-store rule, 0 / 1
+ * turn it off / on; graph 1 replays each chunk's step launches from a
+ * captured HIP graph.  Entries change the schedule only, never the results.
+ * aero_step_tiles returns the tile count of one context's step. */
+int aero_tune_set(int32_t q, int32_t collision, int32_t storage, int32_t thermal, int32_t grid_class,
+                  int32_t xcd_run, int32_t nt, int32_t graph);
+int aero_tune_clear(void);
+int aero_tune_grid_class(int64_t ntiles, int32_t* grid_class);
+int aero_step_tiles(aero_ctx* ctx, int64_t* ntiles);
+
 /* Observables of the current state (compute_macroscopic, d2q9.py:44-63). */
 int aero_macroscopic(aero_ctx* ctx, double* rho, double* ux, double* uy, double* uz);
 int aero_check(aero_ctx* ctx, double* stats);

@@ -1,0 +1,44 @@
+"""Register budget of the hot step kernels (CPU: compiles for gfx950).
+
+The fused step is HBM-bound only while its waves neither spill much nor lose
+occupancy.  The D3Q19 TRT batch kernels (the 512 x 256 x 256 throughput
+workload) must keep their register target (7 waves per SIMD, at most 72
+VGPRs) with at most 32 bytes of scratch per lane on rare paths; a code change
+that makes them spill (a loop around the step body once cost 96-108 bytes
+per lane and 30% of the bandwidth) fails here.
+"""
+
+import pathlib
+import re
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parent.parent
+CSRC = ROOT / "aero_amd" / "csrc"
+
+
+@pytest.mark.skipif(shutil.which("hipcc") is None and not pathlib.Path("/opt/rocm/bin/hipcc").exists(),
+                    reason="hipcc not available")
+def test_d3q19_trt_batch_kernels_keep_their_register_budget(tmp_path):
+    from aero_amd import build as B
+    cmd = [B.HIPCC, *B.FLAGS, "-c", str(CSRC / "step_d3q19_trt.hip"), "-o", str(tmp_path / "t.o"),
+           "-Rpass-analysis=kernel-resource-usage"]
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr[-2000:]
+    text = out.stderr
+    blocks = re.split(r"remark: Function Name: ", text)[1:]
+    seen = 0
+    for b in blocks:
+        name = b.split()[0]
+        # batch_step_kernel<D3Q19, TRT, storage, thermal>: mangled ...ILi1ELi<ST>ELb<TH>E...
+        m = re.search(r"batch_step_kernelINS_5D3Q19ELi1ELi(\d)ELb([01])E", name)
+        if not m or m.group(1) == "0":  # FP64 keeps the compiler's own choice
+            continue
+        vgpr = int(re.search(r"VGPRs: (\d+)", b).group(1))
+        scratch = int(re.search(r"ScratchSize \[bytes/lane\]: (\d+)", b).group(1))
+        seen += 1
+        assert vgpr <= 72, (name, vgpr)
+        assert scratch <= 32, (name, scratch)
+    assert seen == 6
