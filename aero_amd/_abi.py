@@ -23,6 +23,8 @@ WALL_NONE, WALL_SLIP, WALL_NOSLIP, WALL_MOVING = range(4)
 LES_OFF, LES_SMAGORINSKY, LES_WALE = range(3)
 NCOEF = 20
 NCHECK = 5
+NSTAT = 15
+STAT_MAX_WINDOW = 4096
 
 
 class AeroDesc(ctypes.Structure):
@@ -71,6 +73,11 @@ _SIGS = {
     "aero_tune_clear": ([], ctypes.c_int),
     "aero_tune_grid_class": ([ctypes.c_int64, _P], ctypes.c_int),
     "aero_step_tiles": ([_P, _P], ctypes.c_int),
+    "aero_set_history_ring": ([_P, ctypes.c_int64], ctypes.c_int),
+    "aero_stationarity": ([ctypes.c_int32, _P, _P, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_double,
+                           _P], ctypes.c_int),
+    "aero_stationarity_arrays": ([ctypes.c_int32, _P, _P, _P, _P, _P, _P, ctypes.c_int32, ctypes.c_double,
+                                  ctypes.c_int32, ctypes.c_double, _P], ctypes.c_int),
     "aero_macroscopic": ([_P, _P, _P, _P, _P], ctypes.c_int),
     "aero_check": ([_P, _P], ctypes.c_int),
     "aero_link_forces": ([_P, _P, _P, _P], ctypes.c_int),

@@ -146,9 +146,10 @@ TuneVal tune_lookup(int q, int coll, int st, int th, int cls) {
 // pinned staging, and the last-step force reductions.  They only grow, so
 // stepping a set of cases no larger than before allocates nothing.
 struct StepRunner {
-  DevBuf cases[2], tiles, rows[2], reds[2];
+  DevBuf cases[2], tiles, rows[2], reds[2], rapp[2];
   BatchCase* cases_host[2] = {nullptr, nullptr};
   ForceReduce* reds_host[2] = {nullptr, nullptr};
+  RingAppend* rapp_host[2] = {nullptr, nullptr};
   double* stage[2] = {nullptr, nullptr};
   size_t ncase_cap = 0, rows_cap = 0;
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -190,12 +191,14 @@ struct StepRunner {
       if (!ev[b]) ck(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming), "hipEventCreate");
       cases[b].alloc(sizeof(BatchCase) * ncase);
       reds[b].alloc(sizeof(ForceReduce) * ncase);
+      rapp[b].alloc(sizeof(RingAppend) * ncase);
       rows[b].alloc(sizeof(double) * nrows);
     }
     if ((size_t)ncase > ncase_cap) {
       for (int b = 0; b < 2; ++b) {
         host_grow(cases_host[b], ncase);
         host_grow(reds_host[b], ncase);
+        host_grow(rapp_host[b], ncase);
       }
       ncase_cap = ncase;
     }
@@ -209,8 +212,11 @@ struct StepRunner {
       cases[b].release();
       rows[b].release();
       reds[b].release();
+      rapp[b].release();
       if (cases_host[b]) (void)hipHostFree(cases_host[b]);
       if (reds_host[b]) (void)hipHostFree(reds_host[b]);
+      if (rapp_host[b]) (void)hipHostFree(rapp_host[b]);
+      rapp_host[b] = nullptr;
       if (stage[b]) (void)hipHostFree(stage[b]);
       if (ev[b]) (void)hipEventDestroy(ev[b]);
       cases_host[b] = nullptr;
@@ -267,6 +273,14 @@ struct aero_ctx {
   int ibm_on = 0;          // immersed-boundary mode (aero_set_ibm)
   int sponge_x0 = 0;       // first sponge column (nx: none)
   double ibm_coeff = 0.0;
+  // FP64 rings of the drag and lift histories (aero_set_history_ring): three
+  // series of ring_cap doubles, sample s of the stream at index s % ring_cap
+  DevBuf ring, ring_rec;  // ring_rec: the LES path's two append records
+  long long ring_cap = 0;
+  std::atomic<long long> ring_count{0};  // read by aero_stationarity while a step call appends
+  RingAppend ring_append(const double* rows, int nk) const {
+    return RingAppend{rows, ring.as<double>(), ring_count.load(), ring_cap, nk};
+  }
   // LES split buffers
   DevBuf fsplit, usplit;
   // caches of the reference state in f64
@@ -703,6 +717,16 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
         ck(launch_reduce_forces_many(q, R.reds[b].as<ForceReduce>() + m0, m - m0, stream), "force reduction");
       }
     }
+    int nr = 0;
+    for (int i = 0; i < n; ++i)
+      if (cs[i]->ring_cap > 0) R.rapp_host[b][nr++] = cs[i]->ring_append(rows + (size_t)i * CH * NCOEF, nk);
+    if (nr > 0) {
+      ck(hipMemcpyAsync(R.rapp[b].p, R.rapp_host[b], sizeof(RingAppend) * nr, hipMemcpyHostToDevice, stream),
+         "upload ring appends");
+      ck(launch_ring_append(R.rapp[b].as<RingAppend>(), nr, nk, stream), "ring append");
+      for (aero_ctx* c : cs)
+        if (c->ring_cap > 0) c->ring_count += nk;
+    }
     if (any_hist)
       ck(hipMemcpyAsync(R.stage[b], rows, sizeof(double) * NCOEF * CH * (size_t)n, hipMemcpyDeviceToHost, stream),
          "download rows");
@@ -852,7 +876,7 @@ int aero_destroy(aero_ctx* c) {
   DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags, &c->wave_any,
                     &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
                     &c->damp, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
-                    &c->gcache, &c->tmp, &c->checkpart, &c->checkout};
+                    &c->gcache, &c->tmp, &c->checkpart, &c->checkout, &c->ring, &c->ring_rec};
   for (DevBuf* b : bufs) b->release();
   c->runner.release();
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -1357,6 +1381,13 @@ int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, 
       } else {
         ck(hipMemsetAsync(rows, 0, sizeof(double) * NCOEF * (size_t)n, c->stream), "zero history");
       }
+      if (c->ring_cap > 0) {
+        const RingAppend ra = c->ring_append(rows, n);
+        RingAppend* rec = c->ring_rec.as<RingAppend>() + b;
+        ck(hipMemcpyAsync(rec, &ra, sizeof(RingAppend), hipMemcpyHostToDevice, c->stream), "upload ring append");
+        ck(launch_ring_append(rec, 1, n, c->stream), "ring append");
+        c->ring_count += n;
+      }
       if (hist) {
         ck(hipMemcpyAsync(c->hist_stage[b], rows, sizeof(double) * NCOEF * (size_t)n, hipMemcpyDeviceToHost,
                           c->stream),
@@ -1414,12 +1445,13 @@ void* noalloc_scratch(int device) {
 // Scratch layout: case tables for the two step parities' chunk, reduction
 // record, coefficient rows of one chunk, then the tile list.
 struct NoallocLayout {
-  size_t cases, red, rows, tiles, end;
+  size_t cases, red, rapp, rows, tiles, end;
   explicit NoallocLayout(int ntiles) {
     auto up = [](size_t v) { return (v + 255) / 256 * 256; };
     cases = 0;
     red = up(cases + sizeof(BatchCase));
-    rows = up(red + sizeof(ForceReduce));
+    rapp = up(red + sizeof(ForceReduce));
+    rows = up(rapp + sizeof(RingAppend));
     tiles = up(rows + sizeof(double) * NCOEF * STEP_CHUNK);
     end = tiles + sizeof(int2) * (size_t)ntiles;
   }
@@ -1497,6 +1529,13 @@ int aero_step_noalloc(aero_ctx* c, int32_t nsteps, int64_t step0, double* hist) 
         ck(hipMemcpyAsync(d_red, &r, sizeof(ForceReduce), hipMemcpyHostToDevice, st), "noalloc reduction");
         ck(launch_reduce_forces_many(c->Q, d_red, 1, st), "noalloc force reduction");
       }
+      if (c->ring_cap > 0) {
+        const RingAppend ra = c->ring_append(d_rows, nk);
+        RingAppend* d_rapp = (RingAppend*)(base + L.rapp);
+        ck(hipMemcpyAsync(d_rapp, &ra, sizeof(RingAppend), hipMemcpyHostToDevice, st), "noalloc ring append");
+        ck(launch_ring_append(d_rapp, 1, nk, st), "noalloc ring append");
+        c->ring_count += nk;
+      }
       if (hist)
         ck(hipMemcpyAsync(hist + (size_t)NCOEF * s0, d_rows, sizeof(double) * NCOEF * nk, hipMemcpyDeviceToHost, st),
            "noalloc rows");
@@ -1540,6 +1579,133 @@ int aero_tune_grid_class(int64_t ntiles, int32_t* cls) {
 
 int aero_step_tiles(aero_ctx* c, int64_t* ntiles) {
   return guarded([&] { *ntiles = (int64_t)c->sblocks + 1; });
+}
+
+int aero_set_history_ring(aero_ctx* c, int64_t cap) {
+  return guarded([&] {
+    if (cap < 0) throw std::invalid_argument("aero_set_history_ring: negative capacity");
+    c->set_device();
+    ck(hipStreamSynchronize(c->stream), "ring sync");
+    c->ring.release();
+    c->ring_rec.release();
+    c->ring_cap = 0;
+    c->ring_count = 0;
+    if (cap == 0) return;
+    c->ring.alloc(sizeof(double) * 3 * (size_t)cap);
+    c->ring_rec.alloc(sizeof(RingAppend) * 2);
+    c->ring_cap = cap;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+// Buffers of the stationarity test, grown as needed, with their own stream.
+struct StatScratch {
+  hipStream_t stream = nullptr;
+  DevBuf series, out, work, siglen, peak, data;
+};
+std::mutex g_stat_mu;
+StatScratch g_stat[64];
+
+void run_stationarity(std::vector<StatSeries>& ser, const StatParams& sp, double* out_host,
+                      const std::vector<std::vector<double>>* upload) {
+  int dev = 0;
+  ck(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= 64) throw std::runtime_error("aero_stationarity: device index out of range");
+  std::lock_guard<std::mutex> lock(g_stat_mu);
+  StatScratch& S = g_stat[dev];
+  if (!S.stream) ck(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking), "hipStreamCreate");
+  const int n = (int)ser.size();
+  if (upload) {  // host histories: plain arrays in one device buffer
+    size_t total = 0;
+    for (const auto& v : *upload) total += v.size();
+    S.data.alloc(sizeof(double) * std::max<size_t>(total, 1));
+    size_t off = 0;
+    std::vector<const double*> base(upload->size());
+    for (size_t k = 0; k < upload->size(); ++k) {
+      const auto& v = (*upload)[k];
+      if (!v.empty())
+        ck(hipMemcpyAsync(S.data.as<double>() + off, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice,
+                          S.stream),
+           "upload histories");
+      base[k] = S.data.as<double>() + off;
+      off += v.size();
+    }
+    for (int i = 0; i < n; ++i) {
+      ser[i].cd = base[3 * i];
+      ser[i].ly = base[3 * i + 1];
+      ser[i].lz = (*upload)[3 * i + 2].empty() ? nullptr : base[3 * i + 2];
+    }
+  }
+  S.series.alloc(sizeof(StatSeries) * n);
+  S.out.alloc(sizeof(double) * NSTAT * n);
+  S.work.alloc(sizeof(double) * 3 * (size_t)n * sp.st_window);
+  S.siglen.alloc(sizeof(int) * 3 * n);
+  S.peak.alloc(sizeof(int) * 3 * n);
+  ck(hipMemcpyAsync(S.series.p, ser.data(), sizeof(StatSeries) * n, hipMemcpyHostToDevice, S.stream), "upload series");
+  ck(launch_stationarity(S.series.as<StatSeries>(), n, sp, S.out.as<double>(), S.work.as<double>(),
+                         S.siglen.as<int>(), S.peak.as<int>(), S.stream),
+     "stationarity");
+  ck(hipMemcpyAsync(out_host, S.out.p, sizeof(double) * NSTAT * n, hipMemcpyDeviceToHost, S.stream), "download");
+  ck(hipStreamSynchronize(S.stream), "stationarity sync");
+}
+
+StatParams stat_params(int32_t conv_window, double conv_tol, int32_t st_window, double st_tol) {
+  if (conv_window < 1 || st_window < 1 || st_window > STAT_MAX_WINDOW)
+    throw std::invalid_argument("aero_stationarity: windows must be positive and the Strouhal window at most " +
+                                std::to_string(STAT_MAX_WINDOW));
+  return StatParams{conv_window, conv_tol, st_window, st_tol};
+}
+}  // namespace
+
+extern "C" {
+
+int aero_stationarity(int32_t n, aero_ctx* const* ctxs, const int64_t* counts, int32_t conv_window, double conv_tol,
+                      int32_t st_window, double st_tol, double* out) {
+  return guarded([&] {
+    if (n <= 0) return;
+    const StatParams sp = stat_params(conv_window, conv_tol, st_window, st_tol);
+    std::vector<StatSeries> ser(n);
+    int dev = -1;
+    for (int i = 0; i < n; ++i) {
+      aero_ctx* c = ctxs[i];
+      if (!c || c->ring_cap <= 0) throw std::invalid_argument("aero_stationarity: context without a history ring");
+      if (dev >= 0 && c->device != dev) throw std::invalid_argument("aero_stationarity: contexts on different devices");
+      dev = c->device;
+      if (c->ring_cap < std::max<long long>(conv_window, st_window))
+        throw std::invalid_argument("aero_stationarity: ring smaller than the windows");
+      if (counts[i] < 0 || counts[i] > c->ring_count)
+        throw std::invalid_argument("aero_stationarity: count beyond the samples the ring has received");
+      const double* r = c->ring.as<double>();
+      ser[i] = StatSeries{r, r + c->ring_cap, c->Q == 19 ? r + 2 * c->ring_cap : nullptr, counts[i], c->ring_cap,
+                          c->d.D, c->d.u0};
+    }
+    ck(hipSetDevice(dev), "hipSetDevice");
+    run_stationarity(ser, sp, out, nullptr);
+  });
+}
+
+int aero_stationarity_arrays(int32_t n, const double* const* cd, const double* const* ly, const double* const* lz,
+                             const int64_t* len, const double* D, const double* u0, int32_t conv_window,
+                             double conv_tol, int32_t st_window, double st_tol, double* out) {
+  return guarded([&] {
+    if (n <= 0) return;
+    const StatParams sp = stat_params(conv_window, conv_tol, st_window, st_tol);
+    const long long keep = std::max<long long>(conv_window, st_window);
+    std::vector<std::vector<double>> up(3 * (size_t)n);
+    std::vector<StatSeries> ser(n);
+    for (int i = 0; i < n; ++i) {
+      if (len[i] < 0) throw std::invalid_argument("aero_stationarity_arrays: negative length");
+      const long long m = std::min<long long>(len[i], keep);
+      const long long from = len[i] - m;
+      up[3 * i].assign(cd[i] + from, cd[i] + len[i]);
+      up[3 * i + 1].assign(ly[i] + from, ly[i] + len[i]);
+      if (lz && lz[i]) up[3 * i + 2].assign(lz[i] + from, lz[i] + len[i]);
+      ser[i] = StatSeries{nullptr, nullptr, nullptr, m, 0, D[i], u0[i]};
+    }
+    run_stationarity(ser, sp, out, &up);
+  });
 }
 
 int aero_set_timing(aero_ctx* c, int32_t enabled) {
@@ -1640,7 +1806,7 @@ int aero_device_bytes(aero_ctx* c, int64_t* bytes) {
     const DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags, &c->wave_any,
                             &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
                             &c->damp, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
-                            &c->gcache, &c->tmp, &c->checkpart, &c->checkout};
+                            &c->gcache, &c->tmp, &c->checkpart, &c->checkout, &c->ring, &c->ring_rec};
     int64_t t = 0;
     for (const DevBuf* b : bufs) t += (int64_t)b->bytes;
     *bytes = t;

@@ -55,6 +55,45 @@ hipError_t launch_store_from_f64(int st, int q, long long n, int nx, int xp, con
                                  double shift_scale, hipStream_t s);
 hipError_t launch_compute_from_f64(int st, long long count, const double* src, void* dst, hipStream_t s);
 hipError_t launch_ibm_field(int q, int st, long long n, void* F, const uint8_t* flags, double coeff, hipStream_t s);
+// Convergence statistics (stats.hip).  A case's drag and lift histories,
+// as FP64 ring buffers (newest sample at index (count - 1) mod cap) or plain
+// arrays (cap 0, count samples from index 0); lz null in 2D.
+struct StatSeries {
+  const double* cd;
+  const double* ly;
+  const double* lz;
+  long long count;
+  long long cap;
+  double D, u0;
+};
+struct StatParams {
+  int conv_window;
+  double conv_tol;
+  int st_window;
+  double st_tol;
+};
+// Per case NSTAT doubles: stop, Cd converged, ratio, mean |Cd|, std Cd,
+// St (NaN when none), peak frequency, peak amplitude, window used, St drift
+// between the half windows (NaN when not formed), stationary, lift series
+// (1: Cl / Cly, 2: Clz), then the three signals' peak amplitudes.
+constexpr int NSTAT = 15;
+// The largest Strouhal window the device test takes (its DFT twiddle table
+// sits in LDS: n + 2 doubles for a window of n samples).
+constexpr int STAT_MAX_WINDOW = 4096;
+hipError_t launch_stationarity(const StatSeries* series, int n, const StatParams& sp, double* out, double* work,
+                               int* siglen, int* peak, hipStream_t s);
+// Append nk coefficient rows (NCOEF doubles per step: Cd, Cl_y, Cl_z first)
+// to a case's ring of capacity cap (three series of cap doubles) from
+// position start.
+struct RingAppend {
+  const double* rows;
+  double* ring;
+  long long start;
+  long long cap;
+  int nk;
+};
+hipError_t launch_ring_append(const RingAppend* ra, int n, int max_nk, hipStream_t s);
+
 // STL geometry (mesh.hip).
 size_t mesh_inside_scratch_bytes(long long ntri);
 hipError_t launch_mesh_inside(const double* tris, long long ntri, void* scratch, const double* pts, long long npts,

@@ -233,7 +233,7 @@ class Server:
     def __init__(self, chunk: int = 500, max_live: int = 512, mem_reserve: int = 8 << 30,
                  convergence_window: int = 5000, convergence_tol: float = 0.01, strouhal_window: int = 4096,
                  strouhal_tol: float = 0.05, batch=None, free_memory: Optional[Callable[[], int]] = None,
-                 keep_history: bool = False, build_workers: int = 8):
+                 keep_history: bool = False, build_workers: int = 8, device_stats: bool = True):
         self.chunk = int(chunk)
         self.build_workers = max(1, int(build_workers))
         self.max_live = int(max_live)
@@ -243,6 +243,11 @@ class Server:
         self._batch = batch
         self._free_memory = free_memory
         self.keep_history = keep_history
+        # the auto-stop test on the device (aero_amd/stationarity.py) over rings
+        # of each case's drag and lift history, one call for every live case;
+        # the host restatement when the Strouhal window exceeds the device's
+        from . import _abi
+        self.device_stats = bool(device_stats) and 1 <= strouhal_window <= _abi.STAT_MAX_WINDOW
         self.steps_done = 0      # lattice updates, for aggregate MLUPS
         self.busy_seconds = 0.0  # time inside batched stepping
 
@@ -261,12 +266,27 @@ class Server:
         _abi.check(_abi.lib().aero_mem_info(_abi.ctypes.byref(free), _abi.ctypes.byref(total)))
         return int(free.value)
 
-    def _done(self, lv: _Live) -> Optional[str]:
+    def _stops(self, lvs: Sequence[_Live]) -> dict:
+        """The auto-stop decision of each case on its history so far: one
+        device call for all of them, or the host test per case."""
+        lvs = [lv for lv in lvs if lv.solver.Cd_history]
+        if not lvs:
+            return {}
+        if self.device_stats and all(getattr(lv.solver, "_ring_base", None) is not None for lv in lvs):
+            from .stationarity import stationarity
+            out = stationarity([lv.solver for lv in lvs], counts=[len(lv.solver.Cd_history) for lv in lvs],
+                               **self.stop_kw)
+            return {id(lv): bool(row[0]) for lv, row in zip(lvs, out)}
+        return {id(lv): detect_statistical_stationarity(
+            lv.solver.Cd_history, lv.solver.lift_history(self.stop_kw["strouhal_window"]), lv.solver.D,
+            lv.solver.u0, **self.stop_kw)[2] for lv in lvs}
+
+    def _done(self, lv: _Live, stop: Optional[bool] = None) -> Optional[str]:
         s = lv.solver
         if lv.error is not None or not np.isfinite(s.Cd_history[-1]):
             return "diverged"
-        _, _, stop = detect_statistical_stationarity(s.Cd_history, s.lift_history(self.stop_kw["strouhal_window"]),
-                                                     s.D, s.u0, **self.stop_kw)
+        if stop is None:
+            stop = self._stops([lv]).get(id(lv), False)
         if stop:
             return "converged"
         if len(s.Cd_history) >= lv.spec.max_steps:
@@ -335,7 +355,12 @@ class Server:
                 for b in [b for b in building if b[0].done()]:
                     building.remove(b)
                     fut, arr, rid, spec, _ = b
-                    live.append(_Live(rid, spec, fut.result(), arr, clock() - t0))
+                    solver = fut.result()
+                    if self.device_stats and hasattr(solver, "_ctx"):
+                        from .stationarity import enable_ring
+                        w = max(self.stop_kw["convergence_window"], self.stop_kw["strouhal_window"])
+                        enable_ring(solver, w + self.chunk)  # room for the chunk stepped during the check
+                    live.append(_Live(rid, spec, solver, arr, clock() - t0))
                 if not live:
                     if building:
                         wait([b[0] for b in building], return_when=FIRST_COMPLETED)
@@ -346,8 +371,11 @@ class Server:
                 # whatever it finds, so it is decided now rather than stepped for
                 # a speculative chunk: same reason, same history cut, same result.
                 now = clock() - t0
-                for lv in [lv for lv in live if len(lv.solver.Cd_history) >= lv.spec.max_steps]:
-                    done.append(self._result(lv, self._done(lv), now, len(lv.solver.Cd_history)))
+                at_limit = [lv for lv in live if len(lv.solver.Cd_history) >= lv.spec.max_steps]
+                stops = self._stops(at_limit)
+                for lv in at_limit:
+                    done.append(self._result(lv, self._done(lv, stops.get(id(lv), False)), now,
+                                             len(lv.solver.Cd_history)))
                     lv.solver.close()
                     live.remove(lv)
                 if not live:
@@ -359,9 +387,10 @@ class Server:
                 fut = pool.submit(batch.advance, [lv.solver for lv in stepping], n, False)
                 # stop checks on the rows recorded so far, while the device steps
                 retire = []
+                stops = self._stops(stepping)
                 for lv in stepping:
                     if lv.solver.Cd_history:
-                        reason = self._done(lv)
+                        reason = self._done(lv, stops.get(id(lv), False))
                         if reason is not None:
                             retire.append((lv, reason, len(lv.solver.Cd_history)))
                 rows = fut.result()

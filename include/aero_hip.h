@@ -214,6 +214,30 @@ int aero_tune_clear(void);
 int aero_tune_grid_class(int64_t ntiles, int32_t* grid_class);
 int aero_step_tiles(aero_ctx* ctx, int64_t* ntiles);
 
+/* Convergence statistics on the device, in FP64 (the auto-stop test of
+ * aero/diagnostics.py:87-120, 153-228 and the 3D lift choice of
+ * aero/lbm/solver3d.py:634-638).  aero_set_history_ring gives a context
+ * FP64 rings of `cap` samples of its drag and lift histories (Cd, Cl_y,
+ * Cl_z), appended on the device by every step call from then on (cap 0
+ * removes them).  aero_stationarity evaluates n contexts at once over the
+ * first counts[i] samples their rings received; aero_stationarity_arrays
+ * does the same for host histories (lz NULL or lz[i] NULL in 2D).  Per case
+ * AERO_NSTAT doubles: stop, Cd converged, std/mean ratio, mean |Cd|, std Cd,
+ * St (NaN when none), peak frequency, peak amplitude, samples in the
+ * Strouhal window, St drift between its halves (NaN when not formed),
+ * stationary, lift series (1: Cl / Cl_y, 2: Cl_z), then the peak amplitudes
+ * of the window and of its two halves.  The Strouhal window is at most
+ * AERO_STAT_MAX_WINDOW samples; a ring read while its context steps needs
+ * cap >= the larger window + the steps of that call. */
+#define AERO_NSTAT 15
+#define AERO_STAT_MAX_WINDOW 4096
+int aero_set_history_ring(aero_ctx* ctx, int64_t cap);
+int aero_stationarity(int32_t n, aero_ctx* const* ctxs, const int64_t* counts, int32_t conv_window,
+                      double conv_tol, int32_t st_window, double st_tol, double* out);
+int aero_stationarity_arrays(int32_t n, const double* const* cd, const double* const* ly, const double* const* lz,
+                             const int64_t* len, const double* D, const double* u0, int32_t conv_window,
+                             double conv_tol, int32_t st_window, double st_tol, double* out);
+
 /* Observables of the current state (compute_macroscopic, d2q9.py:44-63). */
 int aero_macroscopic(aero_ctx* ctx, double* rho, double* ux, double* uy, double* uz);
 int aero_check(aero_ctx* ctx, double* stats);
