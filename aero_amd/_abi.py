@@ -20,7 +20,7 @@ STORE = {"f64": 0, "f32": 1, "f16": 2, "bf16": 3}
 IN_NONE, IN_ZOUHE, IN_PRESSURE, IN_FIELD, IN_SEM, IN_RECYCLING = range(6)
 OUT_NONE, OUT_CONVECTIVE, OUT_ZEROGRAD, OUT_PRESSURE = range(4)
 WALL_NONE, WALL_SLIP, WALL_NOSLIP, WALL_MOVING = range(4)
-LES_OFF, LES_SMAGORINSKY, LES_WALE = range(3)
+LES_OFF, LES_SMAGORINSKY, LES_WALE, LES_SMAGORINSKY_LOCAL = range(4)
 NCOEF = 20
 NCHECK = 5
 NSTAT = 15

@@ -38,7 +38,28 @@ def resolve_backend(backend: str) -> str:
 # the reference's from_checkpoint (aero/lbm/solver.py:711-727,
 # solver3d.py:816-838) can rebuild a solver from a checkpoint written here.
 EXTRA_PARAMS_KEY = "aero_amd_params_json"
-EXTRA_PARAM_NAMES = ("storage", "trt_variant", "bouzidi_variant")
+EXTRA_PARAM_NAMES = ("storage", "trt_variant", "bouzidi_variant", "les_form")
+
+LES_FORMS = ("fd", "local")
+
+
+def les_code(les: bool, les_model: str, les_form: str) -> int:
+    """The device LES mode.  les_form "fd" is the reference's finite-difference
+    Smagorinsky / WALE on the pre-collision velocity (les.py:273-344), run as a
+    split step; "local" is Smagorinsky from each cell's non-equilibrium
+    stress, fused into the step (and batchable).  WALE needs the rotation part
+    of the velocity gradient, which the local stress does not carry, so it
+    has no local form."""
+    if les_form not in LES_FORMS:
+        raise ValueError(f"les_form must be one of {LES_FORMS}.")
+    if not les:
+        return _abi.LES_OFF
+    if les_form == "local":
+        if les_model != "smagorinsky":
+            raise ValueError("les_form='local' is Smagorinsky only: WALE needs the velocity gradient's "
+                             "rotation part, which the local non-equilibrium stress does not carry.")
+        return _abi.LES_SMAGORINSKY_LOCAL
+    return _abi.LES_WALE if les_model == "wale" else _abi.LES_SMAGORINSKY
 
 
 def extra_params_entry(extra: dict) -> dict:

@@ -7,7 +7,16 @@
 namespace aero {
 
 hipError_t launch_batch_step(int q, int coll, int st, bool thermal, const BatchCase* cases, const int2* tiles,
-                             int ntiles, int k, hipStream_t s, void* e0, void* e1) {
+                             int ntiles, int k, hipStream_t s, void* e0, void* e1, bool les_local) {
+  if (les_local) {  // local Smagorinsky kernels (BGK and TRT; MRT ignores LES)
+    if (q == 9 && coll == COLL_BGK) return launch_batch_les_d2q9_bgk(st, thermal, cases, tiles, ntiles, k, s, e0, e1);
+    if (q == 9 && coll == COLL_TRT) return launch_batch_les_d2q9_trt(st, thermal, cases, tiles, ntiles, k, s, e0, e1);
+    if (q == 19 && coll == COLL_BGK)
+      return launch_batch_les_d3q19_bgk(st, thermal, cases, tiles, ntiles, k, s, e0, e1);
+    if (q == 19 && coll == COLL_TRT)
+      return launch_batch_les_d3q19_trt(st, thermal, cases, tiles, ntiles, k, s, e0, e1);
+    return hipErrorInvalidValue;
+  }
   if (q == 9) {
     switch (coll) {
       case COLL_BGK: return launch_batch_d2q9_bgk(st, thermal, cases, tiles, ntiles, k, s, e0, e1);

@@ -161,8 +161,9 @@ struct StepRunner {
   // kernel groups of that case list: lattice, collision, storage, scalar
   // transport, the group's slice of the tile table and its launch tuning
   struct Group {
-    int q, coll, st, th, toff, ntiles;
+    int q, coll, st, th, les, toff, ntiles;  // les: local Smagorinsky kernels
     TuneVal tune;
+    bool holds(const aero_ctx* c) const;
   };
   std::vector<Group> groups;
   // captured chunks (graph launch tuning): by buffer parity and chunk length,
@@ -253,6 +254,7 @@ struct aero_ctx {
   int prev_side = 0;
   // geometry
   DevBuf flags, wave_any, lrec, rec_of, qdir, fpart[2], sigma, minv_d, minv_f, damp;
+  DevBuf les_c2d, les_c2f;  // local Smagorinsky: (C_s d)^2 per cell with Van Driest damping d
   int fpart_side = 0;  // buffer holding the last completed step's per-link-cell force sums
   int nrec = 0;
   std::vector<int> rec_cells;
@@ -296,7 +298,12 @@ struct aero_ctx {
 
   // bytes of one population buffer in the storage layout
   size_t state_bytes() const { return esz * Q * (size_t)nrow * xp; }
-  bool les_split() const { return d.les != AERO_LES_OFF && d.collision != AERO_MRT; }
+  // the reference's finite-difference LES: a split step (MRT ignores LES)
+  bool les_split() const {
+    return (d.les == AERO_LES_SMAGORINSKY || d.les == AERO_LES_WALE) && d.collision != AERO_MRT;
+  }
+  // Smagorinsky from the local non-equilibrium stress, fused into the step
+  bool les_local() const { return d.les == AERO_LES_SMAGORINSKY_LOCAL && d.collision != AERO_MRT; }
   bool need_planes() const { return d.inlet == AERO_IN_FIELD || d.inlet == AERO_IN_SEM; }
   // Inlet planes of steps s0 .. s0+n-1 of a call into planes[b]: uploaded
   // from the host planes, or, with the device SEM, generated from the host's
@@ -355,6 +362,11 @@ struct aero_ctx {
     p.omega = d.omega;
     p.trt_lambda = d.trt_lambda;
     p.trt_sm = trt_s_minus(d.omega, d.trt_lambda, d.trt_variant);
+    p.trt_variant = d.trt_variant;
+    p.les_tau0 = 1.0 / d.omega;
+    p.les_c2 = d.les_cs * d.les_cs;
+    p.les_c2d = (decltype(p.les_c2d))(les_c2d.as<double>());
+    p.les_c2f = (decltype(p.les_c2f))(les_c2f.as<float>());
     p.u0 = d.u0;
     p.u_conv = d.u0;
     p.inlet_amp = d.inlet_perturbation > 0.0 ? d.inlet_perturbation : 0.0;
@@ -467,7 +479,7 @@ struct aero_ctx {
     cp.les_cs = d.les_cs;
     cp.u0 = d.u0;
     for (int a = 0; a < NSHIFT; ++a) cp.fsh[a] = bp.fsh[a];
-    cp.les = U ? d.les : AERO_LES_OFF;
+    cp.les = (U || les_local()) ? d.les : AERO_LES_OFF;
     ck(launch_collide(Q, d.collision, d.storage, cp, stream), "collide launch");
   }
 
@@ -515,6 +527,11 @@ struct aero_ctx {
     gcache_valid = false;
   }
 };
+
+bool StepRunner::Group::holds(const aero_ctx* c) const {
+  return q == c->d.q && coll == c->d.collision && st == c->d.storage && th == (c->d.thermal != 0) &&
+         les == (c->les_local() ? 1 : 0);
+}
 
 namespace {
 
@@ -571,11 +588,10 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
       const aero_desc& d = cs[i]->d;
       int g = 0;
       for (; g < (int)groups.size(); ++g)
-        if (groups[g].q == d.q && groups[g].coll == d.collision && groups[g].st == d.storage &&
-            groups[g].th == (d.thermal != 0))
-          break;
+        if (groups[g].holds(cs[i])) break;
       if (g == (int)groups.size()) {
-        groups.push_back(StepRunner::Group{d.q, d.collision, d.storage, d.thermal != 0, 0, 0, TuneVal{}});
+        groups.push_back(StepRunner::Group{d.q, d.collision, d.storage, d.thermal != 0, cs[i]->les_local() ? 1 : 0,
+                                           0, 0, TuneVal{}});
         members.emplace_back();
       }
       members[g].push_back(i);
@@ -628,9 +644,7 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
   std::vector<char> nt(n, nt_rule);  // per case, the tuning of its group may override the rule
   for (int i = 0; i < n; ++i)
     for (const StepRunner::Group& g : groups)
-      if (g.q == cs[i]->d.q && g.coll == cs[i]->d.collision && g.st == cs[i]->d.storage &&
-          g.th == (cs[i]->d.thermal != 0) && g.tune.nt >= 0)
-        nt[i] = g.tune.nt == 1;
+      if (g.holds(cs[i]) && g.tune.nt >= 0) nt[i] = g.tune.nt == 1;
   // the chunk's step launches replay from a captured graph when every
   // group's tuning asks for it (not with per-step host work in between:
   // the recycling prepass, or the timing events)
@@ -668,7 +682,8 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
           if (c->d.inlet == AERO_IN_RECYCLING) c->recycling_prepass((c->side + k) & 1, stream);
         for (const StepRunner::Group& g : groups)
           ck(launch_batch_step(g.q, g.coll, g.st, g.th, cases, R.tiles.as<int2>() + g.toff, g.ntiles, k, stream,
-                               timed ? cs[0]->ev_pool[2 * k] : nullptr, timed ? cs[0]->ev_pool[2 * k + 1] : nullptr),
+                               timed ? cs[0]->ev_pool[2 * k] : nullptr, timed ? cs[0]->ev_pool[2 * k + 1] : nullptr,
+                               g.les != 0),
              "step launch");
       }
     };
@@ -784,6 +799,7 @@ int aero_create(const aero_desc* desc, aero_ctx** out) {
     if (d.q == 9 && d.nz != 1) throw std::invalid_argument("D2Q9 requires nz == 1");
     if (d.nx < 3 || d.ny < 1 || d.nz < 1) throw std::invalid_argument("grid must have nx >= 3, ny >= 1, nz >= 1");
     if (d.collision < 0 || d.collision > 2) throw std::invalid_argument("unknown collision");
+    if (d.les < AERO_LES_OFF || d.les > AERO_LES_SMAGORINSKY_LOCAL) throw std::invalid_argument("unknown LES model");
     if (d.storage < 0 || d.storage > 3) throw std::invalid_argument("unknown storage");
     if (d.q == 9 && (d.inlet == AERO_IN_SEM || d.inlet == AERO_IN_RECYCLING))
       throw std::invalid_argument("SEM and recycling inlets are 3D only");
@@ -875,7 +891,7 @@ int aero_destroy(aero_ctx* c) {
   c->sem = nullptr;
   DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags, &c->wave_any,
                     &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
-                    &c->damp, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
+                    &c->damp, &c->les_c2d, &c->les_c2f, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
                     &c->gcache, &c->tmp, &c->checkpart, &c->checkout, &c->ring, &c->ring_rec};
   for (DevBuf* b : bufs) b->release();
   c->runner.release();
@@ -1049,10 +1065,26 @@ int aero_set_damping(aero_ctx* c, const double* damp) {
     c->set_device();
     if (!damp) {
       c->damp.release();
+      c->les_c2d.release();
+      c->les_c2f.release();
       return;
     }
     c->damp.alloc(sizeof(double) * c->N);
     ck(hipMemcpy(c->damp.p, damp, sizeof(double) * c->N, hipMemcpyHostToDevice), "copy damping");
+    if (c->d.les == AERO_LES_SMAGORINSKY_LOCAL) {
+      // the local step's C^2 per cell, in both arithmetic types
+      std::vector<double> c2d((size_t)c->N);
+      std::vector<float> c2f((size_t)c->N);
+      for (size_t i = 0; i < c2d.size(); ++i) {
+        const double cs = c->d.les_cs * damp[i];
+        c2d[i] = cs * cs;
+        c2f[i] = (float)c2d[i];
+      }
+      c->les_c2d.alloc(sizeof(double) * c->N);
+      c->les_c2f.alloc(sizeof(float) * c->N);
+      ck(hipMemcpy(c->les_c2d.p, c2d.data(), sizeof(double) * c->N, hipMemcpyHostToDevice), "copy C^2");
+      ck(hipMemcpy(c->les_c2f.p, c2f.data(), sizeof(float) * c->N, hipMemcpyHostToDevice), "copy C^2");
+    }
   });
 }
 
@@ -1518,7 +1550,7 @@ int aero_step_noalloc(aero_ctx* c, int32_t nsteps, int64_t step0, double* hist) 
       for (int k = 0; k < nk; ++k) {
         if (c->d.inlet == AERO_IN_RECYCLING) c->recycling_prepass((c->side + k) & 1, st);
         ck(launch_batch_step(c->Q, c->d.collision, c->d.storage, c->d.thermal != 0, d_cases, d_tiles, ntiles, k, st,
-                             nullptr, nullptr),
+                             nullptr, nullptr, c->les_local()),
            "noalloc step launch");
       }
       if (c->nrec > 0) {
@@ -1805,7 +1837,7 @@ int aero_device_bytes(aero_ctx* c, int64_t* bytes) {
   return guarded([&] {
     const DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags, &c->wave_any,
                             &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
-                            &c->damp, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
+                            &c->damp, &c->les_c2d, &c->les_c2f, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
                             &c->gcache, &c->tmp, &c->checkpart, &c->checkout, &c->ring, &c->ring_rec};
     int64_t t = 0;
     for (const DevBuf* b : bufs) t += (int64_t)b->bytes;

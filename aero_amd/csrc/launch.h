@@ -22,6 +22,10 @@ AERO_DECL_BATCH(d2q9_mrt)
 AERO_DECL_BATCH(d3q19_bgk)
 AERO_DECL_BATCH(d3q19_trt)
 AERO_DECL_BATCH(d3q19_mrt)
+AERO_DECL_BATCH(les_d2q9_bgk)
+AERO_DECL_BATCH(les_d2q9_trt)
+AERO_DECL_BATCH(les_d3q19_bgk)
+AERO_DECL_BATCH(les_d3q19_trt)
 AERO_DECL_SPLIT(d2q9_bgk)
 AERO_DECL_SPLIT(d2q9_trt)
 AERO_DECL_SPLIT(d3q19_bgk)
@@ -42,7 +46,8 @@ void sem_planes(SemGen* g, const double* eddies_host, int nk, double* planes_dev
 // and scalar transport: one launch over `ntiles` packed blocks.  Events, when
 // given, are recorded at the kernel's start and end.
 hipError_t launch_batch_step(int q, int coll, int st, bool thermal, const BatchCase* cases, const int2* tiles,
-                             int ntiles, int k, hipStream_t s, void* ev_start, void* ev_stop);
+                             int ntiles, int k, hipStream_t s, void* ev_start, void* ev_stop,
+                             bool les_local = false);
 // Split step of one case (LES finite-difference pass follows; BGK / TRT).
 hipError_t launch_split_step(int q, int coll, int st, bool thermal, const StepParams& p, hipStream_t s);
 // Last-step force reduction of many cases of one lattice: block b reduces rs[b].

@@ -25,7 +25,9 @@ enum InletMode {
 };
 enum OutletMode { OUT_NONE = 0, OUT_CONVECTIVE = 1, OUT_ZEROGRAD = 2, OUT_PRESSURE = 3 };
 enum WallMode { WALL_NONE = 0, WALL_SLIP = 1, WALL_NOSLIP = 2, WALL_MOVING = 3 };
-enum LesMode { LES_OFF = 0, LES_FD_SMAG = 1, LES_FD_WALE = 2 };
+// LES_FD_*: the reference's finite-difference models (split step); LES_LOCAL_SMAG:
+// Smagorinsky from the local non-equilibrium stress, fused into the step.
+enum LesMode { LES_OFF = 0, LES_FD_SMAG = 1, LES_FD_WALE = 2, LES_LOCAL_SMAG = 3 };
 // Per-cell flags: solid, or a fluid cell with at least one surface link.
 enum CellFlag { FLAG_SOLID = 1, FLAG_LINKS = 2, FLAG_IBM = 4 };  // FLAG_IBM: fluid cell with 0 < phi <= 1.5
 
@@ -366,6 +368,13 @@ struct StepParams {
   double mrt_s[19];
   float fsh[NSHIFT];  // storage shift of f by fsh_class (half forms): feq_i(1, (u0, 0, 0))
   double omega, trt_sm, trt_lambda, u0, u_conv;
+  // local Smagorinsky (LES_LOCAL_SMAG kernels): tau0 = 1/omega and C_s^2, or
+  // per cell (C_s d)^2 with the Van Driest factor d (les_c2d for FP64
+  // arithmetic, les_c2f for FP32; null without damping)
+  double les_tau0, les_c2;
+  const AERO_GLOBAL double* les_c2d;
+  const AERO_GLOBAL float* les_c2f;
+  int trt_variant;
   double inlet_amp, phase;
   double rho_in, rho_out, u_top, u_bot;
   double bf[3], bf_coeff;

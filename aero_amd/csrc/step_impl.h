@@ -710,11 +710,72 @@ struct MrtEq {
   static constexpr Tab T = make();
 };
 
+// Inputs of the local Smagorinsky relaxation time: C^2 = (C_s d)^2 (d the Van
+// Driest factor, 1 without damping), tau0 = 1/omega, and the TRT rule for
+// the antisymmetric rate.
+template <class C>
+struct LesLocal {
+  C c2, tau0, lam;
+  int variant;
+};
+
+// Antisymmetric TRT rate from a cell's relaxation time (kernels_trt.py:14-18
+// for variant 0; variant 1 is the magic-parameter form).
+template <class C>
+__device__ __forceinline__ C trt_sm_of(C tau, C lam, int variant) {
+  if (variant == 1) return recip(lam * recip(tau - C(0.5)) + C(0.5));
+  const C tp = lam * tau + (C(1) - lam);
+  return recip((tp * tau - C(0.5)) * recip(tp - C(0.5)));
+}
+
+// Smagorinsky closure on the local non-equilibrium stress.  With the
+// Chapman-Enskog estimate Pi^neq = -2 rho c_s^2 tau S of the pre-collision
+// populations (c_s^2 = 1/3), |S| = sqrt(2 S:S) = 3 Q / (2 rho tau) with
+// Q = sqrt(2 Pi^neq:Pi^neq), so nu = nu0 + C^2 |S| and tau = 3 nu + 1/2 give
+// tau^2 - tau0 tau - 9 C^2 Q / (2 rho) = 0, whose positive root is
+//   tau = (tau0 + sqrt(tau0^2 + 18 C^2 Q / rho)) / 2
+// (Hou, Sterling, Chen & Doolen, Fields Inst. Commun. 6 (1996) 151).
+// Pi_ab = sum_i e_ia e_ib f_i is even under i <-> opp(i), so it sums the pair
+// sums sp over one direction of each pair; Pi^eq = rho (delta/3 + u u).
+template <class L, class C>
+__device__ __forceinline__ C les_local_tau(const LesLocal<C>& la, C rho, C inv, C ux, C uy, C uz, const C sp[L::Q]) {
+  C pxx = C(0), pyy = C(0), pzz = C(0), pxy = C(0), pxz = C(0), pyz = C(0);
+#pragma unroll
+  for (int i = 1; i < L::Q; ++i) {
+    if (i > L::OPP[i]) continue;
+    const int ex = L::EX[i], ey = L::EY[i], ez = L::EZ[i];
+    if (ex != 0) pxx += sp[i];
+    if (ey != 0) pyy += sp[i];
+    if (ez != 0) pzz += sp[i];
+    if (ex * ey > 0) pxy += sp[i];
+    if (ex * ey < 0) pxy -= sp[i];
+    if (ex * ez > 0) pxz += sp[i];
+    if (ex * ez < 0) pxz -= sp[i];
+    if (ey * ez > 0) pyz += sp[i];
+    if (ey * ez < 0) pyz -= sp[i];
+  }
+  const C r3 = rho * C(1.0 / 3.0);
+  const C nxx = pxx - (r3 + rho * ux * ux), nyy = pyy - (r3 + rho * uy * uy);
+  const C nxy = pxy - rho * ux * uy;
+  C pp = nxx * nxx + nyy * nyy + C(2) * nxy * nxy;
+  if constexpr (L::DIM == 3) {
+    const C nzz = pzz - (r3 + rho * uz * uz);
+    const C nxz = pxz - rho * ux * uz, nyz = pyz - rho * uy * uz;
+    pp += nzz * nzz + C(2) * (nxz * nxz + nyz * nyz);
+  }
+  const C qn = sqrt(C(2) * pp);
+  return C(0.5) * (la.tau0 + sqrt(la.tau0 * la.tau0 + C(18) * la.c2 * qn * inv));
+}
+
 // Collision (BGK kernels.py:42-92, TRT kernels_trt.py:228-277,
 // MRT kernels_mrt.py:30-82 / kernels3d_mrt.py:158-217).  u = 0 in solids.
-template <class L, int COLL, class C>
+// LESL: BGK / TRT rates from the local Smagorinsky relaxation time of each
+// fluid cell (solids keep om, sm); MRT ignores it, as MRT ignores the
+// reference's LES field.
+template <class L, int COLL, class C, bool LESL = false>
 __device__ __forceinline__ void collide(const double* mrt_s, const double* minv_d, const float* minv_f,
-                                        bool solid, C om, C sm, const C fn[L::Q], C out[L::Q]) {
+                                        bool solid, C om, C sm, const C fn[L::Q], C out[L::Q],
+                                        const LesLocal<C>& la = LesLocal<C>{}) {
   constexpr int Q = L::Q;
   if constexpr (COLL == COLL_BGK || COLL == COLL_TRT) {
     // Opposite pairs (i, j = OPP[i], i < j): s = f_i + f_j and d = f_i - f_j
@@ -745,6 +806,13 @@ __device__ __forceinline__ void collide(const double* mrt_s, const double* minv_
       ux = C(0);
       uy = C(0);
       uz = C(0);
+    }
+    if constexpr (LESL) {
+      if (!solid) {
+        const C tau = les_local_tau<L, C>(la, rho, inv, ux, uy, uz, sp);
+        om = recip(tau);
+        if constexpr (COLL == COLL_TRT) sm = trt_sm_of<C>(tau, la.lam, la.variant);
+      }
     }
     const C base = C(1) - C(1.5) * (ux * ux + uy * uy + uz * uz);
     // feq_i = even + odd with even = w rho (base + 4.5 eu^2), odd = 3 w rho eu
@@ -911,9 +979,9 @@ __device__ __forceinline__ void collide(const double* mrt_s, const double* minv_
 }
 
 // ---------------------------------------------------------------- kernels
-// Tail of a step for one cell: collide into fout (KMODE 0), or -- split step
-// for the LES finite-difference pass -- write the post-boundary populations
-// and velocity (KMODE 1).
+// Tail of a step for one cell: collide into fout (KMODE 0; KMODE 2 with the
+// local Smagorinsky rates), or -- split step for the LES finite-difference
+// pass -- write the post-boundary populations and velocity (KMODE 1).
 // `sb` is the storage element of the wave's first cell: its row's base plus
 // the wave's first x (wave-uniform), so a lane's cell sits at sb + lane.
 template <class L, int COLL, int ST, int KMODE>
@@ -924,9 +992,20 @@ __device__ __forceinline__ void finish_cell(const StepParams& p, int cell, size_
   using T = typename S::T;
   constexpr int Q = L::Q;
   const size_t N = (size_t)p.ncell;
-  if constexpr (KMODE == 0) {
+  if constexpr (KMODE == 0 || KMODE == 2) {
     C out[Q];
-    collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, (C)p.omega, (C)p.trt_sm, fn, out);
+    if constexpr (KMODE == 2) {
+      C c2 = (C)p.les_c2;
+      if constexpr (sizeof(C) == 8) {
+        if (p.les_c2d) c2 = (C)p.les_c2d[cell];
+      } else {
+        if (p.les_c2f) c2 = (C)p.les_c2f[cell];
+      }
+      const LesLocal<C> la{c2, (C)p.les_tau0, (C)p.trt_lambda, p.trt_variant};
+      collide<L, COLL, C, true>(p.mrt_s, p.minv_d, p.minv_f, solid, (C)p.omega, (C)p.trt_sm, fn, out, la);
+    } else {
+      collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, (C)p.omega, (C)p.trt_sm, fn, out);
+    }
     // the store offset is the lane index, rebuilt here rather than held in a
     // register across the whole step; the run bases are scalar
     const unsigned lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -1038,10 +1117,14 @@ __device__ __forceinline__ void reduce_forces_block(const ForceReduce& r) {
 // 5%; likewise D2Q9 MRT (+5%).  D3Q19 MRT in the 2-byte forms
 // targets 6 (the compiler's own choice, 5, measured 10% slower)
 // unless it carries the scalar too (6 waves spill there: 16% slower); FP64,
-// FP32 MRT and MRT with the scalar keep the compiler's choice.
-template <class L, int COLL, int ST, bool THERMAL>
+// FP32 MRT and MRT with the scalar keep the compiler's choice.  The D3Q19
+// local-Smagorinsky kernels (KMODE 2) without the scalar target 6: at 512 x
+// 256 x 256, 0.852 vs 0.888 ms (FP32) and 0.513 vs 0.566 ms (FP16) at 7,
+// while the plain kernels stay fastest at 7 (0.832 vs 0.845, 0.463 vs 0.485).
+template <class L, int COLL, int ST, bool THERMAL, int KMODE = 0>
 struct StepWaves {
-  static constexpr int value = (ST != ST_F64 && COLL != COLL_MRT)
+  static constexpr int value = (KMODE == 2 && L::Q == 19 && ST != ST_F64 && !THERMAL) ? 6
+                              : (ST != ST_F64 && COLL != COLL_MRT)
                                   ? ((L::Q == 9 && (ST == ST_F16S || ST == ST_BF16S)) ? 8 : 7)
                               : (COLL == COLL_MRT && (ST == ST_F16S || ST == ST_BF16S) && !THERMAL) ? (L::Q == 9 ? 8 : 6)
                                                                                                     : 1;
@@ -1139,8 +1222,9 @@ void step_kernel(StepParams p) {
 // ping-pong buffers swap every step); the per-step fields are derived here.
 // A single case runs through this same kernel as a batch of one, so a case
 // computes the same bits alone or among others.
-template <class L, int COLL, int ST, bool THERMAL>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(StepWaves<L, COLL, ST, THERMAL>::value)))
+// KMODE 0: the plain collision; 2: with the local Smagorinsky rates.
+template <class L, int COLL, int ST, bool THERMAL, int KMODE = 0>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(StepWaves<L, COLL, ST, THERMAL, KMODE>::value)))
 void batch_step_kernel(const BatchCase* __restrict__ cases, const int2* __restrict__ tiles, int k) {
   const int2 t = tiles[blockIdx.x];
   const BatchCase& bc = cases[t.x];
@@ -1149,18 +1233,18 @@ void batch_step_kernel(const BatchCase* __restrict__ cases, const int2* __restri
                     bc.hist + (size_t)NCOEF * (k > 0 ? k - 1 : 0), k == 0 ? bc.first_g_skip : 0,
                     (k > 0 && p.nrec > 0) ? 1 : 0};
   const int blk = t.y;
-  AERO_STEP_BODY(L, COLL, ST, THERMAL, 0);
+  AERO_STEP_BODY(L, COLL, ST, THERMAL, KMODE);
 }
 
-template <class L, int COLL, int ST, bool THERMAL>
+template <class L, int COLL, int ST, bool THERMAL, int KMODE = 0>
 hipError_t launch_batch(const BatchCase* cases, const int2* tiles, int ntiles, int k, hipStream_t s, void* ev_start,
                         void* ev_stop) {
   const dim3 grid((unsigned)ntiles);
   if (ev_start)
-    hipExtLaunchKernelGGL(batch_step_kernel<L, COLL, ST, THERMAL>, grid, dim3(BLOCK), 0, s, (hipEvent_t)ev_start,
-                          (hipEvent_t)ev_stop, 0, cases, tiles, k);
+    hipExtLaunchKernelGGL(batch_step_kernel<L, COLL, ST, THERMAL, KMODE>, grid, dim3(BLOCK), 0, s,
+                          (hipEvent_t)ev_start, (hipEvent_t)ev_stop, 0, cases, tiles, k);
   else
-    batch_step_kernel<L, COLL, ST, THERMAL><<<grid, BLOCK, 0, s>>>(cases, tiles, k);
+    batch_step_kernel<L, COLL, ST, THERMAL, KMODE><<<grid, BLOCK, 0, s>>>(cases, tiles, k);
   return hipGetLastError();
 }
 
@@ -1375,22 +1459,29 @@ __global__ __launch_bounds__(BLOCK) void collide_kernel(CollideParams p) {
   for (int i = 0; i < Q; ++i) fn[i] = F[(size_t)i * N + cell];
   const bool solid = p.flags[cell] & FLAG_SOLID;
   C om = (C)p.omega, sm = (C)p.trt_sm;
-  if (p.les != LES_OFF && COLL != COLL_MRT) {
-    const int x = cell % p.nx, yz = cell / p.nx, y = yz % p.ny, z = yz / p.ny;
-    om = les_omega_fd<C>(p, (const C*)p.U, cell, x, y, z, solid);
-    if (COLL == COLL_TRT) {  // per-cell s- from the local omega (kernels_trt.py:14-18)
-      const C tau = C(1) / om;
-      const C lam = (C)p.trt_lambda;
-      if (p.trt_variant == 1) {
-        sm = C(1) / (lam / (tau - C(0.5)) + C(0.5));
-      } else {
-        const C tp = lam * tau + (C(1) - lam);
-        const C tm = (tp * tau - C(0.5)) / (tp - C(0.5));
-        sm = C(1) / tm;
+  if (p.les == LES_LOCAL_SMAG && COLL != COLL_MRT) {
+    // the first collision of a state set from the host, as the fused local step does it
+    const double cs = p.damp ? p.les_cs * p.damp[cell] : p.les_cs;  // C^2 rounded as the step's table
+    const LesLocal<C> la{(C)(cs * cs), (C)(1.0 / p.omega), (C)p.trt_lambda, p.trt_variant};
+    collide<L, COLL, C, true>(p.mrt_s, p.minv_d, p.minv_f, solid, om, sm, fn, out, la);
+  } else {
+    if (p.les != LES_OFF && COLL != COLL_MRT) {
+      const int x = cell % p.nx, yz = cell / p.nx, y = yz % p.ny, z = yz / p.ny;
+      om = les_omega_fd<C>(p, (const C*)p.U, cell, x, y, z, solid);
+      if (COLL == COLL_TRT) {  // per-cell s- from the local omega (kernels_trt.py:14-18)
+        const C tau = C(1) / om;
+        const C lam = (C)p.trt_lambda;
+        if (p.trt_variant == 1) {
+          sm = C(1) / (lam / (tau - C(0.5)) + C(0.5));
+        } else {
+          const C tp = lam * tau + (C(1) - lam);
+          const C tm = (tp * tau - C(0.5)) / (tp - C(0.5));
+          sm = C(1) / tm;
+        }
       }
     }
+    collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, om, sm, fn, out);
   }
-  collide<L, COLL, C>(p.mrt_s, p.minv_d, p.minv_f, solid, om, sm, fn, out);
   const unsigned row = (unsigned)cell / (unsigned)p.nx;
   T* Bo = (T*)p.fout + (size_t)row * (unsigned)p.qxp;
   const unsigned x = (unsigned)cell - row * (unsigned)p.nx;

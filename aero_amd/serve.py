@@ -304,6 +304,18 @@ class Server:
                       float(np.mean(cd[-w:])), float(np.std(cl[-w:])), st, lv.warnings,
                       cd.tolist() if self.keep_history else [])
 
+    def _join_built(self, building: list, live: list, clock, t0: float) -> None:
+        """Move finished builds into the live set (with their history ring)."""
+        for b in [b for b in building if b[0].done()]:
+            building.remove(b)
+            fut, arr, rid, spec, _ = b
+            solver = fut.result()
+            if self.device_stats and hasattr(solver, "_ctx"):
+                from .stationarity import enable_ring
+                w = max(self.stop_kw["convergence_window"], self.stop_kw["strouhal_window"])
+                enable_ring(solver, w + self.chunk)  # room for the chunk stepped during the check
+            live.append(_Live(rid, spec, solver, arr, clock() - t0))
+
     def _stability(self, lv: _Live) -> None:
         """check_stability on the case's state at the end of the last chunk,
         as the reference's run loop does every check_every steps."""
@@ -339,6 +351,9 @@ class Server:
         builders = ThreadPoolExecutor(max_workers=self.build_workers)
         try:
             while waiting or live or building:
+                # finished builds join first, so their memory is counted once
+                # (in the device's free memory) rather than also as pending
+                self._join_built(building, live, clock, t0)
                 now = clock() - t0
                 free = None
                 pending = sum(b[4] for b in building)
@@ -352,15 +367,7 @@ class Server:
                     waiting.pop(0)
                     building.append((builders.submit(spec.build), arr, rid, spec, need))
                     pending += need
-                for b in [b for b in building if b[0].done()]:
-                    building.remove(b)
-                    fut, arr, rid, spec, _ = b
-                    solver = fut.result()
-                    if self.device_stats and hasattr(solver, "_ctx"):
-                        from .stationarity import enable_ring
-                        w = max(self.stop_kw["convergence_window"], self.stop_kw["strouhal_window"])
-                        enable_ring(solver, w + self.chunk)  # room for the chunk stepped during the check
-                    live.append(_Live(rid, spec, solver, arr, clock() - t0))
+                self._join_built(building, live, clock, t0)
                 if not live:
                     if building:
                         wait([b[0] for b in building], return_when=FIRST_COMPLETED)

@@ -16,7 +16,7 @@ from typing import Callable, Optional
 import numpy as np
 
 from . import _abi
-from ._base import (DeviceSolver, checkpoint_params, extra_params_entry, resolve_backend, run_loop,
+from ._base import (DeviceSolver, checkpoint_params, extra_params_entry, les_code, resolve_backend, run_loop,
                     warn_hdf5)
 from .diagnostics import coefficient_spectrum, detect_statistical_stationarity
 from .inflow import SyntheticInflow2D
@@ -41,7 +41,7 @@ class Solver(DeviceSolver):
                  thermal: bool = False, T_hot: float = 1.0, T_cold: float = 0.0, alpha_T: float = 1e-3,
                  buoyancy: bool = False, g_gravity: float = 0.0, beta: float = 1e-3, T_ref: float = 0.5,
                  storage: str = "f64", trt_variant: str = "reference",
-                 bouzidi_variant: str = "reference"):
+                 bouzidi_variant: str = "reference", les_form: str = "fd"):
         """Arguments as the reference (solver.py:75-142), plus:
 
         storage     -- population format on the device: "f64" (reference
@@ -55,7 +55,14 @@ class Solver(DeviceSolver):
                        bounce-back (q = phi at the fluid cell, boundary.py:303-352);
                        "linear" is the Bouzidi-Firdaouss-Lallemand rule with q the
                        link fraction from ``phi`` sampled at the integer cell
-                       coordinates (e.g. ``geometry.sdf_field(..., offset=0.0)``)."""
+                       coordinates (e.g. ``geometry.sdf_field(..., offset=0.0)``).
+        les_form    -- "fd" (the reference's finite-difference Smagorinsky /
+                       WALE, les.py:273-344, a split step) or "local":
+                       Smagorinsky from each cell's non-equilibrium stress,
+                       fused into the step, tau = (tau0 + sqrt(tau0^2 +
+                       18 C^2 |Pi^neq| / rho)) / 2 with C = les_cs times the
+                       Van Driest factor; BGK and TRT (MRT ignores LES, as in
+                       the reference)."""
         self.Ny, self.Nx = int(Ny), int(Nx)
         self.solid = np.ascontiguousarray(solid, dtype=np.bool_)
         if self.solid.shape != (self.Ny, self.Nx):
@@ -71,6 +78,8 @@ class Solver(DeviceSolver):
         if les_model not in ("smagorinsky", "wale"):
             raise ValueError("les_model must be 'smagorinsky' or 'wale'.")
         self.les_model = str(les_model)
+        self._les_code = les_code(self.les, self.les_model, les_form)
+        self.les_form = les_form
         self.ibm_enabled = bool(ibm_enabled)
         self.phi = phi
         self.bouzidi = bool(bouzidi)
@@ -130,8 +139,7 @@ class Solver(DeviceSolver):
         d.bouzidi = 0 if not self.bouzidi else (2 if self.bouzidi_variant == "linear" else 1)
         d.trt_variant = 1 if self.trt_variant == "lambda" else 0
         d.sponge_thickness = self.sponge_thickness
-        d.les = (_abi.LES_OFF if not self.les else
-                 (_abi.LES_WALE if self.les_model == "wale" else _abi.LES_SMAGORINSKY))
+        d.les = self._les_code
         d.thermal, d.buoyancy, d.body_force_on = int(self.thermal), int(self.buoyancy), 0
         d.omega, d.u0, d.rho0, d.D, d.trt_lambda = self.omega, self.u0, self.rho0, self.D, self.trt_lambda
         d.rho_in, d.rho_out, d.inlet_perturbation = self.rho_in, self.rho_out, self.inlet_perturbation
@@ -293,7 +301,8 @@ class Solver(DeviceSolver):
             collision=self.collision, backend="auto")
 
     def _extra_params(self) -> dict:
-        return dict(storage=self.storage, trt_variant=self.trt_variant, bouzidi_variant=self.bouzidi_variant)
+        return dict(storage=self.storage, trt_variant=self.trt_variant, bouzidi_variant=self.bouzidi_variant,
+                    les_form=self.les_form)
 
     def save_checkpoint(self, path: str) -> None:
         """Same keys and layout as the reference (solver.py:638-686)."""

@@ -15,7 +15,7 @@ from typing import Callable, List, Optional
 import numpy as np
 
 from . import _abi
-from ._base import (DeviceSolver, checkpoint_params, extra_params_entry, resolve_backend, run_loop,
+from ._base import (DeviceSolver, checkpoint_params, extra_params_entry, les_code, resolve_backend, run_loop,
                     warn_hdf5)
 from .diagnostics import coefficient_spectrum, detect_statistical_stationarity
 from .inflow import SEMInlet, SyntheticInflow3D
@@ -41,9 +41,9 @@ class Solver3D(DeviceSolver):
                  sem_L_int: float = 10.0, sem_N: int = 200, thermal: bool = False, T_hot: float = 1.0,
                  T_cold: float = 0.0, alpha_T: float = 1e-3, buoyancy: bool = False, g_gravity: float = 0.0,
                  beta: float = 1e-3, T_ref: float = 0.5, storage: str = "f64", trt_variant: str = "reference",
-                 bouzidi_variant: str = "reference"):
+                 bouzidi_variant: str = "reference", les_form: str = "fd"):
         """Arguments as the reference (solver3d.py:71-135) plus ``storage``,
-        ``trt_variant`` and ``bouzidi_variant`` (see Solver)."""
+        ``trt_variant``, ``bouzidi_variant`` and ``les_form`` (see Solver)."""
         self.Nz, self.Ny, self.Nx = int(Nz), int(Ny), int(Nx)
         self.solid = np.ascontiguousarray(solid, dtype=np.bool_)
         if self.solid.shape != (self.Nz, self.Ny, self.Nx):
@@ -61,6 +61,8 @@ class Solver3D(DeviceSolver):
         if les_model not in ("smagorinsky", "wale"):
             raise ValueError("les_model must be 'smagorinsky' or 'wale'.")
         self.les_model = str(les_model)
+        self._les_code = les_code(self.les, self.les_model, les_form)
+        self.les_form = les_form
         self.ibm_enabled = bool(ibm_enabled)
         self.phi = phi
         self.bouzidi = bool(bouzidi)
@@ -123,8 +125,7 @@ class Solver3D(DeviceSolver):
         d.bouzidi = 0 if not self.bouzidi else (2 if self.bouzidi_variant == "linear" else 1)
         d.trt_variant = 1 if self.trt_variant == "lambda" else 0
         d.sponge_thickness = self.sponge_thickness
-        d.les = (_abi.LES_OFF if not self.les else
-                 (_abi.LES_WALE if self.les_model == "wale" else _abi.LES_SMAGORINSKY))
+        d.les = self._les_code
         d.thermal, d.buoyancy = int(self.thermal), int(self.buoyancy)
         d.body_force_on = int(any(abs(v) > 0.0 for v in (self.body_force_x, self.body_force_y, self.body_force_z)))
         d.body_force[0], d.body_force[1], d.body_force[2] = self.body_force_x, self.body_force_y, self.body_force_z
@@ -319,7 +320,8 @@ class Solver3D(DeviceSolver):
             buoyancy=self.buoyancy, g_gravity=self.g_gravity, beta=self.beta, T_ref=self.T_ref)
 
     def _extra_params(self) -> dict:
-        return dict(storage=self.storage, trt_variant=self.trt_variant, bouzidi_variant=self.bouzidi_variant)
+        return dict(storage=self.storage, trt_variant=self.trt_variant, bouzidi_variant=self.bouzidi_variant,
+                    les_form=self.les_form)
 
     def save_checkpoint(self, path: str) -> None:
         """Same keys and layout as the reference (solver3d.py:727-783)."""

@@ -38,7 +38,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 
-def make_case(nx, ny, nz, radius, cx, storage, steps_hint=0, seed=0):
+def make_case(nx, ny, nz, radius, cx, storage, steps_hint=0, seed=0, **extra):
     import aero_amd
     from aero_amd.geometry import Sphere
     from aero_amd.lattice import compute_feq_3d
@@ -49,7 +49,7 @@ def make_case(nx, ny, nz, radius, cx, storage, steps_hint=0, seed=0):
     omega = 1.0 / (3.0 * u0 * D / re + 0.5)
     s = aero_amd.Solver3D(Nz=nz, Ny=ny, Nx=nx, solid=solid, omega=omega, u0=u0, D=D, phi=phi, bouzidi=True,
                           collision="trt", trt_lambda=0.1875, trt_variant="lambda", bouzidi_variant="linear",
-                          storage=storage)
+                          storage=storage, **extra)
     rng = np.random.default_rng(seed)
     sh = (nz, ny, nx)
     f0 = compute_feq_3d(np.ones(sh), np.full(sh, u0), rng.uniform(-1e-4, 1e-4, sh), rng.uniform(-1e-4, 1e-4, sh))
@@ -208,6 +208,10 @@ def run_form(name, steps, warmup):
 
 
 LES_FORMS = {"mrt_les_scalar_512_f32": "f32", "mrt_les_scalar_512_f16": "f16"}
+# the TRT sphere with Smagorinsky LES (C_s = 0.16): the reference's
+# finite-difference form (split step) and the fused local-stress form
+TRT_LES_FORMS = {"trt_les_fd_512_f32": ("fd", "f32"), "trt_les_local_512_f32": ("local", "f32"),
+                 "trt_les_local_512_f16": ("local", "f16")}
 CYL2D_FORMS = {"cylinder_2d_400x200_f64": "f64", "cylinder_2d_400x200_f32": "f32",
                "cylinder_2d_400x200_f16": "f16"}
 
@@ -267,6 +271,27 @@ def run_les_form(name, steps, warmup):
     return {"form": name, "MLUPS_wall": round(cells * steps / w / 1e6, 2),
             "MLUPS_kernel": round(cells / (k * 1e-3) / 1e6, 2), "kernel_ms": round(k, 5),
             "roofline_frac": round(bpl / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "cd_last": float(rows[-1, 0])}
+
+
+def run_trt_les_form(name, steps, warmup):
+    """The 512x256x256 TRT sphere with Smagorinsky LES.  "fd" is the
+    reference's finite-difference model: each step writes the post-boundary
+    populations and velocity in compute precision, then a second kernel
+    collides with the eddy viscosity from central differences.  "local" is one
+    fused launch with the eddy viscosity from each cell's non-equilibrium
+    stress, so its algorithmic bytes are the plain step's, 2 * 19 * s per
+    cell; the figure for "fd" is the wall rate only."""
+    nx, ny, nz = 512, 256, 256
+    form, st = TRT_LES_FORMS[name]
+    s, _ = make_case(nx, ny, nz, 32.0, 128.0, st, les=True, les_cs=0.16, les_form=form)
+    w, k, rows = timed(s, steps, warmup)
+    cells = nx * ny * nz
+    s.close()
+    out = {"form": name, "MLUPS": round(cells * steps / w / 1e6, 2), "cd_last": float(rows[-1, 0])}
+    if form == "local":
+        bpl = 2 * 19 * {"f32": 4, "f16": 2}[st] * cells
+        out.update(kernel_ms=round(k, 5), roofline_frac=round(bpl / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+    return out
 
 
 def stl_geometry(nz=128, ny=128, nx=256, sample=4000):
@@ -404,7 +429,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving / batched-solves leg")
     ap.add_argument("--serve-full", action="store_true",
                     help="serving leg on the full 256-case workload and the full arrival-rate sweep")
-    ap.add_argument("--form", choices=sorted(FORMS) + sorted(LES_FORMS) + sorted(CYL2D_FORMS), default=None,
+    ap.add_argument("--form", choices=sorted(FORMS) + sorted(LES_FORMS) + sorted(CYL2D_FORMS) + sorted(TRT_LES_FORMS),
+                    default=None,
                     help="only time one storage form of the TRT sphere (prints one JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (calibrates the HBM byte counters)")
@@ -422,6 +448,9 @@ def main():
         return
     if a.form in LES_FORMS:
         print(json.dumps(run_les_form(a.form, a.steps, a.warmup)), flush=True)
+        return
+    if a.form in TRT_LES_FORMS:
+        print(json.dumps(run_trt_les_form(a.form, a.steps, a.warmup)), flush=True)
         return
     if a.form:
         print(json.dumps(run_form(a.form, a.steps, a.warmup)), flush=True)

@@ -61,6 +61,12 @@ extern "C" {
 #define AERO_LES_OFF 0
 #define AERO_LES_SMAGORINSKY 1
 #define AERO_LES_WALE 2
+/* Smagorinsky eddy viscosity from the local non-equilibrium stress of each
+ * cell, fused into the step (BGK / TRT; MRT ignores LES as in the reference).
+ * No counterpart in the reference: tau = (tau0 + sqrt(tau0^2 + 18 C^2 |Pi^neq| / rho)) / 2
+ * with |Pi^neq| = sqrt(2 Pi^neq:Pi^neq), C = les_cs times the Van Driest
+ * factor of aero_set_damping.  Such contexts batch with aero_batch_step. */
+#define AERO_LES_SMAGORINSKY_LOCAL 3
 
 /* Number of doubles per step in the coefficient history written by aero_step:
  * Cd, Cl_y, Cl_z, Cm_x, Cm_y, Cm_z, Cd_p, Cd_v, then the raw FP64 sums

@@ -587,6 +587,49 @@ int orc_les_omega(int dim, int nz, int ny, int nx, const double *ux, const doubl
     return 0;
 }
 
+/* Local Smagorinsky relaxation field from the populations f (Q x N) about to
+   be collided (no reference counterpart: the relaxation-time closure of Hou,
+   Sterling, Chen & Doolen 1996).  Per fluid cell, with Pi_ab = sum_i e_ia e_ib
+   f_i, Pi^neq = Pi - rho (delta_ab / 3 + u_a u_b),
+   Qn = sqrt(2 sum_ab Pi^neq_ab^2) and C = cs (times damp[c] when given):
+       tau = (tau0 + sqrt(tau0^2 + 18 C^2 Qn / rho)) / 2,  omega_out = 1 / tau.
+   Solid cells get omega.  The closure solves tau = tau0 + 3 C^2 |S| with the
+   Chapman-Enskog strain |S| = 3 Qn / (2 rho tau) of the same populations. */
+int orc_les_local_omega(int Q, int nz, int ny, int nx, const double *f, const uint8_t *solid,
+                        const int *ex, const int *ey, const int *ez, double omega, double cs,
+                        const double *damp, double *omega_out) {
+    const size_t N = (size_t)nz * ny * nx;
+    const double tau0 = 1.0 / omega;
+#pragma omp parallel for schedule(static)
+    for (long long c = 0; c < (long long)N; ++c) {
+        if (solid[c]) { omega_out[c] = omega; continue; }
+        double rho = 0.0, m[3] = {0.0, 0.0, 0.0}, P[3][3] = {{0.0}};
+        for (int i = 0; i < Q; ++i) {
+            const double fi = f[(size_t)i * N + c];
+            const int e[3] = {ex[i], ey[i], ez[i]};
+            rho += fi;
+            for (int a = 0; a < 3; ++a) {
+                m[a] += e[a] * fi;
+                for (int b = 0; b < 3; ++b) P[a][b] += e[a] * e[b] * fi;
+            }
+        }
+        const double inv = rho > 0.0 ? 1.0 / rho : 0.0;
+        const double u[3] = {m[0] * inv, m[1] * inv, m[2] * inv};
+        const int dim = Q == 9 ? 2 : 3;
+        double pp = 0.0;
+        for (int a = 0; a < dim; ++a)
+            for (int b = 0; b < dim; ++b) {
+                const double v = P[a][b] - rho * ((a == b ? 1.0 / 3.0 : 0.0) + u[a] * u[b]);
+                pp += v * v;
+            }
+        const double C = damp ? cs * damp[c] : cs;
+        const double qn = sqrt(2.0 * pp);
+        const double tau = 0.5 * (tau0 + sqrt(tau0 * tau0 + 18.0 * C * C * qn * inv));
+        omega_out[c] = 1.0 / tau;
+    }
+    return 0;
+}
+
 /* Passive scalar + Boussinesq step (solver.py:427-443, solver3d.py:480-503,
    thermal.py): f gets the buoyancy force, g is collided, streamed and given
    the y-wall Dirichlet values.  gout must not alias g. */

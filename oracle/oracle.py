@@ -18,7 +18,11 @@ Pinning (see tests/test_oracle.py):
   sponge, walls, LES (Smagorinsky, WALE, Van Driest), thermal + buoyancy,
   SEM, synthetic inflow, recycling and body force;
 * lattice tables and feq against closed-form moments; MRT inverses against
-  numpy's inv/pinv; links against brute force on small grids.
+  numpy's inv/pinv; links against brute force on small grids;
+* the local Smagorinsky closure (no reference counterpart) against the
+  Chapman-Enskog identity: populations built as feq + f^neq with
+  Pi^neq = -2 rho c_s^2 tau* S for a chosen strain S must return exactly
+  tau* = tau0 + 3 C^2 |S|, and C = 0 must return tau0.
 """
 
 from __future__ import annotations
@@ -226,7 +230,7 @@ class OracleSolver:
                  wall_velocity_top=0.0, wall_velocity_bottom=0.0, inlet_mode="zouhe",
                  thermal=False, T_hot=1.0, T_cold=0.0, alpha_T=1e-3, buoyancy=False,
                  g_gravity=0.0, beta=1e-3, T_ref=0.5, links=None, q_vals=None, trt_variant="reference",
-                 bouzidi_variant="reference", ibm=False):
+                 bouzidi_variant="reference", ibm=False, les_form="fd"):
         self.is3d = solid.ndim == 3
         self.solid = np.ascontiguousarray(solid, dtype=bool)
         s3 = self.solid if self.is3d else self.solid[None]
@@ -250,6 +254,7 @@ class OracleSolver:
         self.sponge_on = int(sponge_thickness) > 0
         self.les, self.les_cs = bool(les), float(les_cs)
         self.les_model = {"smagorinsky": 0, "wale": 1}[les_model]
+        self.les_form = {"fd": 0, "local": 1}[les_form]
         self.phi = phi
         # Guo IBM (solver.py:280-285, 379-383): forcing of band cells before the
         # collision when phi is given; the surface links never bounce back
@@ -349,14 +354,21 @@ class OracleSolver:
         return rho.reshape(sh), ux.reshape(sh), uy.reshape(sh), uz.reshape(sh)
 
     def omega_field(self, f):
-        """LES relaxation field from the pre-collision state (les.py:273-344)."""
+        """LES relaxation field from the pre-collision state (les.py:273-344),
+        or, for les_form "local", the local Smagorinsky closure on the
+        non-equilibrium stress of f (orc_les_local_omega)."""
         Q, nz, ny, nx = self._dims()
-        _, ux, uy, uz = self.macroscopic(f)
         damp = None
         if self.van_driest and self.phi is not None:
             ph = self.phi if self.is3d else self.phi[None]
             damp = _f64((1.0 - np.exp(-np.abs(ph) / self.van_driest_A)) ** 2)
         om = np.empty((nz, ny, nx))
+        if self.les_form == 1:
+            lib().orc_les_local_omega(Q, nz, ny, nx, _p(_f64(f)), _p(self.solid_u8), _p(self.ex), _p(self.ey),
+                                      _p(self.ez), ctypes.c_double(self.omega), ctypes.c_double(self.les_cs),
+                                      _p(damp), _p(om))
+            return om
+        _, ux, uy, uz = self.macroscopic(f)
         lib().orc_les_omega(3 if self.is3d else 2, nz, ny, nx, _p(ux), _p(uy), _p(uz),
                             _p(self.solid_u8), ctypes.c_double(self.base_nu),
                             ctypes.c_double(self.les_cs), self.les_model, _p(damp), _p(om))

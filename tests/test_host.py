@@ -335,3 +335,18 @@ def test_synthetic_inflow_chunks_equal_single_steps():
     y2, z2 = b3.next_n(7)
     np.testing.assert_array_equal(np.concatenate([y1, y2]), np.array([s[0] for s in steps]))
     np.testing.assert_array_equal(np.concatenate([z1, z2]), np.array([s[1] for s in steps]))
+
+
+def test_les_form_selects_the_device_mode():
+    """les_form "fd" keeps the reference's models; "local" is Smagorinsky
+    only (WALE needs the rotation part of the velocity gradient)."""
+    from aero_amd import _abi
+    from aero_amd._base import les_code
+    assert les_code(False, "smagorinsky", "local") == _abi.LES_OFF
+    assert les_code(True, "smagorinsky", "fd") == _abi.LES_SMAGORINSKY
+    assert les_code(True, "wale", "fd") == _abi.LES_WALE
+    assert les_code(True, "smagorinsky", "local") == _abi.LES_SMAGORINSKY_LOCAL == 3
+    with pytest.raises(ValueError, match="Smagorinsky only"):
+        les_code(True, "wale", "local")
+    with pytest.raises(ValueError, match="les_form"):
+        les_code(True, "smagorinsky", "lattice")

@@ -33,7 +33,7 @@ def test_d3q19_trt_batch_kernels_keep_their_register_budget(tmp_path):
     for b in blocks:
         name = b.split()[0]
         # batch_step_kernel<D3Q19, TRT, storage, thermal>: mangled ...ILi1ELi<ST>ELb<TH>E...
-        m = re.search(r"batch_step_kernelINS_5D3Q19ELi1ELi(\d)ELb([01])E", name)
+        m = re.search(r"batch_step_kernelINS_5D3Q19ELi1ELi(\d)ELb([01])ELi0E", name)
         if not m or m.group(1) == "0":  # FP64 keeps the compiler's own choice
             continue
         vgpr = int(re.search(r"VGPRs: (\d+)", b).group(1))
@@ -41,4 +41,29 @@ def test_d3q19_trt_batch_kernels_keep_their_register_budget(tmp_path):
         seen += 1
         assert vgpr <= 72, (name, vgpr)
         assert scratch <= 32, (name, scratch)
+    assert seen == 6
+
+
+@pytest.mark.skipif(shutil.which("hipcc") is None and not pathlib.Path("/opt/rocm/bin/hipcc").exists(),
+                    reason="hipcc not available")
+def test_d3q19_trt_local_les_kernels_keep_their_register_budget(tmp_path):
+    """The local-Smagorinsky D3Q19 TRT kernels: FP32 without the scalar fits
+    6 waves with no scratch; every 4- and 2-byte form stays within 80 VGPRs
+    and 80 bytes of scratch per lane."""
+    from aero_amd import build as B
+    cmd = [B.HIPCC, *B.FLAGS, "-c", str(CSRC / "step_d3q19_trt_les.hip"), "-o", str(tmp_path / "t.o"),
+           "-Rpass-analysis=kernel-resource-usage"]
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr[-2000:]
+    seen = 0
+    for b in re.split(r"remark: Function Name: ", out.stderr)[1:]:
+        m = re.search(r"batch_step_kernelINS_5D3Q19ELi1ELi(\d)ELb([01])ELi2E", b.split()[0])
+        if not m or m.group(1) == "0":
+            continue
+        vgpr = int(re.search(r"VGPRs: (\d+)", b).group(1))
+        scratch = int(re.search(r"ScratchSize \[bytes/lane\]: (\d+)", b).group(1))
+        seen += 1
+        assert vgpr <= 80 and scratch <= 80, (m.groups(), vgpr, scratch)
+        if m.groups() == ("1", "0"):
+            assert scratch == 0, (vgpr, scratch)
     assert seen == 6
