@@ -440,17 +440,33 @@ hipError_t launch_check_f64(int q, int nx, int ny, int nz, int zc, const double*
 // Momentum-exchange totals of one step in the same fixed order the step
 // kernel's block 0 uses (reduce_forces_block), for the last step of a chunk.
 template <class L>
-__global__ __launch_bounds__(BLOCK) void reduce_forces_kernel(ForceReduce r) { reduce_forces_block<L>(r); }
+__global__ __launch_bounds__(BLOCK) void reduce_forces_kernel(ForceReduce r) { reduce_forces_block<L, 8>(r); }
 
 template <class L>
 __global__ __launch_bounds__(BLOCK) void reduce_forces_many_kernel(const ForceReduce* rs) {
-  reduce_forces_block<L>(rs[blockIdx.x]);
+  reduce_forces_block<L, 8>(rs[blockIdx.x]);
 }
 
 hipError_t launch_reduce_forces_many(int q, const ForceReduce* rs, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (q == 9) reduce_forces_many_kernel<D2Q9><<<n, BLOCK, 0, s>>>(rs);
   else reduce_forces_many_kernel<D3Q19><<<n, BLOCK, 0, s>>>(rs);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(BLOCK) void rows_to_host_kernel(const double* __restrict__ rows, double* host, int n,
+                                                             int ch, int nk) {
+  const long long per = (long long)nk * NCOEF;
+  const long long e = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= per * n) return;
+  const long long i = e / per, r = e - i * per;
+  host[i * ch * NCOEF + r] = rows[i * ch * NCOEF + r];
+}
+
+hipError_t launch_rows_to_host(const double* rows, double* host, int n, int ch, int nk, hipStream_t s) {
+  const long long total = (long long)n * nk * NCOEF;
+  if (total <= 0) return hipSuccess;
+  rows_to_host_kernel<<<(unsigned)((total + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(rows, host, n, ch, nk);
   return hipGetLastError();
 }
 

@@ -742,9 +742,8 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
       for (aero_ctx* c : cs)
         if (c->ring_cap > 0) c->ring_count += nk;
     }
-    if (any_hist)
-      ck(hipMemcpyAsync(R.stage[b], rows, sizeof(double) * NCOEF * CH * (size_t)n, hipMemcpyDeviceToHost, stream),
-         "download rows");
+    if (any_hist)  // the chunk's rows of every case, stored by a kernel into the pinned staging buffer
+      ck(launch_rows_to_host(rows, R.stage[b], n, CH, nk, stream), "download rows");
     ck(hipEventRecord(R.ev[b], stream), "chunk event");
     pend_off[b] = s0;
     pend_n[b] = nk;
@@ -1421,9 +1420,7 @@ int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, 
         c->ring_count += n;
       }
       if (hist) {
-        ck(hipMemcpyAsync(c->hist_stage[b], rows, sizeof(double) * NCOEF * (size_t)n, hipMemcpyDeviceToHost,
-                          c->stream),
-           "download history");
+        ck(launch_rows_to_host(rows, c->hist_stage[b], 1, n, n, c->stream), "download history");
       }
       ck(hipEventRecord(c->chunk_ev[b], c->stream), "chunk event");
       pend_off[b] = s0;

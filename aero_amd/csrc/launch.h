@@ -52,6 +52,12 @@ hipError_t launch_batch_step(int q, int coll, int st, bool thermal, const BatchC
 hipError_t launch_split_step(int q, int coll, int st, bool thermal, const StepParams& p, hipStream_t s);
 // Last-step force reduction of many cases of one lattice: block b reduces rs[b].
 hipError_t launch_reduce_forces_many(int q, const ForceReduce* rs, int n, hipStream_t s);
+// Coefficient rows of a chunk to pinned host memory: for each of n cases the
+// first nk rows (NCOEF doubles each) of its block of ch rows, same layout.  A
+// kernel storing into the pinned buffer rather than hipMemcpyAsync, whose
+// first device-to-host call in a process blocked the host ~6 ms with work
+// queued on the stream.
+hipError_t launch_rows_to_host(const double* rows, double* host, int n, int ch, int nk, hipStream_t s);
 hipError_t launch_materialize(int q, int st, bool thermal, const StepParams& p, hipStream_t s);
 hipError_t launch_collide(int q, int coll, int st, const CollideParams& p, hipStream_t s);
 hipError_t launch_recycling_plane(int st, const StepParams& p, double* out, hipStream_t s);

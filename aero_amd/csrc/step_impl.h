@@ -1060,18 +1060,49 @@ __device__ __forceinline__ void write_coefficients(const double F[NFORCE], doubl
 // never depends on timing or on what else runs, so a case reproduces its own
 // bits alone or in a batch, whichever launch performs the reduction.  6 KB of
 // LDS, so the step kernel that hosts it keeps its occupancy.
-template <class L>
+// PF link cells of a thread have their inputs loaded before the first is
+// summed (same order of additions for every PF): the standalone reduction
+// launch, which sits on the critical path at the end of a chunk, uses PF = 8
+// to overlap the memory latency of successive cells; inside the step kernel,
+// where it overlaps the cell blocks and shares their register budget, PF = 1.
+template <class L, int PF = 1>
 __device__ __forceinline__ void reduce_forces_block(const ForceReduce& r) {
   __shared__ double red[NFORCE][64];
   double acc[NFORCE];
 #pragma unroll
   for (int k = 0; k < NFORCE; ++k) acc[k] = 0.0;
-  for (int k = threadIdx.x; k < r.nrec; k += BLOCK) {
-    double t[NFORCE];
-    const LinkCell lc = r.lrec[k];
-    cell_force_terms<L>(r.part + (size_t)k * NPART, lc.cell, lc.mask, r.nx, r.ny, r.cx, r.cy, r.cz, t);
+  if constexpr (PF == 1) {
+    // operands read as they are used: fewer registers live in the step kernel
+    for (int k = threadIdx.x; k < r.nrec; k += BLOCK) {
+      double t[NFORCE];
+      const LinkCell lc = r.lrec[k];
+      cell_force_terms<L>(r.part + (size_t)k * NPART, lc.cell, lc.mask, r.nx, r.ny, r.cx, r.cy, r.cz, t);
 #pragma unroll
-    for (int j = 0; j < NFORCE; ++j) acc[j] += t[j];
+      for (int j = 0; j < NFORCE; ++j) acc[j] += t[j];
+    }
+  } else {
+    for (int k0 = threadIdx.x; k0 < r.nrec; k0 += PF * BLOCK) {
+      LinkCell lc[PF];
+      double pp[PF][NPART];
+#pragma unroll
+      for (int j = 0; j < PF; ++j) {
+        const int k = k0 + j * BLOCK;
+        if (k < r.nrec) {
+          lc[j] = r.lrec[k];
+#pragma unroll
+          for (int q = 0; q < NPART; ++q) pp[j][q] = r.part[(size_t)k * NPART + q];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < PF; ++j) {
+        if (k0 + j * BLOCK < r.nrec) {
+          double t[NFORCE];
+          cell_force_terms<L>(pp[j], lc[j].cell, lc[j].mask, r.nx, r.ny, r.cx, r.cy, r.cz, t);
+#pragma unroll
+          for (int q = 0; q < NFORCE; ++q) acc[q] += t[q];
+        }
+      }
+    }
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int w = 0; w < BLOCK / 64; ++w) {
