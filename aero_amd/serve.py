@@ -233,8 +233,11 @@ class Server:
     def __init__(self, chunk: int = 500, max_live: int = 512, mem_reserve: int = 8 << 30,
                  convergence_window: int = 5000, convergence_tol: float = 0.01, strouhal_window: int = 4096,
                  strouhal_tol: float = 0.05, batch=None, free_memory: Optional[Callable[[], int]] = None,
-                 keep_history: bool = False, build_workers: int = 8, device_stats: bool = True):
+                 keep_history: bool = False, build_workers: int = 8, device_stats: bool = True,
+                 on_result: Optional[Callable[["Result"], None]] = None):
+        """on_result, when given, is called with each Result as it is decided."""
         self.chunk = int(chunk)
+        self.on_result = on_result
         self.build_workers = max(1, int(build_workers))
         self.max_live = int(max_live)
         self.mem_reserve = int(mem_reserve)
@@ -300,9 +303,12 @@ class Server:
         w = max(1, len(cd) // 5)
         from .diagnostics import compute_strouhal
         st = compute_strouhal(cl.tolist(), s.D, s.u0) if reason != "diverged" else None
-        return Result(lv.rid, lv.spec, lv.arrival, lv.admitted, now, upto, reason,
-                      float(np.mean(cd[-w:])), float(np.std(cl[-w:])), st, lv.warnings,
-                      cd.tolist() if self.keep_history else [])
+        res = Result(lv.rid, lv.spec, lv.arrival, lv.admitted, now, upto, reason,
+                     float(np.mean(cd[-w:])), float(np.std(cl[-w:])), st, lv.warnings,
+                     cd.tolist() if self.keep_history else [])
+        if self.on_result is not None:
+            self.on_result(res)
+        return res
 
     def _join_built(self, building: list, live: list, clock, t0: float) -> None:
         """Move finished builds into the live set (with their history ring)."""

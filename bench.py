@@ -344,7 +344,9 @@ def _sequential(cases, chunk=500):
     out = []
     t0 = time.perf_counter()
     updates = 0
-    for c in cases:
+    for k, c in enumerate(cases):
+        if k and k % 32 == 0:
+            print(f"[serving] sequential_2d: {k} cases, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
         s = c.build()
         with warnings.catch_warnings(record=True) as caught:
             warnings.simplefilter("always")
@@ -361,6 +363,19 @@ def _sequential(cases, chunk=500):
     return out, time.perf_counter() - t0, updates
 
 
+class _Progress:
+    """Prints a line to stderr every `every` decided requests of a serving leg."""
+
+    def __init__(self, leg, every=32):
+        self.leg, self.every, self.n, self.t0 = leg, every, 0, time.perf_counter()
+
+    def __call__(self, res):
+        self.n += 1
+        if self.n % self.every == 0:
+            print(f"[serving] {self.leg}: {self.n} requests decided, {time.perf_counter() - self.t0:.1f} s",
+                  file=sys.stderr, flush=True)
+
+
 def serving(full=False):
     """Batched 2D solves (the 256-case workload; a bounded sample unless
     `full`), the same cases run one at a time through the plain single-case
@@ -370,7 +385,7 @@ def serving(full=False):
     out = {}
     n_cases, max_steps = (256, 30000) if full else (64, 6000)
     cases = serve.c4_cases(n_cases, seed=0, max_steps=max_steps)
-    srv = serve.Server(chunk=500)
+    srv = serve.Server(chunk=500, on_result=_Progress("batched_2d"))
     t0 = time.perf_counter()
     res = srv.serve([(0.0, c) for c in cases])
     wall = time.perf_counter() - t0
@@ -378,7 +393,7 @@ def serving(full=False):
                              max_steps=max_steps)
     # the same cases in the FP16-shifted storage form (FP32 arithmetic)
     cases16 = serve.c4_cases(n_cases, seed=0, max_steps=max_steps, storage="f16")
-    srv16 = serve.Server(chunk=500)
+    srv16 = serve.Server(chunk=500, on_result=_Progress("batched_2d_f16"))
     t0 = time.perf_counter()
     res16 = srv16.serve([(0.0, c) for c in cases16])
     wall16 = time.perf_counter() - t0
@@ -411,7 +426,7 @@ def serving(full=False):
     for rate in ((2.0, 10.0) if not full else (1.0, 5.0, 10.0, 20.0, 50.0)):
         n = 48 if not full else 200
         reqs = serve.c5_requests(n, rate=rate, seed=int(rate), max_steps=4000 if not full else 30000)
-        srv = serve.Server(chunk=500)
+        srv = serve.Server(chunk=500, on_result=_Progress(f"poisson_{rate:g}", every=16))
         t0 = time.perf_counter()
         r5 = srv.serve(reqs)
         out[f"poisson_{rate:g}_per_s"] = dict(serve.summarize(r5, time.perf_counter() - t0, srv.steps_done,
