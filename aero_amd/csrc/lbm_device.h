@@ -1,9 +1,10 @@
 // Device-side building blocks of the MI355X lattice-Boltzmann step.
 //
-// Populations live in HBM as structure-of-arrays planes, f[q][z][y][x] with x
-// fastest -- the reference's own (Q, Nz, Ny, Nx) C-order layout
-// (aero/lbm/solver3d.py:239-247) -- so one wavefront of 64 lanes reads 64
-// consecutive cells of one direction and every access is coalesced.
+// Populations live in HBM row-interleaved, f[z][y][q][xp] with x fastest and
+// rows padded to xp (a multiple of 64; see padded_row below), so one
+// wavefront of 64 lanes reads 64 consecutive cells of one direction of one
+// row and every access is coalesced.  The host converts from and to the
+// reference's (Q, Nz, Ny, Nx) layout (aero/lbm/solver3d.py:239-247).
 #pragma once
 
 #include <hip/hip_bf16.h>

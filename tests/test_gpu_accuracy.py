@@ -87,14 +87,25 @@ def _c2_f64():
     return _C2_F64["run"]
 
 
-def test_c1_fp32_cd_and_strouhal_within_half_percent_of_fp64():
-    cd64, st64, h64 = _c1("f64")
-    cd32, st32, h32 = _c1("f32")
-    print(f"[gpu] C1 Cd f64 {cd64:.5f} f32 {cd32:.5f}  St f64 {st64} f32 {st32}", flush=True)
+_C1_RUNS = {}
+
+
+def _c1_cached(storage):
+    if storage not in _C1_RUNS:
+        _C1_RUNS[storage] = _c1(storage)
+    return _C1_RUNS[storage]
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_c1_cd_and_strouhal_within_half_percent_of_fp64(storage):
+    """FP32 and FP16-shifted storage."""
+    cd64, st64, h64 = _c1_cached("f64")
+    cd, st, h = _c1_cached(storage)
+    print(f"[gpu] C1 Cd f64 {cd64:.5f} {storage} {cd:.5f}  St f64 {st64} {storage} {st}", flush=True)
     assert 1.0 < cd64 < 2.2  # the reference bands for this case: 1.0-1.35 (configs/reference_cylinder.json), [1.2, 2.2] at ~20% blockage (aero/benchmarks.py:73-78)
-    assert abs(cd32 - cd64) / abs(cd64) < 5e-3
+    assert abs(cd - cd64) / abs(cd64) < 5e-3
     if st64 is not None and 0.1 < st64 < 0.3:
-        assert st32 is not None and abs(st32 - st64) / st64 < 5e-3
+        assert st is not None and abs(st - st64) / st64 < 5e-3
 
 
 def _fft_bin(st, n, D=40.0, u0=0.05):

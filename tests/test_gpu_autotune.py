@@ -20,7 +20,9 @@ pytestmark = pytest.mark.gpu
 
 def _mix():
     return [_cylinder(160, 80, 6.0, "bgk", "f32", 1), _cylinder(96, 48, 5.0, "trt", "f16", 2),
-            _cylinder(128, 64, 6.0, "mrt", "f32", 3), make_solver(3, G.load(3, "trt_bouzidi_conv_slip"), "f32")]
+            _cylinder(128, 64, 6.0, "mrt", "f32", 3), make_solver(3, G.load(3, "trt_bouzidi_conv_slip"), "f32"),
+            _cylinder(640, 96, 8.0, "bgk", "f16", 4), _cylinder(512, 128, 10.0, "trt", "bf16", 5)]
+
 
 
 def _run(solvers, steps=300):
@@ -46,13 +48,14 @@ def test_tuned_schedules_are_bit_identical():
         for a, b in zip(ref_f, fs):
             np.testing.assert_array_equal(a, b)
         # single-case stepping under the same table (a batch of one)
-        s, t = _mix()[0], _mix()[0]
-        autotune.clear()
-        r0 = s._advance(300)
-        for key in autotune.group_keys([t]):
-            autotune.set_entry(key, {"xcd_run": 8, "nt": 0, "graph": 1})
-        np.testing.assert_array_equal(r0, t._advance(300))
-        np.testing.assert_array_equal(s.f, t.f)
+        for k in (0, 4):
+            s, t = _mix()[k], _mix()[k]
+            autotune.clear()
+            r0 = s._advance(300)
+            for key in autotune.group_keys([t]):
+                autotune.set_entry(key, {"xcd_run": 8, "nt": 0, "graph": 1})
+            np.testing.assert_array_equal(r0, t._advance(300))
+            np.testing.assert_array_equal(s.f, t.f)
     finally:
         autotune.clear()
         for k, v in saved.items():
