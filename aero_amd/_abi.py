@@ -73,6 +73,10 @@ _SIGS = {
     "aero_tune_clear": ([], ctypes.c_int),
     "aero_tune_grid_class": ([ctypes.c_int64, _P], ctypes.c_int),
     "aero_step_tiles": ([_P, _P], ctypes.c_int),
+    "aero_halo_bytes": ([_P, _P], ctypes.c_int),
+    "aero_halo_pack": ([_P, ctypes.c_int32, ctypes.c_int32, _P], ctypes.c_int),
+    "aero_halo_unpack": ([_P, ctypes.c_int32, ctypes.c_int32, _P], ctypes.c_int),
+    "aero_halo_exchange": ([_P, _P], ctypes.c_int),
     "aero_set_history_ring": ([_P, ctypes.c_int64], ctypes.c_int),
     "aero_stationarity": ([ctypes.c_int32, _P, _P, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_double,
                            _P], ctypes.c_int),

@@ -470,6 +470,46 @@ hipError_t launch_rows_to_host(const double* rows, double* host, int n, int ch, 
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------ halo faces
+// The D3Q19 populations that cross a z face (e_z = dir), of one z plane of
+// the row-interleaved storage, to or from a compact buffer
+// [5][ny][nx] of raw storage elements: what one z-slab sends its neighbour
+// each step (the neighbour's pull at its first / last plane reads them).
+template <class E, bool PACK>
+__global__ __launch_bounds__(BLOCK) void halo_kernel(E* __restrict__ base, E* __restrict__ buf, int nx, int ny,
+                                                     int xp, int z, int dir) {
+  constexpr int UP[5] = {5, 11, 12, 15, 16};    // e_z = +1
+  constexpr int DOWN[5] = {6, 13, 14, 17, 18};  // e_z = -1
+  const long long plane = (long long)nx * ny;
+  const long long e = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= 5 * plane) return;
+  const int k = (int)(e / plane);
+  const long long r = e - (long long)k * plane;
+  const int y = (int)(r / nx), x = (int)(r - (long long)y * nx);
+  int i = 0;
+#pragma unroll
+  for (int j = 0; j < 5; ++j)
+    if (j == k) i = dir > 0 ? UP[j] : DOWN[j];
+  E* cell = base + (((size_t)z * ny + y) * 19 + i) * (size_t)xp + x;
+  if (PACK) buf[e] = *cell;
+  else *cell = buf[e];
+}
+
+template <bool PACK>
+static hipError_t halo_n(int esz, void* base, void* buf, int nx, int ny, int xp, int z, int dir, hipStream_t s) {
+  const long long n = 5ll * nx * ny;
+  const dim3 g((unsigned)((n + BLOCK - 1) / BLOCK));
+  if (esz == 8) halo_kernel<uint64_t, PACK><<<g, BLOCK, 0, s>>>((uint64_t*)base, (uint64_t*)buf, nx, ny, xp, z, dir);
+  else if (esz == 4) halo_kernel<uint32_t, PACK><<<g, BLOCK, 0, s>>>((uint32_t*)base, (uint32_t*)buf, nx, ny, xp, z, dir);
+  else halo_kernel<uint16_t, PACK><<<g, BLOCK, 0, s>>>((uint16_t*)base, (uint16_t*)buf, nx, ny, xp, z, dir);
+  return hipGetLastError();
+}
+
+hipError_t launch_halo(bool pack, int esz, void* base, void* buf, int nx, int ny, int xp, int z, int dir,
+                       hipStream_t s) {
+  return pack ? halo_n<true>(esz, base, buf, nx, ny, xp, z, dir, s) : halo_n<false>(esz, base, buf, nx, ny, xp, z, dir, s);
+}
+
 hipError_t launch_reduce_forces(const ForceReduce& r, hipStream_t s) {
   if (r.q == 9) reduce_forces_kernel<D2Q9><<<1, BLOCK, 0, s>>>(r);
   else reduce_forces_kernel<D3Q19><<<1, BLOCK, 0, s>>>(r);

@@ -255,6 +255,7 @@ struct aero_ctx {
   // geometry
   DevBuf flags, wave_any, lrec, rec_of, qdir, fpart[2], sigma, minv_d, minv_f, damp;
   DevBuf les_c2d, les_c2f;  // local Smagorinsky: (C_s d)^2 per cell with Van Driest damping d
+  DevBuf halo_buf;          // aero_halo_exchange staging (this slab as the lower one)
   int fpart_side = 0;  // buffer holding the last completed step's per-link-cell force sums
   int nrec = 0;
   std::vector<int> rec_cells;
@@ -890,7 +891,7 @@ int aero_destroy(aero_ctx* c) {
   c->sem = nullptr;
   DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags, &c->wave_any,
                     &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
-                    &c->damp, &c->les_c2d, &c->les_c2f, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
+                    &c->damp, &c->les_c2d, &c->les_c2f, &c->halo_buf, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
                     &c->gcache, &c->tmp, &c->checkpart, &c->checkout, &c->ring, &c->ring_rec};
   for (DevBuf* b : bufs) b->release();
   c->runner.release();
@@ -1606,6 +1607,56 @@ int aero_tune_grid_class(int64_t ntiles, int32_t* cls) {
   });
 }
 
+// Halo faces of a z-slab decomposition (D3Q19): the populations with
+// e_z = dir of plane z of the state the next step reads, to or from a
+// device buffer of aero_halo_bytes bytes: f, then g when the scalar is on.
+static void halo_call(aero_ctx* c, bool pack, int32_t z, int32_t dir, void* buf) {
+  if (c->Q != 19) throw std::invalid_argument("halo faces are D3Q19 (z slabs)");
+  if (z < 0 || z >= c->d.nz || (dir != 1 && dir != -1)) throw std::invalid_argument("halo: bad plane or direction");
+  if (!buf) throw std::invalid_argument("halo: null buffer");
+  c->set_device();
+  const size_t fb = 5 * (size_t)c->d.nx * c->d.ny * c->esz;
+  ck(launch_halo(pack, (int)c->esz, c->B[c->side].p, buf, c->d.nx, c->d.ny, c->xp, z, dir, c->stream), "halo f");
+  if (c->d.thermal)
+    ck(launch_halo(pack, (int)c->esz, c->G[c->side].p, (char*)buf + fb, c->d.nx, c->d.ny, c->xp, z, dir, c->stream),
+       "halo g");
+  if (!pack) c->invalidate();
+}
+
+int aero_halo_bytes(aero_ctx* c, int64_t* bytes) {
+  return guarded([&] { *bytes = (int64_t)(5 * (size_t)c->d.nx * c->d.ny * c->esz * (c->d.thermal ? 2 : 1)); });
+}
+
+int aero_halo_pack(aero_ctx* c, int32_t z, int32_t dir, void* buf) {
+  return guarded([&] { halo_call(c, true, z, dir, buf); });
+}
+
+int aero_halo_unpack(aero_ctx* c, int32_t z, int32_t dir, const void* buf) {
+  return guarded([&] { halo_call(c, false, z, dir, (void*)buf); });
+}
+
+int aero_halo_exchange(aero_ctx* lower, aero_ctx* upper) {
+  return guarded([&] {
+    if (!lower || !upper || lower == upper) throw std::invalid_argument("halo exchange: two distinct slabs");
+    if (lower->d.nx != upper->d.nx || lower->d.ny != upper->d.ny || lower->d.storage != upper->d.storage ||
+        lower->d.thermal != upper->d.thermal || lower->device != upper->device)
+      throw std::invalid_argument("halo exchange: slabs differ in plane shape, storage, scalar or device");
+    const size_t hb = 5 * (size_t)lower->d.nx * lower->d.ny * lower->esz * (lower->d.thermal ? 2 : 1);
+    lower->set_device();
+    lower->halo_buf.alloc(2 * hb);
+    ck(hipStreamSynchronize(upper->stream), "halo sync");
+    // lower's top plane -> upper's bottom halo; upper's bottom plane -> lower's top halo
+    halo_call(lower, true, lower->d.nz - 2, +1, lower->halo_buf.p);
+    halo_call(upper, true, 1, -1, (char*)lower->halo_buf.p + hb);
+    ck(hipStreamSynchronize(lower->stream), "halo sync");
+    ck(hipStreamSynchronize(upper->stream), "halo sync");
+    halo_call(upper, false, 0, +1, lower->halo_buf.p);
+    halo_call(lower, false, lower->d.nz - 1, -1, (char*)lower->halo_buf.p + hb);
+    ck(hipStreamSynchronize(upper->stream), "halo sync");
+    ck(hipStreamSynchronize(lower->stream), "halo sync");
+  });
+}
+
 int aero_step_tiles(aero_ctx* c, int64_t* ntiles) {
   return guarded([&] { *ntiles = (int64_t)c->sblocks + 1; });
 }
@@ -1834,7 +1885,7 @@ int aero_device_bytes(aero_ctx* c, int64_t* bytes) {
   return guarded([&] {
     const DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags, &c->wave_any,
                             &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
-                            &c->damp, &c->les_c2d, &c->les_c2f, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
+                            &c->damp, &c->les_c2d, &c->les_c2f, &c->halo_buf, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
                             &c->gcache, &c->tmp, &c->checkpart, &c->checkout, &c->ring, &c->ring_rec};
     int64_t t = 0;
     for (const DevBuf* b : bufs) t += (int64_t)b->bytes;

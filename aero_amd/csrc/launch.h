@@ -58,6 +58,10 @@ hipError_t launch_reduce_forces_many(int q, const ForceReduce* rs, int n, hipStr
 // first device-to-host call in a process blocked the host ~6 ms with work
 // queued on the stream.
 hipError_t launch_rows_to_host(const double* rows, double* host, int n, int ch, int nk, hipStream_t s);
+// The 5 D3Q19 populations with e_z = dir of z plane `z` (row-interleaved
+// storage, esz-byte elements) to (pack) or from (unpack) a [5][ny][nx] buffer.
+hipError_t launch_halo(bool pack, int esz, void* base, void* buf, int nx, int ny, int xp, int z, int dir,
+                       hipStream_t s);
 hipError_t launch_materialize(int q, int st, bool thermal, const StepParams& p, hipStream_t s);
 hipError_t launch_collide(int q, int coll, int st, const CollideParams& p, hipStream_t s);
 hipError_t launch_recycling_plane(int st, const StepParams& p, double* out, hipStream_t s);

@@ -60,6 +60,7 @@ class CaseSpec:
     thermal: bool = False
     max_steps: int = 30000
     seed: int = 0
+    les_cs: float = 0.0              # > 0: Smagorinsky LES from the local stress (batchable), this C_s
 
     @property
     def cells(self) -> int:
@@ -96,6 +97,11 @@ class CaseSpec:
         per_cell = 2 * q * s * (2 if self.thermal else 1) + 1 + 4
         return self.cells * per_cell + (1 << 20)
 
+    def _les_kw(self) -> dict:
+        if self.les_cs > 0.0:
+            return dict(les=True, les_cs=float(self.les_cs), les_form="local")
+        return {}
+
     def build(self):
         """The solver for this case, at equilibrium (rho = 1, u = u0) plus
         seeded 1e-4 transverse velocity noise, as the benchmark configs state."""
@@ -111,7 +117,7 @@ class CaseSpec:
             s = aero_amd.Solver(Ny=self.ny, Nx=self.nx, solid=solid, omega=omega, u0=self.u0, D=D,
                                 collision=self.collision, trt_lambda=0.1875, trt_variant="lambda",
                                 outlet_bc="zerogradient", sponge_thickness=max(8, self.nx // 20),
-                                storage=self.storage, thermal=self.thermal, alpha_T=0.01)
+                                storage=self.storage, thermal=self.thermal, alpha_T=0.01, **self._les_kw())
             sh = (self.ny, self.nx)
             uy = rng.uniform(-1e-4, 1e-4, sh)
             s.set_equilibrium(uy=uy)  # feq(1, u0, uy) on the device
@@ -124,7 +130,7 @@ class CaseSpec:
             s = aero_amd.Solver3D(Nz=self.nz, Ny=self.ny, Nx=self.nx, solid=solid, omega=omega, u0=self.u0, D=D,
                                   collision=self.collision, trt_lambda=0.1875, trt_variant="lambda",
                                   outlet_bc="zerogradient", sponge_thickness=max(8, self.nx // 20),
-                                  storage=self.storage, thermal=self.thermal, alpha_T=0.01)
+                                  storage=self.storage, thermal=self.thermal, alpha_T=0.01, **self._les_kw())
             sh = (self.nz, self.ny, self.nx)
             uy = rng.uniform(-1e-4, 1e-4, sh)
             uz = rng.uniform(-1e-4, 1e-4, sh)

@@ -294,6 +294,69 @@ def run_trt_les_form(name, steps, warmup):
     return out
 
 
+SLAB_FORMS = {"slab_512x256x32_f32": "f32", "slab_512x256x32_f16": "f16"}
+
+
+def run_slab_form(name, steps, warmup):
+    """One GPU's share of the 512x256x256 TRT sphere split into 8 z slabs:
+    the slabs [96, 128) and [128, 160) (the sphere spans both), each 32 own
+    planes plus 2 halo planes, stepped with a halo exchange before every step
+    (aero_halo_exchange: the 5 crossing populations of each face, packed and
+    unpacked on the device).  Reports the step kernel of the upper slab (HIP
+    events; its own 512x256x32 cells' algorithmic bytes, the halo planes'
+    work counted as overhead) and the exchange's wall time per step for both
+    faces of the pair, including the host synchronisation the one-device
+    exchange does."""
+    import ctypes
+    from aero_amd import slabs as S
+    from aero_amd.geometry import Sphere
+    nx, ny, nz, r = 512, 256, 256, 32.0
+    st = SLAB_FORMS[name]
+    g = Sphere(radius=r, cx_frac=128.0 / nx, cy_frac=0.5, cz_frac=0.5)
+    solid, phi = g.mark_solid(nz, ny, nx), g.sdf_field(nz, ny, nx, offset=0.0)
+    u0, D = 0.05, 2 * r
+    kw = dict(omega=1.0 / (3.0 * u0 * D / 100.0 + 0.5), u0=u0, D=D, bouzidi=True, collision="trt",
+              trt_lambda=0.1875, trt_variant="lambda", bouzidi_variant="linear", storage=st)
+    bounds = S.slab_bounds(nz, 8)[3:5]
+    slabs = [S.make_slab(solid, z0, z1, phi=phi, **kw) for z0, z1 in bounds]
+    del solid, phi
+    for s in slabs:
+        s.set_equilibrium()
+    S.advance(slabs, warmup, periodic=False)
+    for s in slabs:
+        s._ctx.call("aero_synchronize")
+    t0 = time.perf_counter()
+    S.advance(slabs, steps, periodic=False)
+    for s in slabs:
+        s._ctx.call("aero_synchronize")
+    wall = time.perf_counter() - t0
+    up = slabs[1]
+    up._ctx.call("aero_set_timing", ctypes.c_int32(1))
+    for _ in range(min(steps, 100)):
+        up._advance(1)
+    ms = ctypes.c_double(0.0)
+    n = ctypes.c_int64(0)
+    up._ctx.call("aero_get_timing", ctypes.byref(ms), ctypes.byref(n))
+    up._ctx.call("aero_set_timing", ctypes.c_int32(0))
+    k = ms.value / max(n.value, 1)
+    t1 = time.perf_counter()
+    nx_ex = 50
+    for _ in range(nx_ex):
+        S.exchange(slabs, periodic=False)
+    ex = (time.perf_counter() - t1) / nx_ex
+    hb = ctypes.c_int64(0)
+    up._ctx.call("aero_halo_bytes", ctypes.byref(hb))
+    own = nx * ny * (bounds[1][1] - bounds[1][0])
+    bpc = 2 * 19 * {"f32": 4, "f16": 2}[st]
+    for s in slabs:
+        s.close()
+    return {"form": name, "slab_cells": own, "halo_planes": 2, "kernel_ms": round(k, 5),
+            "MLUPS_kernel": round(own / (k * 1e-3) / 1e6, 1),
+            "roofline_frac": round(own * bpc / (k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "pair_step_ms_wall": round(wall / steps * 1e3, 4), "exchange_ms_wall": round(ex * 1e3, 4),
+            "face_bytes": int(hb.value), "faces_per_exchange": 2}
+
+
 def stl_geometry(nz=128, ny=128, nx=256, sample=4000):
     """STL preprocessing on the device: the sample airplane (3,458 triangles)
     voxelised and its narrow-band SDF on a 256x128x128 grid, beside the
@@ -444,8 +507,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving / batched-solves leg")
     ap.add_argument("--serve-full", action="store_true",
                     help="serving leg on the full 256-case workload and the full arrival-rate sweep")
-    ap.add_argument("--form", choices=sorted(FORMS) + sorted(LES_FORMS) + sorted(CYL2D_FORMS) + sorted(TRT_LES_FORMS),
-                    default=None,
+    ap.add_argument("--form", choices=sorted(FORMS) + sorted(LES_FORMS) + sorted(CYL2D_FORMS) + sorted(TRT_LES_FORMS)
+                    + sorted(SLAB_FORMS), default=None,
                     help="only time one storage form of the TRT sphere (prints one JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (calibrates the HBM byte counters)")
@@ -466,6 +529,9 @@ def main():
         return
     if a.form in TRT_LES_FORMS:
         print(json.dumps(run_trt_les_form(a.form, a.steps, a.warmup)), flush=True)
+        return
+    if a.form in SLAB_FORMS:
+        print(json.dumps(run_slab_form(a.form, a.steps, a.warmup)), flush=True)
         return
     if a.form:
         print(json.dumps(run_form(a.form, a.steps, a.warmup)), flush=True)

@@ -220,6 +220,25 @@ int aero_tune_clear(void);
 int aero_tune_grid_class(int64_t ntiles, int32_t* grid_class);
 int aero_step_tiles(aero_ctx* ctx, int64_t* ntiles);
 
+/* Halo faces of a z-slab decomposition (no counterpart in the reference,
+ * whose MPISolver3D splits along z on the host: aero/lbm/mpi_solver3d.py).
+ * A slab context holds its planes plus one halo plane below (z = 0) and one
+ * above (z = nz-1); each step, before aero_step, it sends the populations that
+ * cross its faces and receives its neighbours': aero_halo_pack copies the 5
+ * D3Q19 populations with e_z = dir of plane z of the state the next step
+ * reads into buf (device memory, aero_halo_bytes bytes: [5][ny][nx] storage
+ * elements of f, then of g with the scalar on); aero_halo_unpack writes them
+ * into plane z.  Bottom face: pack (1, -1), the neighbour below unpacks into
+ * its (nz-1, -1); top face: pack (nz-2, +1), the neighbour above unpacks
+ * into its (0, +1).  Calls are ordered on the context's stream. */
+int aero_halo_bytes(aero_ctx* ctx, int64_t* bytes);
+int aero_halo_pack(aero_ctx* ctx, int32_t z, int32_t dir, void* buf);
+int aero_halo_unpack(aero_ctx* ctx, int32_t z, int32_t dir, const void* buf);
+/* Both faces between two slab contexts on one device (lower below upper):
+ * the pack / unpack pairs above through a staging buffer of the lower
+ * context, synchronised with both contexts' streams. */
+int aero_halo_exchange(aero_ctx* lower, aero_ctx* upper);
+
 /* Convergence statistics on the device, in FP64 (the auto-stop test of
  * aero/diagnostics.py:87-120, 153-228 and the 3D lift choice of
  * aero/lbm/solver3d.py:634-638).  aero_set_history_ring gives a context

@@ -1,7 +1,8 @@
 """Serving loop on the device: results are the same alone and in the mix.
 
 A stream of mixed requests (2D cylinders / rectangles / polygons under BGK,
-TRT and MRT, a 3D sphere, one case with a passive scalar) is served with
+TRT and MRT, a 3D sphere, one case with a passive scalar, 2D and 3D cases
+with local-stress LES) is served with
 continuous batching; every request is then run alone (a server with one
 slot).  Stop step, stop reason and the whole drag history must agree
 bit-for-bit.  MI355X only.
@@ -25,7 +26,12 @@ def _requests():
     specs[1].thermal = True
     specs.append(serve.CaseSpec(dim=3, nx=64, ny=32, nz=32, shape="sphere", size=(4.0,), Re=80.0,
                                 collision="trt", max_steps=1000, seed=5))
-    arrivals = [0.0, 0.0, 0.02, 0.05, 0.05, 0.3, 0.1]
+    # local-stress LES cases batch with the rest (their own kernel group)
+    specs.append(serve.CaseSpec(dim=2, nx=160, ny=80, shape="cylinder", size=(8.0,), Re=400.0, collision="trt",
+                                les_cs=0.17, max_steps=1500, seed=6))
+    specs.append(serve.CaseSpec(dim=3, nx=64, ny=32, nz=32, shape="sphere", size=(4.0,), Re=300.0,
+                                collision="bgk", les_cs=0.17, storage="f16", max_steps=1000, seed=7))
+    arrivals = [0.0, 0.0, 0.02, 0.05, 0.05, 0.3, 0.1, 0.0, 0.05]
     return list(zip(arrivals, specs))
 
 
