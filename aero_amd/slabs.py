@@ -1,8 +1,10 @@
 """z-slab decomposition of one D3Q19 domain: the one-device part of a
 multi-GPU run.
 
-The reference distributes Solver3D over MPI ranks in z slabs on the host
-(aero/lbm/mpi_solver3d.py).  Here a slab is an ordinary Solver3D context
+The reference distributes Solver3D over MPI ranks in z slabs on the host,
+each padded with one ghost plane on either side and exchanging whole
+boundary planes (all 19 populations) with MPI Sendrecv after every step
+(aero/lbm/mpi_solver3d.py:116-150).  Here a slab is an ordinary Solver3D context
 whose grid holds its own planes plus one halo plane below and one above; the
 row-interleaved population storage keeps a slab contiguous.  Each step a
 slab needs, in its halo planes, the 5 populations its neighbours' boundary

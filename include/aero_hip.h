@@ -220,8 +220,10 @@ int aero_tune_clear(void);
 int aero_tune_grid_class(int64_t ntiles, int32_t* grid_class);
 int aero_step_tiles(aero_ctx* ctx, int64_t* ntiles);
 
-/* Halo faces of a z-slab decomposition (no counterpart in the reference,
- * whose MPISolver3D splits along z on the host: aero/lbm/mpi_solver3d.py).
+/* Halo faces of a z-slab decomposition.  Replaces the ghost-plane exchange
+ * of MPISolver3D._halo_exchange (aero/lbm/mpi_solver3d.py:116-150), which
+ * sends whole boundary planes with MPI Sendrecv on the host; here only the 5
+ * populations that cross the face move, on the device.
  * A slab context holds its planes plus one halo plane below (z = 0) and one
  * above (z = nz-1); each step, before aero_step, it sends the populations that
  * cross its faces and receives its neighbours': aero_halo_pack copies the 5
