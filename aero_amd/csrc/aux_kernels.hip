@@ -442,13 +442,32 @@ hipError_t launch_check_f64(int q, int nx, int ny, int nz, int zc, const double*
 template <class L>
 __global__ __launch_bounds__(BLOCK) void reduce_forces_kernel(ForceReduce r) { reduce_forces_block<L, 8>(r); }
 
+// The NFORCE terms of every link cell of the cases with a terms buffer, one
+// thread per cell (case = blockIdx.y), ahead of the one-block reduction.
 template <class L>
-__global__ __launch_bounds__(BLOCK) void reduce_forces_many_kernel(const ForceReduce* rs) {
-  reduce_forces_block<L, 8>(rs[blockIdx.x]);
+__global__ __launch_bounds__(BLOCK) void force_terms_kernel(const ForceReduce* rs) {
+  const ForceReduce& r = rs[blockIdx.y];
+  const int k = blockIdx.x * BLOCK + threadIdx.x;
+  if (!r.terms || k >= r.nrec) return;
+  double t[NFORCE];
+  const LinkCell lc = r.lrec[k];
+  cell_force_terms<L>(r.part + (size_t)k * NPART, lc.cell, lc.mask, r.nx, r.ny, r.cx, r.cy, r.cz, t);
+#pragma unroll
+  for (int j = 0; j < NFORCE; ++j) r.terms[(size_t)k * NFORCE + j] = t[j];
 }
 
-hipError_t launch_reduce_forces_many(int q, const ForceReduce* rs, int n, hipStream_t s) {
+template <class L>
+__global__ __launch_bounds__(BLOCK) void reduce_forces_many_kernel(const ForceReduce* rs) {
+  reduce_forces_block<L, 8, true>(rs[blockIdx.x]);
+}
+
+hipError_t launch_reduce_forces_many(int q, const ForceReduce* rs, int n, int max_terms, hipStream_t s) {
   if (n <= 0) return hipSuccess;
+  if (max_terms > 0) {
+    const dim3 g((unsigned)((max_terms + BLOCK - 1) / BLOCK), (unsigned)n);
+    if (q == 9) force_terms_kernel<D2Q9><<<g, BLOCK, 0, s>>>(rs);
+    else force_terms_kernel<D3Q19><<<g, BLOCK, 0, s>>>(rs);
+  }
   if (q == 9) reduce_forces_many_kernel<D2Q9><<<n, BLOCK, 0, s>>>(rs);
   else reduce_forces_many_kernel<D3Q19><<<n, BLOCK, 0, s>>>(rs);
   return hipGetLastError();

@@ -256,6 +256,7 @@ struct aero_ctx {
   DevBuf flags, wave_any, lrec, rec_of, qdir, fpart[2], sigma, minv_d, minv_f, damp;
   DevBuf les_c2d, les_c2f;  // local Smagorinsky: (C_s d)^2 per cell with Van Driest damping d
   DevBuf halo_buf;          // aero_halo_exchange staging (this slab as the lower one)
+  DevBuf fterms;            // NFORCE doubles per link cell for the chunk-end force reduction
   int fpart_side = 0;  // buffer holding the last completed step's per-link-cell force sums
   int nrec = 0;
   std::vector<int> rec_cells;
@@ -718,6 +719,7 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
     int m = 0;
     for (int q : {9, 19}) {
       const int m0 = m;
+      int max_terms = 0;
       for (int i = 0; i < n; ++i) {
         aero_ctx* c = cs[i];
         if (c->Q != q || c->nrec == 0) continue;
@@ -725,12 +727,16 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
                                           c->lrec.as<LinkCell>(), c->nrec, c->d.nx, c->d.ny, c->d.center[0],
                                           c->d.center[1], c->d.center[2], c->force_denom(), c->moment_denom(),
                                           rows + (size_t)i * CH * NCOEF + (size_t)NCOEF * (nk - 1));
+        c->fterms.alloc(sizeof(double) * NFORCE * (size_t)c->nrec);
+        R.reds_host[b][m - 1].terms = (decltype(R.reds_host[b][m - 1].terms))(c->fterms.as<double>());
+        max_terms = std::max(max_terms, c->nrec);
       }
       if (m > m0) {
         ck(hipMemcpyAsync(R.reds[b].as<ForceReduce>() + m0, R.reds_host[b] + m0, sizeof(ForceReduce) * (m - m0),
                           hipMemcpyHostToDevice, stream),
            "upload reductions");
-        ck(launch_reduce_forces_many(q, R.reds[b].as<ForceReduce>() + m0, m - m0, stream), "force reduction");
+        ck(launch_reduce_forces_many(q, R.reds[b].as<ForceReduce>() + m0, m - m0, max_terms, stream),
+           "force reduction");
       }
     }
     int nr = 0;
@@ -891,7 +897,7 @@ int aero_destroy(aero_ctx* c) {
   c->sem = nullptr;
   DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags, &c->wave_any,
                     &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
-                    &c->damp, &c->les_c2d, &c->les_c2f, &c->halo_buf, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
+                    &c->damp, &c->les_c2d, &c->les_c2f, &c->halo_buf, &c->fterms, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
                     &c->gcache, &c->tmp, &c->checkpart, &c->checkout, &c->ring, &c->ring_rec};
   for (DevBuf* b : bufs) b->release();
   c->runner.release();
@@ -1557,7 +1563,7 @@ int aero_step_noalloc(aero_ctx* c, int32_t nsteps, int64_t step0, double* hist) 
                                                 c->d.center[1], c->d.center[2], c->force_denom(), c->moment_denom(),
                                                 d_rows + (size_t)NCOEF * (nk - 1));
         ck(hipMemcpyAsync(d_red, &r, sizeof(ForceReduce), hipMemcpyHostToDevice, st), "noalloc reduction");
-        ck(launch_reduce_forces_many(c->Q, d_red, 1, st), "noalloc force reduction");
+        ck(launch_reduce_forces_many(c->Q, d_red, 1, 0, st), "noalloc force reduction");
       }
       if (c->ring_cap > 0) {
         const RingAppend ra = c->ring_append(d_rows, nk);
@@ -1885,7 +1891,7 @@ int aero_device_bytes(aero_ctx* c, int64_t* bytes) {
   return guarded([&] {
     const DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags, &c->wave_any,
                             &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
-                            &c->damp, &c->les_c2d, &c->les_c2f, &c->halo_buf, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
+                            &c->damp, &c->les_c2d, &c->les_c2f, &c->halo_buf, &c->fterms, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
                             &c->gcache, &c->tmp, &c->checkpart, &c->checkout, &c->ring, &c->ring_rec};
     int64_t t = 0;
     for (const DevBuf* b : bufs) t += (int64_t)b->bytes;

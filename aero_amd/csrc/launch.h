@@ -51,7 +51,9 @@ hipError_t launch_batch_step(int q, int coll, int st, bool thermal, const BatchC
 // Split step of one case (LES finite-difference pass follows; BGK / TRT).
 hipError_t launch_split_step(int q, int coll, int st, bool thermal, const StepParams& p, hipStream_t s);
 // Last-step force reduction of many cases of one lattice: block b reduces rs[b].
-hipError_t launch_reduce_forces_many(int q, const ForceReduce* rs, int n, hipStream_t s);
+// max_terms > 0: first form the terms of the cases whose record has a terms
+// buffer (at most max_terms link cells each)
+hipError_t launch_reduce_forces_many(int q, const ForceReduce* rs, int n, int max_terms, hipStream_t s);
 // Coefficient rows of a chunk to pinned host memory: for each of n cases the
 // first nk rows (NCOEF doubles each) of its block of ch rows, same layout.  A
 // kernel storing into the pinned buffer rather than hipMemcpyAsync, whose

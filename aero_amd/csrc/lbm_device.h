@@ -412,12 +412,16 @@ struct ForceReduce {
   double cx, cy, cz;  // moment reference centre
   double fd, md;      // force and moment denominators
   AERO_GLOBAL double* row;        // NCOEF doubles out
+  // NFORCE doubles per link cell, formed beforehand by force_terms_kernel, or
+  // null: the reduction forms each cell's terms itself (the same bits)
+  AERO_GLOBAL double* terms;
 };
 
 // Host-side construction (casts plain device pointers to the struct's types).
 inline ForceReduce make_force_reduce(int q, const double* part, const LinkCell* lrec, int nrec, int nx, int ny,
                                      double cx, double cy, double cz, double fd, double md, double* row) {
   ForceReduce r;
+  r.terms = nullptr;
   r.q = q;
   r.part = (decltype(r.part))part;
   r.lrec = (decltype(r.lrec))lrec;

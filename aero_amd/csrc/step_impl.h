@@ -265,6 +265,10 @@ __device__ __forceinline__ void link_post_sum(const StepParams& p, int rec, cons
 template <class L>
 __device__ __host__ __forceinline__ void cell_force_terms(const double* part, int cell, uint32_t mask, int nx, int ny,
                                                           double cx, double cy, double cz, double out[NFORCE]) {
+  // no contraction into fused multiply-adds: the terms must come out the
+  // same in every kernel that forms them (the step's reduction block and
+  // force_terms_kernel)
+#pragma clang fp contract(off)
   using D = DirPack<L>;
   const double rho = part[0], ux = part[1], uy = part[2], uz = part[3];
   const double base = 1.0 - 1.5 * (ux * ux + uy * uy + uz * uz);
@@ -1065,13 +1069,35 @@ __device__ __forceinline__ void write_coefficients(const double F[NFORCE], doubl
 // launch, which sits on the critical path at the end of a chunk, uses PF = 8
 // to overlap the memory latency of successive cells; inside the step kernel,
 // where it overlaps the cell blocks and shares their register budget, PF = 1.
-template <class L, int PF = 1>
+// TERMS: the record may carry terms formed beforehand (the standalone
+// launch); the step kernel's reduction block forms them itself.
+template <class L, int PF = 1, bool TERMS = false>
 __device__ __forceinline__ void reduce_forces_block(const ForceReduce& r) {
   __shared__ double red[NFORCE][64];
   double acc[NFORCE];
 #pragma unroll
   for (int k = 0; k < NFORCE; ++k) acc[k] = 0.0;
-  if constexpr (PF == 1) {
+  if (TERMS && r.terms) {
+    // terms formed beforehand: a plain sum in the same order, PF cells ahead
+    for (int k0 = threadIdx.x; k0 < r.nrec; k0 += PF * BLOCK) {
+      double tt[PF][NFORCE];
+#pragma unroll
+      for (int j = 0; j < PF; ++j) {
+        const int k = k0 + j * BLOCK;
+        if (k < r.nrec) {
+#pragma unroll
+          for (int q = 0; q < NFORCE; ++q) tt[j][q] = r.terms[(size_t)k * NFORCE + q];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < PF; ++j) {
+        if (k0 + j * BLOCK < r.nrec) {
+#pragma unroll
+          for (int q = 0; q < NFORCE; ++q) acc[q] += tt[j][q];
+        }
+      }
+    }
+  } else if constexpr (PF == 1) {
     // operands read as they are used: fewer registers live in the step kernel
     for (int k = threadIdx.x; k < r.nrec; k += BLOCK) {
       double t[NFORCE];

@@ -3,7 +3,7 @@
 The fused step is HBM-bound only while its waves neither spill much nor lose
 occupancy.  The D3Q19 TRT batch kernels (the 512 x 256 x 256 throughput
 workload) must keep their register target (7 waves per SIMD, at most 72
-VGPRs) with at most 12 bytes of scratch per lane on rare paths (28 with the
+VGPRs) with at most 12 bytes of scratch per lane on rare paths (32 with the
 scalar; none for FP16); a code change that makes them spill fails here (a
 loop around the step body once cost 96-108 bytes per lane and 30% of the
 bandwidth; 20 bytes in the FP16 kernel cost it 20%).
@@ -41,8 +41,8 @@ def test_d3q19_trt_batch_kernels_keep_their_register_budget(tmp_path):
         scratch = int(re.search(r"ScratchSize \[bytes/lane\]: (\d+)", b).group(1))
         seen += 1
         assert vgpr <= 72, (name, vgpr)
-        # without the scalar: at most 12 bytes (FP16 none); with it 28
-        assert scratch <= (28 if m.group(2) == "1" else 12), (name, scratch)
+        # without the scalar: at most 12 bytes (FP16 none); with it 32
+        assert scratch <= (32 if m.group(2) == "1" else 12), (name, scratch)
         if m.groups() == ("2", "0"):
             assert scratch == 0, (name, scratch)
     assert seen == 6
