@@ -81,3 +81,19 @@ def test_noalloc_step_rejects_what_it_does_not_serve():
     sem.close()
     les.close()
     assert _abi.lib() is not None
+
+
+def test_noalloc_step_serves_local_les():
+    """The fused local-stress LES form (no split step) steps allocation-free,
+    with the same bits as aero_step."""
+    case = G.load(3, "trt_ibm_les")
+    kw = dict(case["kw"])
+    kw["les_form"] = "local"
+    case = dict(case, kw=kw)
+    a, b = make_solver(3, case, "f32"), make_solver(3, case, "f32")
+    ra = a._advance(30)
+    rb = _noalloc(b, 30)
+    np.testing.assert_array_equal(ra, rb)
+    np.testing.assert_array_equal(a.f, b.f)
+    a.close()
+    b.close()
