@@ -766,6 +766,7 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
       c->timed_launches += nk;
     }
   }
+  bool tail = false;  // work queued after the last chunk event
   for (aero_ctx* c : cs) {
     // keep the last step's inlet plane for re-deriving its state later
     if (c->need_planes()) {
@@ -773,12 +774,14 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
                         hipMemcpyDeviceToDevice, stream),
          "keep last plane");
       c->last_plane = c->last_plane_buf.as<double>();
+      tail = true;
     }
     c->invalidate();
   }
   drain(0);
   drain(1);
-  ck(hipStreamSynchronize(stream), "step sync");
+  // the chunk events close everything else on the stream
+  if (tail) ck(hipStreamSynchronize(stream), "step sync");
 }
 
 extern "C" {
@@ -1443,7 +1446,7 @@ int aero_step(aero_ctx* c, int32_t nsteps, int64_t step0, const double* planes, 
     c->invalidate();
     drain(0);
     drain(1);
-    ck(hipStreamSynchronize(c->stream), "step sync");
+    if (need_planes) ck(hipStreamSynchronize(c->stream), "step sync");  // the copy above follows the last event
   });
 }
 
