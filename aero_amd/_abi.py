@@ -69,7 +69,7 @@ _SIGS = {
     "aero_get_outlet_prev": ([_P, _P], ctypes.c_int),
     "aero_step": ([_P, ctypes.c_int32, ctypes.c_int64, _P, _P], ctypes.c_int),
     "aero_step_noalloc": ([_P, ctypes.c_int32, ctypes.c_int64, _P], ctypes.c_int),
-    "aero_tune_set": ([ctypes.c_int32] * 8, ctypes.c_int),
+    "aero_tune_set": ([ctypes.c_int32] * 9, ctypes.c_int),
     "aero_tune_clear": ([], ctypes.c_int),
     "aero_tune_grid_class": ([ctypes.c_int64, _P], ctypes.c_int),
     "aero_step_tiles": ([_P, _P], ctypes.c_int),

@@ -10,7 +10,10 @@ walks its tiles:
   its own L2);
 * ``nt``: non-temporal population stores, -1 the built-in rule (on when the
   launch's populations exceed most of the 256 MB last-level cache), 0 / 1;
-* ``graph``: 1 replays each chunk's step launches from a captured HIP graph.
+* ``graph``: 1 replays each chunk's step launches from a captured HIP graph;
+* ``rowgroups``: the wave shape, -1 the built-in one (the 2-byte D2Q9
+  kernels step bulk rows four to a wave, from a per-case wave list), 0 one
+  row per wave.
 
 None of them changes a result: every cell computes the same bits, and the
 per-case force reduction keeps its fixed order.  ``tune`` measures a
@@ -35,8 +38,9 @@ import numpy as np
 from . import _abi
 
 DEFAULT_PATH = pathlib.Path(__file__).resolve().parent / "tuning.json"
-KNOBS = (("xcd_run", (0, -1, 8, 32, 128)), ("nt", (-1, 0, 1)), ("graph", (0, 1)))
-DEFAULT = {"xcd_run": 0, "nt": -1, "graph": 0}
+KNOBS = (("xcd_run", (0, -1, 8, 32, 128)), ("nt", (-1, 0, 1)), ("graph", (0, 1)), ("rowgroups", (-1, 0)))
+DEFAULT = {"xcd_run": 0, "nt": -1, "graph": 0, "rowgroups": -1}
+NAMES = tuple(n for n, _ in KNOBS)
 
 # entries set in this process: (q, collision, storage, thermal, grid class) -> knobs
 _table: Dict[Tuple[int, int, int, int, int], dict] = {}
@@ -51,7 +55,7 @@ def grid_class(ntiles: int) -> int:
 def set_entry(key, knobs: dict) -> None:
     k = {**DEFAULT, **knobs}
     _abi.check(_abi.lib().aero_tune_set(*[ctypes.c_int32(int(v)) for v in key],
-                                        *[ctypes.c_int32(int(k[n])) for n in ("xcd_run", "nt", "graph")]))
+                                        *[ctypes.c_int32(int(k[n])) for n in NAMES]))
     _table[tuple(int(v) for v in key)] = k
 
 
@@ -78,6 +82,13 @@ def group_keys(solvers: Sequence) -> Dict[tuple, list]:
         d = s._ctx.desc
         groups.setdefault((d.q, d.collision, d.storage, int(d.thermal != 0)), []).append(s)
     return {k + (grid_class(sum(_tiles(s) for s in v)),): v for k, v in groups.items()}
+
+
+def lists_waves(key) -> bool:
+    """Kernel groups whose step walks a per-case wave list with grouped bulk
+    rows (the rowgroups knob applies): 2-byte D2Q9 without the scalar."""
+    q, _coll, st, th = key[:4]
+    return q == 9 and st in (2, 3) and not th
 
 
 def _time(solvers, steps: int, batch, reps: int) -> float:
@@ -121,6 +132,8 @@ def tune(solvers: Sequence, steps: int = 100, reps: int = 3, min_gain: float = 0
         for key in group_keys(solvers):
             best = dict(_table.get(key, DEFAULT))
             for name, values in KNOBS:
+                if name == "rowgroups" and not lists_waves(key):
+                    continue
                 for v in values:
                     if v == best[name]:
                         continue
@@ -149,7 +162,7 @@ def load(path=DEFAULT_PATH) -> int:
     data = json.loads(pathlib.Path(path).read_text())
     for r in data["entries"]:
         set_entry((r["q"], r["collision"], r["storage"], r["thermal"], r["grid_class"]),
-                  {n: r[n] for n in ("xcd_run", "nt", "graph")})
+                  {n: r.get(n, DEFAULT[n]) for n in NAMES})
     return len(data["entries"])
 
 

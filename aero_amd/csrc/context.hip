@@ -121,6 +121,7 @@ struct TuneVal {
   int xcd_run = 0;  // 0: built-in rule; -1: natural order; k > 0: runs of k tiles per XCD
   int nt = -1;      // -1: built-in rule (non-temporal stores past NT_STORE_BYTES); 0 / 1: off / on
   int graph = 0;    // 1: a chunk's step launches replay from a captured HIP graph
+  int rowgroups = -1;  // -1: grouped row waves where the kernel lists its waves; 0: one row per wave
 };
 namespace {
 std::mutex g_tune_mu;
@@ -166,6 +167,7 @@ struct StepRunner {
     bool holds(const aero_ctx* c) const;
   };
   std::vector<Group> groups;
+  std::vector<char> listed;  // per case of that list: its step walks the case's wave list
   // captured chunks (graph launch tuning): by buffer parity and chunk length,
   // valid for the tile table and case-table buffers they were captured with
   struct Captured {
@@ -245,7 +247,10 @@ struct aero_ctx {
   int nrow = 0;
   int nblocks = 0;  // blocks over the N cells (per-cell kernels)
   int xp = 0;       // padded row length (storage rows and the step's thread grid)
-  int sblocks = 0;  // blocks over the nrow * xp step threads
+  int sblocks = 0;       // blocks over the step's waves: the listed waves, else nrow * xp threads
+  int sblocks_rows = 0;  // blocks over nrow * xp threads (one row per wave)
+  int nitems = 0;        // grouped kernels: listed waves (witems); 0 for the others
+  unsigned long long layout_gen = 0;  // bumped when the step's wave list changes
   size_t esz = 8;  // storage element size
   size_t csz = 8;  // compute element size
   // state: ping-pong sides, `side` is the one the next step reads
@@ -253,6 +258,7 @@ struct aero_ctx {
   int side = 0;
   int prev_side = 0;
   // geometry
+  DevBuf witems;  // grouped kernels: the step's wave list
   DevBuf flags, wave_any, lrec, rec_of, qdir, fpart[2], sigma, minv_d, minv_f, damp;
   DevBuf les_c2d, les_c2f;  // local Smagorinsky: (C_s d)^2 per cell with Van Driest damping d
   DevBuf halo_buf;          // aero_halo_exchange staging (this slab as the lower one)
@@ -337,6 +343,8 @@ struct aero_ctx {
     p.qxp = Q * xp;
     p.div_xp = make_div_magic((unsigned)xp);
     p.sblocks = sblocks;
+    p.nitems = nitems;
+    p.witems = (decltype(p.witems))(witems.as<uint32_t>());
     p.ibm_on = ibm_on;
     p.ibm_coeff = ibm_coeff;
     p.sponge_x0 = sponge_x0;
@@ -579,7 +587,10 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
   R.reserve(n, CH);
   // groups and the packed tile list, rebuilt only when the case list changes
   std::vector<unsigned long long> key;
-  for (aero_ctx* c : cs) key.push_back(c->serial);
+  for (aero_ctx* c : cs) {
+    key.push_back(c->serial);
+    key.push_back(c->layout_gen);  // a new wave list changes the case's block count
+  }
   const unsigned tgen = g_tune_gen.load();
   if (key != R.tile_key || tgen != R.tune_gen) {
     R.drop_graphs();
@@ -599,19 +610,26 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
       members[g].push_back(i);
     }
     std::vector<int2> tiles, nat;
+    R.listed.assign(n, 0);
     for (size_t g = 0; g < groups.size(); ++g) {
       groups[g].toff = (int)tiles.size();
       nat.clear();
       size_t zreuse = 0;  // planes z-1, z, z+1 of every population, on all XCDs at once
+      long long row_tiles = 0;  // the group's tiles with one row per wave: its grid class
       for (int i : members[g]) {
-        for (int b = 0; b <= cs[i]->sblocks; ++b) nat.push_back(make_int2(i, b));
+        row_tiles += cs[i]->sblocks_rows + 1;
         const aero_ctx* c = cs[i];
         if (c->d.nz > 1)
           zreuse = std::max(zreuse, (size_t)XCDS * 3 * c->Q * (size_t)c->xp * c->d.ny * c->esz);
       }
       const StepRunner::Group& G = groups[g];
-      const TuneVal tv = tune_lookup(G.q, G.coll, G.st, G.th, grid_class((long long)nat.size()));
+      const TuneVal tv = tune_lookup(G.q, G.coll, G.st, G.th, grid_class(row_tiles));
       groups[g].tune = tv;
+      for (int i : members[g]) {
+        R.listed[i] = tv.rowgroups != 0 && cs[i]->nitems > 0;
+        const int nb = R.listed[i] ? cs[i]->sblocks : cs[i]->sblocks_rows;
+        for (int b = 0; b <= nb; ++b) nat.push_back(make_int2(i, b));
+      }
       int run = zreuse > ZREUSE_BYTES ? XCD_RUN_LARGE : ((int)nat.size() + XCDS - 1) / XCDS;
       if (tv.xcd_run > 0) run = tv.xcd_run;
       if (tv.xcd_run < 0) tiles.insert(tiles.end(), nat.begin(), nat.end());
@@ -673,8 +691,15 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
       }
       R.cases_host[b][i] = c->make_case(step0s[i] + s0, pdev, rows + (size_t)i * CH * NCOEF);
     }
-    for (int i = 0; i < n; ++i)
+    for (int i = 0; i < n; ++i) {
       if (nt[i]) R.cases_host[b][i].p[0].nt_store = R.cases_host[b][i].p[1].nt_store = 1;
+      if (!R.listed[i]) {  // one row per wave over the whole thread grid
+        for (StepParams& p : R.cases_host[b][i].p) {
+          p.nitems = 0;
+          p.sblocks = cs[i]->sblocks_rows;
+        }
+      }
+    }
     ck(hipMemcpyAsync(R.cases[b].p, R.cases_host[b], sizeof(BatchCase) * n, hipMemcpyHostToDevice, stream),
        "upload cases");
     const BatchCase* cases = R.cases[b].as<BatchCase>();
@@ -824,7 +849,8 @@ int aero_create(const aero_desc* desc, aero_ctx** out) {
     c->nrow = d.nz * d.ny;
     c->xp = padded_row(d.nx);
     c->nblocks = (int)((N + BLOCK - 1) / BLOCK);
-    c->sblocks = (int)((nthread + BLOCK - 1) / BLOCK);
+    c->sblocks_rows = (int)((nthread + BLOCK - 1) / BLOCK);
+    c->sblocks = c->sblocks_rows;  // grouped kernels: set with the flags
     c->esz = storage_bytes(d.storage);
     c->csz = d.storage == ST_F64 ? 8 : 4;
     ck(hipGetDevice(&c->device), "hipGetDevice");
@@ -1001,6 +1027,37 @@ static void upload_flags(aero_ctx* c, const std::vector<uint8_t>& fl) {
   c->wave_any.alloc(sizeof(uint32_t) * words.size());
   ck(hipMemcpy(c->wave_any.p, words.data(), sizeof(uint32_t) * words.size(), hipMemcpyHostToDevice),
      "copy wave flags");
+  // grouped kernels: the wave list, row groups of RPW_GROUP rows by 64-column segments
+  const aero_desc& d = c->d;
+  if (rows_per_wave(d.q, d.collision, d.storage, d.thermal != 0) > 1) {
+    const int RG = RPW_GROUP, segs = c->xp / 64;
+    std::vector<uint32_t> items;
+    items.reserve(nwave);
+    for (int r0 = 0; r0 < c->nrow; r0 += RG) {
+      // wall rows (y = 0, ny - 1; 2D, so a row is its y) take the one-row boundary path
+      const bool rows_ok = r0 + RG <= c->nrow && (d.wall == AERO_WALL_NONE || (r0 > 0 && r0 + RG - 1 < d.ny - 1));
+      for (int sgm = 0; sgm < segs; ++sgm) {
+        const int x0 = sgm * 64, x1 = std::min(x0 + 63, nx - 1);
+        // no inlet, outlet, sponge or padding lane (the last segment holds x = nx - 1)
+        bool bulk = rows_ok && x0 > 0 && x1 < nx - 1 && x1 < c->sponge_x0 && x0 + 63 < nx;
+        for (int r = r0; bulk && r < r0 + RG; ++r) {
+          const size_t w = (size_t)r * segs + sgm;
+          bulk = !((words[w >> 5] >> (w & 31)) & 1u);
+        }
+        if (bulk) {
+          items.push_back(WITEM_GROUP | (uint32_t)((size_t)r0 * segs + sgm));
+        } else {
+          for (int r = r0; r < std::min(r0 + RG, c->nrow); ++r) items.push_back((uint32_t)((size_t)r * segs + sgm));
+        }
+      }
+    }
+    c->nitems = (int)items.size();
+    c->witems.alloc(sizeof(uint32_t) * std::max<size_t>(1, items.size()));
+    ck(hipMemcpy(c->witems.p, items.data(), sizeof(uint32_t) * items.size(), hipMemcpyHostToDevice),
+       "copy wave list");
+    c->sblocks = (int)(((long long)c->nitems * 64 + BLOCK - 1) / BLOCK);
+  }
+  ++c->layout_gen;
 }
 
 // Cell flags and link-cell records for one geometry.
@@ -1590,13 +1647,14 @@ int aero_step_noalloc(aero_ctx* c, int32_t nsteps, int64_t step0, double* hist) 
 }  // extern "C"
 
 int aero_tune_set(int32_t q, int32_t collision, int32_t storage, int32_t thermal, int32_t grid_class_,
-                  int32_t xcd_run, int32_t nt, int32_t graph) {
+                  int32_t xcd_run, int32_t nt, int32_t graph, int32_t rowgroups) {
   return guarded([&] {
     if ((q != 9 && q != 19) || collision < 0 || collision > 2 || storage < 0 || storage > 3 || grid_class_ < 0 ||
-        grid_class_ > 40 || nt < -1 || nt > 1 || graph < 0 || graph > 1 || xcd_run < -1)
+        grid_class_ > 40 || nt < -1 || nt > 1 || graph < 0 || graph > 1 || xcd_run < -1 || rowgroups < -1 ||
+        rowgroups > 0)
       throw std::invalid_argument("aero_tune_set: argument out of range");
     std::lock_guard<std::mutex> lock(g_tune_mu);
-    g_tune[{q, collision, storage, thermal != 0, grid_class_}] = TuneVal{xcd_run, nt, graph};
+    g_tune[{q, collision, storage, thermal != 0, grid_class_}] = TuneVal{xcd_run, nt, graph, rowgroups};
     ++g_tune_gen;
   });
 }
@@ -1667,7 +1725,7 @@ int aero_halo_exchange(aero_ctx* lower, aero_ctx* upper) {
 }
 
 int aero_step_tiles(aero_ctx* c, int64_t* ntiles) {
-  return guarded([&] { *ntiles = (int64_t)c->sblocks + 1; });
+  return guarded([&] { *ntiles = (int64_t)c->sblocks_rows + 1; });
 }
 
 int aero_set_history_ring(aero_ctx* c, int64_t cap) {

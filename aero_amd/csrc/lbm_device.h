@@ -231,6 +231,10 @@ template <int ST> struct Storage;
   static __device__ __forceinline__ C ld(const T* plane, unsigned k, C s) {                 \
     return load((const T*)((const char*)plane + k * (unsigned)sizeof(T)), 0, s);            \
   }                                                                                          \
+  /* the stored value, unconverted; cvt(raw(plane, k), s) == ld(plane, k, s) */              \
+  static __device__ __forceinline__ T raw(const T* plane, unsigned k) {                     \
+    return *(const T*)((const char*)plane + k * (unsigned)sizeof(T));                       \
+  }                                                                                          \
   static __device__ __forceinline__ void st(T* plane, unsigned k, C v, C s) {               \
     store((T*)((char*)plane + k * (unsigned)sizeof(T)), 0, v, s);                           \
   }                                                                                          \
@@ -241,6 +245,7 @@ template <> struct Storage<ST_F64> {
   using T = double;
   using C = double;
   static __device__ __forceinline__ C load(const T* p, size_t k, C) { return p[k]; }
+  static __device__ __forceinline__ C cvt(T v, C) { return v; }
   static __device__ __forceinline__ void store(T* p, size_t k, C v, C) { p[k] = v; }
   static __device__ __forceinline__ void store_nt(T* q, C v, C) { __builtin_nontemporal_store(v, q); }
   AERO_PLANE_ACCESS
@@ -249,6 +254,7 @@ template <> struct Storage<ST_F32> {
   using T = float;
   using C = float;
   static __device__ __forceinline__ C load(const T* p, size_t k, C) { return p[k]; }
+  static __device__ __forceinline__ C cvt(T v, C) { return v; }
   static __device__ __forceinline__ void store(T* p, size_t k, C v, C) { p[k] = v; }
   static __device__ __forceinline__ void store_nt(T* q, C v, C) { __builtin_nontemporal_store(v, q); }
   AERO_PLANE_ACCESS
@@ -269,6 +275,7 @@ template <> struct Storage<ST_F16S> {
   static __device__ __forceinline__ C load(const T* p, size_t k, C s) {
     return __builtin_fmaf(__half2float(p[k]), one(), s);
   }
+  static __device__ __forceinline__ C cvt(T v, C s) { return __builtin_fmaf(__half2float(v), one(), s); }
   static __device__ __forceinline__ void store(T* p, size_t k, C v, C s) {
     p[k] = __float2half(__builtin_fmaf(v, one(), -s));
   }
@@ -281,6 +288,7 @@ template <> struct Storage<ST_BF16S> {
   using T = __hip_bfloat16;
   using C = float;
   static __device__ __forceinline__ C load(const T* p, size_t k, C s) { return __bfloat162float(p[k]) + s; }
+  static __device__ __forceinline__ C cvt(T v, C s) { return __bfloat162float(v) + s; }
   static __device__ __forceinline__ void store(T* p, size_t k, C v, C s) { p[k] = __float2bfloat16(v - s); }
   static __device__ __forceinline__ void store_nt(T* q, C v, C s) {
     __builtin_nontemporal_store(__bfloat16_as_ushort(__float2bfloat16(v - s)), (unsigned short*)q);
@@ -328,6 +336,23 @@ constexpr int BLOCK = 256;
 constexpr int ROW_ALIGN = 64;
 __host__ __device__ constexpr int padded_row(int nx) { return (nx + ROW_ALIGN - 1) / ROW_ALIGN * ROW_ALIGN; }
 
+// Row groups of the 2-byte D2Q9 BGK / TRT kernels (no scalar).  Their step
+// waves are listed per case (StepParams::witems): a plain item is one row's
+// 64 aligned columns; a group item (bit 31) covers the same columns in
+// RPW_GROUP consecutive rows, none of them holding a flagged cell, a domain
+// edge, a sponge column or a wall row.  A group wave issues all RPW_GROUP x
+// 9 two-byte loads before it collides its first row: a one-row wave keeps
+// only 9 x 128 bytes in flight, too few to cover HBM latency at the
+// occupancy the kernel reaches (a pull + BGK probe on 8192 x 2048 ran at
+// 48% of 8 TB/s with one row per wave, 71% with four).  Every other kernel
+// runs one row per wave over the whole nrow * xp thread grid.
+constexpr int RPW_GROUP = 4;
+constexpr uint32_t WITEM_GROUP = 0x80000000u;
+__host__ __device__ constexpr int rows_per_wave(int q, int collision, int storage, bool thermal) {
+  return (q == 9 && (storage == ST_F16S || storage == ST_BF16S) && !thermal) ? RPW_GROUP
+                                                                                                      : 1;
+}
+
 // Everything one step launch needs; passed by value as the kernel argument.
 struct StepParams {
   int nx, ny, nz;
@@ -337,7 +362,9 @@ struct StepParams {
   DivMagic div_nx, div_ny, div_xp;  // cell -> (x, y, z) and step thread -> row without integer division
   long long ncell;
   int nblocks;  // blocks covering ncell (per-cell kernels)
-  int sblocks;  // blocks covering nrow * xp step threads (the step launch adds one reduction block)
+  int sblocks;  // blocks covering the step's waves (the step launch adds one reduction block)
+  int nitems;   // grouped kernels (rows_per_wave > 1): waves listed in witems
+  const AERO_GLOBAL uint32_t* witems;  // per wave: its first thread in the one-row grid / 64, | WITEM_GROUP
   int nt_store;  // population stores bypass the caches (launch working set beyond the last-level cache)
   int ibm_on;      // immersed-boundary mode: no link bounce-back; Guo forcing of FLAG_IBM cells
   int sponge_x0;   // first column with sponge sigma > 0 (nx when there is none)

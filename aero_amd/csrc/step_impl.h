@@ -1187,12 +1187,63 @@ struct StepWaves {
                                                                                                     : 1;
 };
 
-// The step of one cell (or, in block 0, the previous step's force
-// reduction), expanded textually into each step kernel with `p` (the case's
-// StepParams), `dyn` (this step's StepDyn) and `blk` (the case-local block
-// index) in scope.  A macro rather than an inlined function: with the body in
-// a callee the register allocator's outcome for the 2-byte storage forms
-// degrades sharply (heavy spilling at the same occupancy target).
+// Bulk rows of a grouped wave (rows_per_wave > 1, D2Q9): rows y0 .. y0+RG-1,
+// columns x of 64 aligned lanes, none of them flagged, at a domain edge, in
+// the sponge or on a wall row.  Every pull of the RG rows is issued, into
+// the raw storage type, before the first row is converted and collided, so
+// the wave keeps RG x Q loads in flight.  Each row then runs what the
+// one-row bulk path runs: the same conversion (cvt(raw) == ld), the same
+// finish_cell.
+template <class L, int COLL, int ST, int KMODE, int RG>
+__device__ __forceinline__ void step_rows_bulk(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
+                                               int x, int y0, unsigned xw) {
+  using S = Storage<ST>;
+  using C = typename S::C;
+  using T = typename S::T;
+  constexpr int Q = L::Q;
+  static_assert(L::DIM == 2 && sizeof(T) == 2 && RG % 2 == 0, "grouped rows: D2Q9, 2-byte storage");
+  // rows r and r + 1 of a population share one register (16-bit halves)
+  typedef unsigned short pair_t __attribute__((ext_vector_type(2)));
+  pair_t raw[RG / 2][Q];
+#pragma unroll
+  for (int r = 0; r < RG; ++r) {
+    const int y = y0 + r;
+    const int ym = y == 0 ? p.ny - 1 : y - 1, yp = y == p.ny - 1 ? 0 : y + 1;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int xs = L::EX[i] == 0 ? x : (L::EX[i] > 0 ? x - 1 : x + 1);  // no lane at x = 0 or nx - 1
+      const int ys = L::EY[i] == 0 ? y : (L::EY[i] > 0 ? ym : yp);
+      const T v = S::raw(srow(p, B, ys, 0) + (unsigned)(i * p.xp), (unsigned)xs);
+      raw[r / 2][i][r % 2] = __builtin_bit_cast(unsigned short, v);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int r = 0; r < RG; ++r) {
+    const int y = y0 + r;
+    C fq[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+      fq[i] = S::cvt(__builtin_bit_cast(T, (unsigned short)raw[r / 2][i][r % 2]), (C)p.fsh[fsh_class<L>(i)]);
+    finish_cell<L, COLL, ST, KMODE>(p, y * p.nx + x, (size_t)(unsigned)y * (unsigned)p.qxp + xw, false, fq);
+  }
+}
+
+template <class L, int COLL, int ST, bool THERMAL>
+struct RowsPerWave {
+  static constexpr int value = rows_per_wave(L::Q, COLL, ST, THERMAL);
+};
+
+// The step of the cells of one wave (or, in block 0, the previous step's
+// force reduction), expanded textually into each step kernel with `p` (the
+// case's StepParams), `dyn` (this step's StepDyn) and `blk` (the case-local
+// block index) in scope.  A macro rather than an inlined function: with the
+// body in a callee the register allocator's outcome for the 2-byte storage
+// forms degrades sharply (heavy spilling at the same occupancy target).
+// A wave covers 64 aligned columns (xp is a multiple of 64) of one row, or,
+// for a group item of the grouped kernels' wave list, of RPW_GROUP rows
+// (step_rows_bulk).  Without a list (nitems 0: other kernels, or the launch
+// tuning's one-row setting) the step's thread grid is nrow * xp.
 #define AERO_STEP_BODY(L, COLL, ST, THERMAL, KMODE)                                                        \
   do {                                                                                                     \
     using S_ = Storage<ST>;                                                                                \
@@ -1206,20 +1257,46 @@ struct StepWaves {
       }                                                                                                    \
       break;                                                                                               \
     }                                                                                                      \
-    /* one thread per padded cell; thread indices fit in 31 bits (checked when the context is */          \
-    /* created).  xp is a multiple of 64, so a wave covers 64 aligned cells of one row: t0_, row, y, */    \
-    /* z and the storage base sb_ of the wave's first cell are wave-uniform (scalar registers). */          \
-    const unsigned t0_ = (unsigned)(blk - 1) * (unsigned)BLOCK + __builtin_amdgcn_readfirstlane(threadIdx.x);\
-    const unsigned row_ = udiv(t0_, p.div_xp);                                                             \
+    /* T0_: the wave's first thread in the one-row grid; thread indices fit in 31 bits (checked */        \
+    /* when the context is created).  T0_, the row, xw_ (the wave's first column) and every row */         \
+    /* base are wave-uniform (scalar registers). */                                                        \
+    constexpr bool LISTED_ = RowsPerWave<L, COLL, ST, THERMAL>::value > 1;                                 \
+    unsigned T0_;                                                                                          \
+    bool group_ = false;                                                                                   \
+    if (LISTED_ && p.nitems > 0) {                                                                         \
+      const unsigned wid_ = (unsigned)(blk - 1) * (unsigned)(BLOCK / 64) +                                 \
+                            ((unsigned)__builtin_amdgcn_readfirstlane(threadIdx.x) >> 6);                  \
+      if (wid_ >= (unsigned)p.nitems) break;                                                               \
+      const unsigned it_ = __builtin_amdgcn_readfirstlane(p.witems[wid_]);                                 \
+      group_ = (it_ & WITEM_GROUP) != 0u;                                                                  \
+      T0_ = (it_ & ~WITEM_GROUP) << 6;                                                                     \
+    } else {                                                                                               \
+      T0_ = (unsigned)(blk - 1) * (unsigned)BLOCK + __builtin_amdgcn_readfirstlane(threadIdx.x);           \
+    }                                                                                                      \
+    const unsigned row_ = udiv(T0_, p.div_xp);                                                             \
     if (row_ >= (unsigned)p.nrow) break;                                                                   \
-    const unsigned xw_ = t0_ - row_ * (unsigned)p.xp;                                                      \
+    const unsigned xw_ = T0_ - row_ * (unsigned)p.xp;                                                      \
     const int x = (int)(xw_ + __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)));          \
     if (x >= p.nx) break; /* row padding */                                                                \
+    const T_* B = (const T_*)p.fin;                                                                        \
+    if constexpr (LISTED_) {                                                                               \
+      if (group_) {                                                                                        \
+        step_rows_bulk<L, COLL, ST, KMODE, RPW_GROUP>(p, B, x, (int)row_, xw_);                            \
+        break;                                                                                             \
+      }                                                                                                    \
+    }                                                                                                      \
+    AERO_STEP_ROW(L, COLL, ST, THERMAL, KMODE);                                                            \
+  } while (0)
+
+// The step of the wave's cells in grid row row_ (columns x = xw_ + lane),
+// inside AERO_STEP_BODY; a `break` ends the wave.
+#define AERO_STEP_ROW(L, COLL, ST, THERMAL, KMODE)                                                         \
+  do {                                                                                                     \
+    const unsigned t0_ = T0_; /* wave index t0_ / 64 indexes the wave flag words */                       \
     const int z = (int)udiv(row_, p.div_ny);                                                               \
     const int y = (int)row_ - z * p.ny;                                                                    \
     const int cell = (int)(row_ * (unsigned)p.nx) + x;                                                     \
     const size_t sb_ = (size_t)row_ * (unsigned)p.qxp + xw_;                                               \
-    const T_* B = (const T_*)p.fin;                                                                        \
     /* Bulk waves: no cell of the wave flagged (solid, link, IBM; one bit per 64 cells, a scalar */        \
     /* load) and no lane at a domain edge or in the sponge -> pull, [body force,] collide, store, */      \
     /* straight-line.  Every boundary condition below is a no-op for such a cell, so this is the */       \

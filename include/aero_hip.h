@@ -208,14 +208,17 @@ int aero_step_noalloc(aero_ctx* ctx, int32_t nsteps, int64_t step0, double* hist
 /* Launch tuning of the batched step (no counterpart in the reference: the
  * autotuner's table, aero_amd/autotune.py).  One entry per kernel group --
  * lattice q, collision, storage, scalar transport -- and grid class (floor
- * of log2 of the group's tile count, aero_tune_grid_class): xcd_run 0 keeps
- * the built-in tile order, -1 the natural order, k > 0 runs of k tiles per
- * XCD; nt -1 keeps the built-in non-temporal-store rule, 0 / 1
- * turn it off / on; graph 1 replays each chunk's step launches from a
- * captured HIP graph.  Entries change the schedule only, never the results.
- * aero_step_tiles returns the tile count of one context's step. */
+ * of log2 of the group's tile count with one row per wave,
+ * aero_tune_grid_class): xcd_run 0 keeps the built-in tile order, -1 the
+ * natural order, k > 0 runs of k tiles per XCD; nt -1 keeps the built-in
+ * non-temporal-store rule, 0 / 1 turn it off / on; graph 1 replays each
+ * chunk's step launches from a captured HIP graph; rowgroups -1 keeps the
+ * built-in wave shape (the 2-byte D2Q9 kernels step bulk rows four to a
+ * wave), 0 steps one row per wave.  Entries change the schedule only, never
+ * the results.  aero_step_tiles returns the tile count of one context's step
+ * with one row per wave (the grid-class basis). */
 int aero_tune_set(int32_t q, int32_t collision, int32_t storage, int32_t thermal, int32_t grid_class,
-                  int32_t xcd_run, int32_t nt, int32_t graph);
+                  int32_t xcd_run, int32_t nt, int32_t graph, int32_t rowgroups);
 int aero_tune_clear(void);
 int aero_tune_grid_class(int64_t ntiles, int32_t* grid_class);
 int aero_step_tiles(aero_ctx* ctx, int64_t* ntiles);

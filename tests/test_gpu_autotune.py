@@ -1,11 +1,12 @@
 """Launch tuning never changes a result (MI355X only).
 
-The same mix of cases -- 2D BGK / TRT / MRT cylinders in FP32 and FP16 and
-a 3D sphere -- stepped with the built-in schedule and with every knob of the
-tuning table moved (natural tile order, other XCD run lengths,
-non-temporal stores forced, chunks replayed from captured HIP graphs), and
-after the autotuner has picked its own entries: coefficient rows and final
-states must be bit-identical.
+The same mix of cases -- 2D BGK / TRT / MRT cylinders in FP32, FP16 and
+BF16 and a 3D sphere -- stepped with the built-in schedule and with every
+knob of the tuning table moved (natural tile order, other XCD run lengths,
+non-temporal stores forced, chunks replayed from captured HIP graphs, one
+row per wave instead of the grouped bulk rows of the 2-byte D2Q9 kernels),
+and after the autotuner has picked its own entries: coefficient rows and
+final states must be bit-identical.
 """
 
 import numpy as np
@@ -21,7 +22,8 @@ pytestmark = pytest.mark.gpu
 def _mix():
     return [_cylinder(160, 80, 6.0, "bgk", "f32", 1), _cylinder(96, 48, 5.0, "trt", "f16", 2),
             _cylinder(128, 64, 6.0, "mrt", "f32", 3), make_solver(3, G.load(3, "trt_bouzidi_conv_slip"), "f32"),
-            _cylinder(640, 96, 8.0, "bgk", "f16", 4), _cylinder(512, 128, 10.0, "trt", "bf16", 5)]
+            _cylinder(640, 96, 8.0, "bgk", "f16", 4), _cylinder(512, 128, 10.0, "trt", "bf16", 5),
+            _cylinder(384, 96, 8.0, "mrt", "f16", 6)]
 
 
 
@@ -41,19 +43,19 @@ def test_tuned_schedules_are_bit_identical():
         ref_rows, ref_f = _run(_mix())
         mix = _mix()
         for key in autotune.group_keys(mix):
-            autotune.set_entry(key, {"xcd_run": -1, "nt": 1, "graph": 1})
+            autotune.set_entry(key, {"xcd_run": -1, "nt": 1, "graph": 1, "rowgroups": 0})
         rows, fs = _run(mix)
         for a, b in zip(ref_rows, rows):
             np.testing.assert_array_equal(a, b)
         for a, b in zip(ref_f, fs):
             np.testing.assert_array_equal(a, b)
         # single-case stepping under the same table (a batch of one)
-        for k in (0, 4):
+        for k in (0, 4, 6):
             s, t = _mix()[k], _mix()[k]
             autotune.clear()
             r0 = s._advance(300)
             for key in autotune.group_keys([t]):
-                autotune.set_entry(key, {"xcd_run": 8, "nt": 0, "graph": 1})
+                autotune.set_entry(key, {"xcd_run": 8, "nt": 0, "graph": 1, "rowgroups": 0})
             np.testing.assert_array_equal(r0, t._advance(300))
             np.testing.assert_array_equal(s.f, t.f)
     finally:
