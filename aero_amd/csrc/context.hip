@@ -1027,7 +1027,9 @@ static void upload_flags(aero_ctx* c, const std::vector<uint8_t>& fl) {
   c->wave_any.alloc(sizeof(uint32_t) * words.size());
   ck(hipMemcpy(c->wave_any.p, words.data(), sizeof(uint32_t) * words.size(), hipMemcpyHostToDevice),
      "copy wave flags");
-  // grouped kernels: the wave list, row groups of RPW_GROUP rows by 64-column segments
+  // grouped kernels: the wave list, row groups of RPW_GROUP rows by 64-column segments;
+  // a group item where no cell of the group (nor the outlet's x = nx - 2 source) is flagged
+  // and no row is a wall row
   const aero_desc& d = c->d;
   if (rows_per_wave(d.q, d.collision, d.storage, d.thermal != 0) > 1) {
     const int RG = RPW_GROUP, segs = c->xp / 64;
@@ -1037,12 +1039,14 @@ static void upload_flags(aero_ctx* c, const std::vector<uint8_t>& fl) {
       // wall rows (y = 0, ny - 1; 2D, so a row is its y) take the one-row boundary path
       const bool rows_ok = r0 + RG <= c->nrow && (d.wall == AERO_WALL_NONE || (r0 > 0 && r0 + RG - 1 < d.ny - 1));
       for (int sgm = 0; sgm < segs; ++sgm) {
-        const int x0 = sgm * 64, x1 = std::min(x0 + 63, nx - 1);
-        // no inlet, outlet, sponge or padding lane (the last segment holds x = nx - 1)
-        bool bulk = rows_ok && x0 > 0 && x1 < nx - 1 && x1 < c->sponge_x0 && x0 + 63 < nx;
+        const int x1 = std::min(sgm * 64 + 63, nx - 1);
+        // a convective outlet row keeps per-row state: its waves step one row each
+        bool bulk = rows_ok && !(x1 == nx - 1 && d.outlet == AERO_OUT_CONVECTIVE);
         for (int r = r0; bulk && r < r0 + RG; ++r) {
           const size_t w = (size_t)r * segs + sgm;
           bulk = !((words[w >> 5] >> (w & 31)) & 1u);
+          // the outlet lane pulls x = nx - 2, in the previous wave when x = nx - 1 is lane 0
+          if (bulk && sgm * 64 == nx - 1) bulk = !((words[(w - 1) >> 5] >> ((w - 1) & 31)) & 1u);
         }
         if (bulk) {
           items.push_back(WITEM_GROUP | (uint32_t)((size_t)r0 * segs + sgm));

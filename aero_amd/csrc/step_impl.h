@@ -49,6 +49,18 @@ __device__ __forceinline__ StepDyn dyn_of(const StepParams& p) {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// `false` as a per-lane value the optimiser cannot see through.  The paths
+// that know their cells are fluid (bulk waves, grouped rows) pass it as the
+// collision's solid flag, so the collision compiles to the same arithmetic
+// as in the full path, where the flag is loaded: with a constant the
+// optimiser would fuse u = m / rho into the equilibrium's sums, and a cell's
+// last bit would depend on the path that stepped it.
+__device__ __forceinline__ bool opaque_false() {
+  int z = 0;
+  asm volatile("" : "+v"(z));
+  return z != 0;
+}
+
 // Grid coordinates of a linear cell index (x fastest).
 __device__ __forceinline__ void cell_xyz(const StepParams& p, unsigned cell, int& x, int& y, int& z) {
   const unsigned yz = udiv(cell, p.div_nx);
@@ -326,8 +338,12 @@ __device__ __forceinline__ void body_force(const StepParams& p, C fn[L::Q]) {
 }
 
 // ---------------------------------------------------------------- inlets
+// The 2D boundary conditions below (inlet, pressure outlet, sponge, walls)
+// compute with floating-point contraction off: a cell gets the same bits
+// whichever step path (one row per wave, grouped rows) inlines them.
 template <class C>
 __device__ __forceinline__ void inlet_2d(const StepParams& p, const StepDyn& dyn, int y, C f[9]) {
+#pragma clang fp contract(off)
   if (p.inlet_mode == IN_ZOUHE) {  // boundary.py:15-45
     const C u0 = (C)p.u0;
     const C rho = ((f[0] + f[2] + f[4]) + C(2) * (f[3] + f[6] + f[7])) / (C(1) - u0);
@@ -358,6 +374,17 @@ __device__ __forceinline__ void inlet_2d(const StepParams& p, const StepDyn& dyn
     f[5] = f[7] - C(0.5) * (f[2] - f[4]) + C(1.0 / 6.0) * rin * ux;
     f[8] = f[6] + C(0.5) * (f[2] - f[4]) + C(1.0 / 6.0) * rin * ux;
   }
+}
+
+// 2D Zou-He pressure outlet at x = nx-1 (boundary.py:139-159).
+template <class C>
+__device__ __forceinline__ void pressure_outlet_2d(const StepParams& p, C fn[9]) {
+#pragma clang fp contract(off)
+  const C rout = (C)p.rho_out;
+  const C ux = C(-1) + (fn[0] + fn[2] + fn[4] + C(2) * (fn[1] + fn[5] + fn[8])) / rout;
+  fn[3] = fn[1] - C(2.0 / 3.0) * rout * ux;
+  fn[7] = fn[5] + C(0.5) * (fn[2] - fn[4]) - C(1.0 / 6.0) * rout * ux;
+  fn[6] = fn[8] - C(0.5) * (fn[2] - fn[4]) - C(1.0 / 6.0) * rout * ux;
 }
 
 template <class C>
@@ -426,6 +453,7 @@ __device__ __forceinline__ void apply_inlet(const StepParams& p, const StepDyn& 
 // (boundary.py:162-226), in the reference's order; both read only the cell.
 template <class L, class C>
 __device__ __forceinline__ void sponge_and_walls(const StepParams& p, int x, int y, C fn[L::Q]) {
+#pragma clang fp contract(off)
   constexpr int Q = L::Q;
   if (p.sigma != nullptr && x >= p.sponge_x0) {  // sponge.py:24-70; sigma is 0 before sponge_x0
     const double s = p.sigma[x];
@@ -598,13 +626,7 @@ __device__ __forceinline__ int post_bc(const StepParams& p, const StepDyn& dyn,
     }
   }
   if constexpr (L::DIM == 2) {
-    if (x == p.nx - 1 && p.outlet_mode == OUT_PRESSURE) {  // boundary.py:139-159
-      const C rout = (C)p.rho_out;
-      const C ux = C(-1) + (fn[0] + fn[2] + fn[4] + C(2) * (fn[1] + fn[5] + fn[8])) / rout;
-      fn[3] = fn[1] - C(2.0 / 3.0) * rout * ux;
-      fn[7] = fn[5] + C(0.5) * (fn[2] - fn[4]) - C(1.0 / 6.0) * rout * ux;
-      fn[6] = fn[8] - C(0.5) * (fn[2] - fn[4]) - C(1.0 / 6.0) * rout * ux;
-    }
+    if (x == p.nx - 1 && p.outlet_mode == OUT_PRESSURE) pressure_outlet_2d<C>(p, fn);
   }
   sponge_and_walls<L, C>(p, x, y, fn);
   return rec;
@@ -1187,21 +1209,27 @@ struct StepWaves {
                                                                                                     : 1;
 };
 
-// Bulk rows of a grouped wave (rows_per_wave > 1, D2Q9): rows y0 .. y0+RG-1,
-// columns x of 64 aligned lanes, none of them flagged, at a domain edge, in
-// the sponge or on a wall row.  Every pull of the RG rows is issued, into
-// the raw storage type, before the first row is converted and collided, so
-// the wave keeps RG x Q loads in flight.  Each row then runs what the
-// one-row bulk path runs: the same conversion (cvt(raw) == ld), the same
-// finish_cell.
+// The rows of a group item (grouped kernels, D2Q9): rows y0 .. y0+RG-1 of 64
+// aligned columns x, none of them flagged or on a wall row (the inlet, a
+// zero-gradient or pressure outlet and sponge columns may be among them).
+// Every pull of the RG rows is issued, into the raw storage type, before the
+// first row is converted: the wave keeps RG x Q loads in flight.  Each row
+// then runs what the one-row path runs for such a cell, in the same order and
+// with the same arithmetic: the pull (the outlet column pulls x = nx-2, as
+// post_bc), the inlet at x = 0, the pressure outlet, the sponge, the
+// collision (cvt(raw) == ld).
 template <class L, int COLL, int ST, int KMODE, int RG>
-__device__ __forceinline__ void step_rows_bulk(const StepParams& p, const typename Storage<ST>::T* __restrict__ B,
-                                               int x, int y0, unsigned xw) {
+__device__ __forceinline__ void step_rows_group(const StepParams& p, const StepDyn& dyn,
+                                                const typename Storage<ST>::T* __restrict__ B, int x, int y0,
+                                                unsigned xw) {
   using S = Storage<ST>;
   using C = typename S::C;
   using T = typename S::T;
   constexpr int Q = L::Q;
   static_assert(L::DIM == 2 && sizeof(T) == 2 && RG % 2 == 0, "grouped rows: D2Q9, 2-byte storage");
+  const bool redirect = x == p.nx - 1 && (p.outlet_mode == OUT_ZEROGRAD || p.outlet_mode == OUT_CONVECTIVE);
+  const int xg = redirect ? p.nx - 2 : x;
+  const int xm = xg == 0 ? p.nx - 1 : xg - 1, xq = xg == p.nx - 1 ? 0 : xg + 1;
   // rows r and r + 1 of a population share one register (16-bit halves)
   typedef unsigned short pair_t __attribute__((ext_vector_type(2)));
   pair_t raw[RG / 2][Q];
@@ -1211,7 +1239,7 @@ __device__ __forceinline__ void step_rows_bulk(const StepParams& p, const typena
     const int ym = y == 0 ? p.ny - 1 : y - 1, yp = y == p.ny - 1 ? 0 : y + 1;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      const int xs = L::EX[i] == 0 ? x : (L::EX[i] > 0 ? x - 1 : x + 1);  // no lane at x = 0 or nx - 1
+      const int xs = L::EX[i] == 0 ? xg : (L::EX[i] > 0 ? xm : xq);
       const int ys = L::EY[i] == 0 ? y : (L::EY[i] > 0 ? ym : yp);
       const T v = S::raw(srow(p, B, ys, 0) + (unsigned)(i * p.xp), (unsigned)xs);
       raw[r / 2][i][r % 2] = __builtin_bit_cast(unsigned short, v);
@@ -1221,11 +1249,15 @@ __device__ __forceinline__ void step_rows_bulk(const StepParams& p, const typena
 #pragma unroll
   for (int r = 0; r < RG; ++r) {
     const int y = y0 + r;
-    C fq[Q];
+    C fn[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i)
-      fq[i] = S::cvt(__builtin_bit_cast(T, (unsigned short)raw[r / 2][i][r % 2]), (C)p.fsh[fsh_class<L>(i)]);
-    finish_cell<L, COLL, ST, KMODE>(p, y * p.nx + x, (size_t)(unsigned)y * (unsigned)p.qxp + xw, false, fq);
+      fn[i] = S::cvt(__builtin_bit_cast(T, (unsigned short)raw[r / 2][i][r % 2]), (C)p.fsh[fsh_class<L>(i)]);
+    if (xg == 0) apply_inlet<L, C>(p, dyn, y, 0, fn);
+    if (x == p.nx - 1 && p.outlet_mode == OUT_PRESSURE) pressure_outlet_2d<C>(p, fn);
+    sponge_and_walls<L, C>(p, x, y, fn);  // no wall row among them
+    finish_cell<L, COLL, ST, KMODE>(p, y * p.nx + x, (size_t)(unsigned)y * (unsigned)p.qxp + xw, opaque_false(),
+                                    fn);
   }
 }
 
@@ -1242,7 +1274,7 @@ struct RowsPerWave {
 // forms degrades sharply (heavy spilling at the same occupancy target).
 // A wave covers 64 aligned columns (xp is a multiple of 64) of one row, or,
 // for a group item of the grouped kernels' wave list, of RPW_GROUP rows
-// (step_rows_bulk).  Without a list (nitems 0: other kernels, or the launch
+// (step_rows_group).  Without a list (nitems 0: other kernels, or the launch
 // tuning's one-row setting) the step's thread grid is nrow * xp.
 #define AERO_STEP_BODY(L, COLL, ST, THERMAL, KMODE)                                                        \
   do {                                                                                                     \
@@ -1281,7 +1313,7 @@ struct RowsPerWave {
     const T_* B = (const T_*)p.fin;                                                                        \
     if constexpr (LISTED_) {                                                                               \
       if (group_) {                                                                                        \
-        step_rows_bulk<L, COLL, ST, KMODE, RPW_GROUP>(p, B, x, (int)row_, xw_);                            \
+        step_rows_group<L, COLL, ST, KMODE, RPW_GROUP>(p, dyn, B, x, (int)row_, xw_);                            \
         break;                                                                                             \
       }                                                                                                    \
     }                                                                                                      \
@@ -1300,8 +1332,7 @@ struct RowsPerWave {
     /* Bulk waves: no cell of the wave flagged (solid, link, IBM; one bit per 64 cells, a scalar */        \
     /* load) and no lane at a domain edge or in the sponge -> pull, [body force,] collide, store, */      \
     /* straight-line.  Every boundary condition below is a no-op for such a cell, so this is the */       \
-    /* same arithmetic; being a separate copy, the compiler may contract it into fused multiply-adds */   \
-    /* differently, so a rare population can differ in its last bit from the full sequence.  Only */      \
+    /* same arithmetic (the solid flag passed opaque, as the full sequence's loaded one).  Only */         \
     /* D2Q9 BGK/TRT keep it (+6-8% in FP16 batches); in D3Q19 the full sequence below, which skips */     \
     /* the flag load in unflagged waves, measured as fast (and faster on small grids), and MRT */         \
     /* kernels are register-bound. */                                                                     \
@@ -1313,7 +1344,7 @@ struct RowsPerWave {
         C_ fq_[L::Q];                                                                                      \
         gather<L, ST>(p, B, x, y, z, (C_)p.u0, fq_);                                                       \
         if (L::DIM == 3 && p.body_force_on) body_force<L, C_>(p, fq_);                                     \
-        finish_cell<L, COLL, ST, KMODE>(p, cell, sb_, false, fq_);                                         \
+        finish_cell<L, COLL, ST, KMODE>(p, cell, sb_, opaque_false(), fq_);                                \
         break;                                                                                             \
       }                                                                                                    \
     }                                                                                                      \
