@@ -98,6 +98,7 @@ _SIGS = {
     "aero_batch_create": ([ctypes.POINTER(_P)], ctypes.c_int),
     "aero_batch_destroy": ([_P], ctypes.c_int),
     "aero_batch_step": ([_P, ctypes.c_int32, _P, ctypes.c_int32, _P, _P, _P], ctypes.c_int),
+    "aero_batch_check": ([_P, ctypes.c_int32, _P, _P], ctypes.c_int),
 }
 EXPORTED = tuple(_SIGS)
 

@@ -50,6 +50,16 @@ class Batch:
                 s._record(rows)
         return hists
 
+    def check_stats(self, solvers: Sequence) -> np.ndarray:
+        """check_stability's device statistics of every solver's current
+        state (as Solver._device_stats, one row each), computed in one call."""
+        m = len(solvers)
+        out = np.empty((m, _abi.NCHECK), dtype=np.float64)
+        if m:
+            ctxs = (ctypes.c_void_p * m)(*[s._ctx.handle.value for s in solvers])
+            _abi.check(_abi.lib().aero_batch_check(self._h, ctypes.c_int32(m), ctxs, out.ctypes.data))
+        return out
+
     def close(self):
         if self._h:
             _abi.lib().aero_batch_destroy(self._h)

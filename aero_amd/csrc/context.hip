@@ -237,6 +237,38 @@ struct StepRunner {
 
 static std::atomic<unsigned long long> g_ctx_serial{0};
 
+// Streams of destroyed contexts, per device, handed to the next contexts
+// created there: creating and destroying a HIP stream costs about a
+// millisecond, which a serving loop that retires and admits cases every
+// chunk would pay per case.  A pooled stream is idle (synchronised before it
+// is returned) and keeps the default priority.
+namespace {
+std::mutex g_stream_mu;
+std::map<int, std::vector<hipStream_t>> g_stream_pool;
+
+hipStream_t take_stream(int device) {
+  {
+    std::lock_guard<std::mutex> lock(g_stream_mu);
+    std::vector<hipStream_t>& v = g_stream_pool[device];
+    if (!v.empty()) {
+      hipStream_t s = v.back();
+      v.pop_back();
+      return s;
+    }
+  }
+  hipStream_t s = nullptr;
+  ck(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+  return s;
+}
+
+void return_stream(int device, hipStream_t s) {
+  if (!s) return;
+  (void)hipStreamSynchronize(s);
+  std::lock_guard<std::mutex> lock(g_stream_mu);
+  g_stream_pool[device].push_back(s);
+}
+}  // namespace
+
 struct aero_ctx {
   aero_desc d;
   unsigned long long serial = ++g_ctx_serial;  // unique per context for the process lifetime
@@ -501,13 +533,13 @@ struct aero_ctx {
     ck(launch_reduce_recycling(recyc.as<double>(), nrow, d.u0, s), "recycling reduce");
   }
 
-  // f64 post-boundary state of the last completed step into fcache.
-  void materialize_f() {
-    if (fcache_valid) return;
+  // Parameters that re-derive the post-boundary state of the last completed
+  // step (materialize_kernel), queued on stream s (the recycling inlet's
+  // plane of that step first).
+  StepParams materialize_params(hipStream_t s) {
     if (!has_state) throw std::runtime_error("no state: call aero_set_f first");
-    fcache.alloc(sizeof(double) * Q * N);
     const int old = side ^ 1;
-    if (d.inlet == AERO_IN_RECYCLING) recycling_prepass(old);
+    if (d.inlet == AERO_IN_RECYCLING) recycling_prepass(old, s);
     StepParams p = base_params();
     p.fin = (decltype(p.fin))(B[old].p);
     p.gin = (decltype(p.gin))(G[old].p);
@@ -516,10 +548,34 @@ struct aero_ctx {
     p.phase = 0.17 * (double)last_step_no;
     p.g_bc_skip = last_g_skip;
     p.commit = 0;
+    return p;
+  }
+
+  // f64 post-boundary state of the last completed step into fcache.
+  void materialize_f() {
+    if (fcache_valid) return;
+    if (!has_state) throw std::runtime_error("no state: call aero_set_f first");
+    fcache.alloc(sizeof(double) * Q * N);
+    StepParams p = materialize_params(stream);
     p.out_mode = 0;
     p.fmat = (decltype(p.fmat))(fcache.as<double>());
     ck(launch_materialize(Q, d.storage, d.thermal != 0, p, stream), "materialize launch");
     fcache_valid = true;
+  }
+
+  // check_stability's statistics (diagnostics.py:32-71) of the post-boundary
+  // state of the last completed step, reduced on the device into out (NCHECK
+  // doubles), queued on stream s: per-block partials as the state is
+  // re-derived (3D: the z mid-plane only, solver3d.py:608-615), then one
+  // block sums them in block order.
+  void check_stats(hipStream_t s, double* out) {
+    checkpart.alloc(sizeof(double) * NCHECK * nblocks);
+    StepParams p = materialize_params(s);
+    p.out_mode = 2;
+    p.check_z = Q == 19 ? d.nz / 2 : -1;
+    p.check_part = (decltype(p.check_part))(checkpart.as<double>());
+    ck(launch_materialize(Q, d.storage, d.thermal != 0, p, s), "check launch");
+    ck(launch_reduce_check(checkpart.as<double>(), nblocks, out, s), "check reduce");
   }
 
   void materialize_g() {
@@ -854,7 +910,7 @@ int aero_create(const aero_desc* desc, aero_ctx** out) {
     c->esz = storage_bytes(d.storage);
     c->csz = d.storage == ST_F64 ? 8 : 4;
     ck(hipGetDevice(&c->device), "hipGetDevice");
-    ck(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    c->stream = take_stream(c->device);
     const size_t fbytes = c->state_bytes();
     c->B[0].alloc(fbytes);
     c->B[1].alloc(fbytes);
@@ -889,7 +945,6 @@ int aero_create(const aero_desc* desc, aero_ctx** out) {
     ck(hipMemcpy(c->minv_d.p, d.mrt_minv, sizeof(double) * 361, hipMemcpyHostToDevice), "copy minv");
     ck(hipMemcpy(c->minv_f.p, mf.data(), sizeof(float) * 361, hipMemcpyHostToDevice), "copy minv");
     c->recyc.alloc(sizeof(double) * (4 * (size_t)c->nrow + 1));
-    c->runner.reserve(1, c->chunk_steps());
     if (c->les_split()) {
       for (int b = 0; b < 2; ++b) {
         c->hist_dev[b].alloc(sizeof(double) * NCOEF * STEP_CHUNK);
@@ -925,7 +980,7 @@ int aero_destroy(aero_ctx* c) {
   sem_destroy(c->sem);
   c->sem = nullptr;
   DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags, &c->wave_any,
-                    &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
+                    &c->witems, &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
                     &c->damp, &c->les_c2d, &c->les_c2f, &c->halo_buf, &c->fterms, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
                     &c->gcache, &c->tmp, &c->checkpart, &c->checkout, &c->ring, &c->ring_rec};
   for (DevBuf* b : bufs) b->release();
@@ -935,7 +990,7 @@ int aero_destroy(aero_ctx* c) {
     if (c->hist_stage[b]) (void)hipHostFree(c->hist_stage[b]);
     if (c->chunk_ev[b]) (void)hipEventDestroy(c->chunk_ev[b]);
   }
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  return_stream(c->device, c->stream);
   delete c;
   return 0;
 }
@@ -954,6 +1009,7 @@ struct aero_batch {
   int device = 0;
   hipStream_t stream = nullptr;
   StepRunner runner;
+  DevBuf checkout;  // aero_batch_check: NCHECK doubles per case
 };
 
 int aero_batch_create(aero_batch** out) {
@@ -976,6 +1032,7 @@ int aero_batch_destroy(aero_batch* b) {
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   b->runner.release();
+  b->checkout.release();
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return 0;
@@ -1907,17 +1964,28 @@ int aero_macroscopic(aero_ctx* c, double* rho, double* ux, double* uy, double* u
 int aero_check(aero_ctx* c, double* stats) {
   return guarded([&] {
     c->set_device();
-    c->materialize_f();
-    const int zc = c->Q == 19 ? c->d.nz / 2 : -1;
-    c->checkpart.alloc(sizeof(double) * NCHECK * c->nblocks);
     c->checkout.alloc(sizeof(double) * NCHECK);
-    ck(launch_check_f64(c->Q, c->d.nx, c->d.ny, c->d.nz, zc, c->fcache.as<double>(), c->nblocks,
-                        c->checkpart.as<double>(), c->stream),
-       "check");
-    ck(launch_reduce_check(c->checkpart.as<double>(), c->nblocks, c->checkout.as<double>(), c->stream),
-       "check reduce");
+    c->check_stats(c->stream, c->checkout.as<double>());
     ck(hipMemcpyAsync(stats, c->checkout.p, sizeof(double) * NCHECK, hipMemcpyDeviceToHost, c->stream), "download");
     ck(hipStreamSynchronize(c->stream), "sync");
+  });
+}
+
+int aero_batch_check(aero_batch* b, int32_t n, aero_ctx* const* ctxs, double* stats) {
+  return guarded([&] {
+    if (n <= 0) return;
+    ck(hipSetDevice(b->device), "hipSetDevice");
+    b->checkout.alloc(sizeof(double) * NCHECK * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+      aero_ctx* c = ctxs[i];
+      if (!c) throw std::invalid_argument("null context in batch");
+      if (c->device != b->device) throw std::invalid_argument("every context of a batch must be on its device");
+      ck(hipStreamSynchronize(c->stream), "context sync");  // work queued on the context's own stream
+      c->check_stats(b->stream, b->checkout.as<double>() + (size_t)NCHECK * i);
+    }
+    ck(hipMemcpyAsync(stats, b->checkout.p, sizeof(double) * NCHECK * (size_t)n, hipMemcpyDeviceToHost, b->stream),
+       "download");
+    ck(hipStreamSynchronize(b->stream), "sync");
   });
 }
 
@@ -1955,7 +2023,7 @@ int aero_synchronize(aero_ctx* c) {
 int aero_device_bytes(aero_ctx* c, int64_t* bytes) {
   return guarded([&] {
     const DevBuf* bufs[] = {&c->B[0], &c->B[1], &c->G[0], &c->G[1], &c->prev[0], &c->prev[1], &c->flags, &c->wave_any,
-                            &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
+                            &c->witems, &c->lrec, &c->rec_of, &c->qdir, &c->fpart[0], &c->fpart[1], &c->sigma, &c->minv_d, &c->minv_f,
                             &c->damp, &c->les_c2d, &c->les_c2f, &c->halo_buf, &c->fterms, &c->planes[0], &c->planes[1], &c->recyc, &c->hist_dev[0], &c->hist_dev[1], &c->last_plane_buf, &c->fsplit, &c->usplit, &c->fcache,
                             &c->gcache, &c->tmp, &c->checkpart, &c->checkout, &c->ring, &c->ring_rec};
     int64_t t = 0;

@@ -633,18 +633,26 @@ __device__ __forceinline__ int post_bc(const StepParams& p, const StepDyn& dyn,
 }
 
 // Scalar transport + Boussinesq force (thermal.py:47-124, 149-222).  Pulls
-// the previous scalar post-collision state from G, applies the y-wall
-// Dirichlet values (unless the state was just set from a raw field), adds the
-// buoyancy force to fn and, with commit, writes the new post-collision g.
+// the previous scalar post-collision state from G (scalar_pull), applies the
+// y-wall Dirichlet values (unless the state was just set from a raw field),
+// adds the buoyancy force to fn and, with commit, writes the new
+// post-collision g.  The fused step issues the scalar's pull together with
+// the populations', before the flow's boundary conditions, so the two sets
+// of loads share one memory round trip.
 template <class L, int ST>
-__device__ __forceinline__ void scalar_step(const StepParams& p, const StepDyn& dyn, int cell, int x, int y, int z,
-                                            bool solid,
-                                            typename Storage<ST>::C fn[L::Q], bool write_g) {
+__device__ __forceinline__ void scalar_pull(const StepParams& p, int x, int y, int z,
+                                            typename Storage<ST>::C g[L::Q]) {
+  using S = Storage<ST>;
+  gather<L, ST, 1>(p, (const typename S::T*)p.gin, x, y, z, (typename S::C)p.gshift, g);
+}
+
+template <class L, int ST>
+__device__ __forceinline__ void scalar_apply(const StepParams& p, const StepDyn& dyn, int cell, int x, int y, int z,
+                                             bool solid, typename Storage<ST>::C fn[L::Q],
+                                             typename Storage<ST>::C g[L::Q], bool write_g) {
   using S = Storage<ST>;
   using C = typename S::C;
   constexpr int Q = L::Q;
-  C g[Q];
-  gather<L, ST, 1>(p, (const typename S::T*)p.gin, x, y, z, (C)p.gshift, g);
   if (!dyn.g_bc_skip) {
     if (y == 0) {
 #pragma unroll
@@ -689,6 +697,14 @@ __device__ __forceinline__ void scalar_step(const StepParams& p, const StepDyn& 
       put(j, solid ? wt : g[j] - om * (g[j] - geq_j));
     }
   }
+}
+
+template <class L, int ST>
+__device__ __forceinline__ void scalar_step(const StepParams& p, const StepDyn& dyn, int cell, int x, int y, int z,
+                                            bool solid, typename Storage<ST>::C fn[L::Q], bool write_g) {
+  typename Storage<ST>::C g[L::Q];
+  scalar_pull<L, ST>(p, x, y, z, g);
+  scalar_apply<L, ST>(p, dyn, cell, x, y, z, solid, fn, g, write_g);
 }
 
 // Equilibrium moments of the MRT basis in closed form.  With
@@ -1194,9 +1210,10 @@ __device__ __forceinline__ void reduce_forces_block(const ForceReduce& r) {
 // for 8 (64 VGPRs, a 3-VGPR spill on a rare path): a D2Q9 wave keeps only 9
 // two-byte loads in flight, so the extra wave per SIMD raised their batches
 // 5%; likewise D2Q9 MRT (+5%).  D3Q19 MRT in the 2-byte forms
-// targets 6 (the compiler's own choice, 5, measured 10% slower)
-// unless it carries the scalar too (6 waves spill there: 16% slower); FP64,
-// FP32 MRT and MRT with the scalar keep the compiler's choice.  The D3Q19
+// targets 6 (the compiler's own choice, 5, measured 10% slower), or 5 with
+// the scalar (the compiler's 4: 1.441 against 1.411 ms per step on the
+// 512 x 256 x 256 MRT + scalar configuration; 6 spills, 2.11 ms); FP64,
+// FP32 MRT and D2Q9 MRT with the scalar keep the compiler's choice.  The D3Q19
 // local-Smagorinsky kernels (KMODE 2) without the scalar target 6: at 512 x
 // 256 x 256, 0.852 vs 0.888 ms (FP32) and 0.513 vs 0.566 ms (FP16) at 7,
 // while the plain kernels stay fastest at 7 (0.832 vs 0.845, 0.463 vs 0.485).
@@ -1205,8 +1222,21 @@ struct StepWaves {
   static constexpr int value = (KMODE == 2 && L::Q == 19 && ST != ST_F64 && !THERMAL) ? 6
                               : (ST != ST_F64 && COLL != COLL_MRT)
                                   ? ((L::Q == 9 && (ST == ST_F16S || ST == ST_BF16S)) ? 8 : 7)
-                              : (COLL == COLL_MRT && (ST == ST_F16S || ST == ST_BF16S) && !THERMAL) ? (L::Q == 9 ? 8 : 6)
-                                                                                                    : 1;
+                              : (COLL == COLL_MRT && (ST == ST_F16S || ST == ST_BF16S)) ? (THERMAL ? (L::Q == 19 ? 5 : 1)
+                                                                                                   : (L::Q == 9 ? 8 : 6))
+                                                                                        : 1;
+};
+
+// The D3Q19 FP32 MRT kernel with the scalar issues the scalar's pull with the
+// populations' (scalar_pull): 1.885 against 2.005 ms per step on the
+// 512 x 256 x 256 MRT + scalar configuration, at the 4 waves per SIMD the
+// compiler then picks (6 without it).  Forcing 5 or 6 waves spills, and in
+// the 2-byte forms the early pull drops the kernel to 3 waves and measured
+// slower (1.521 against 1.411 ms at 5 waves without it); the BGK / TRT
+// kernels with the scalar spill heavily at their 7-wave target with it.
+template <class L, int COLL, int ST>
+struct ScalarPrefetch {
+  static constexpr bool value = COLL == COLL_MRT && L::Q == 19 && ST == ST_F32;
 };
 
 // The rows of a group item (grouped kernels, D2Q9): rows y0 .. y0+RG-1 of 64
@@ -1360,10 +1390,13 @@ struct RowsPerWave {
     const bool solid = flag & FLAG_SOLID;                                                                  \
     if (flag & FLAG_LINKS) link_pre_terms<L, ST>(p, B, link_record(p, cell), x, y, z);                    \
     __builtin_amdgcn_sched_barrier(0);                                                                     \
+    C_ g_[THERMAL ? L::Q : 1];                                                                             \
+    if constexpr (THERMAL && ScalarPrefetch<L, COLL, ST>::value) scalar_pull<L, ST>(p, x, y, z, g_);       \
     C_ fn[L::Q];                                                                                           \
     const int rec = post_bc<L, ST>(p, dyn, B, cell, x, y, z, flag, true, fn, pflag_, orow_.pv, orow_.coop,  \
                                   orow_.src);                                                              \
-    if constexpr (THERMAL) scalar_step<L, ST>(p, dyn, cell, x, y, z, solid, fn, true);                     \
+    if constexpr (THERMAL && !ScalarPrefetch<L, COLL, ST>::value) scalar_pull<L, ST>(p, x, y, z, g_);      \
+    if constexpr (THERMAL) scalar_apply<L, ST>(p, dyn, cell, x, y, z, solid, fn, g_, true);                \
     if (rec >= 0) link_post_sum<L, C_>(p, rec, fn);                                                        \
     /* the next step's IBM forcing acts on this post-boundary state (solver.py:280-285) */                 \
     if (p.ibm_on && (flag & FLAG_IBM)) ibm_force<L, C_>((C_)p.ibm_coeff, fn);                             \

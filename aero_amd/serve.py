@@ -328,13 +328,16 @@ class Server:
                 enable_ring(solver, w + self.chunk)  # room for the chunk stepped during the check
             live.append(_Live(rid, spec, solver, arr, clock() - t0))
 
-    def _stability(self, lv: _Live) -> None:
-        """check_stability on the case's state at the end of the last chunk,
-        as the reference's run loop does every check_every steps."""
-        err, warns = stability_findings(lv.solver._device_stats(), len(lv.solver.Cd_history))
-        lv.warnings += len(warns) > 0
-        if err is not None:
-            lv.error = err
+    def _stability(self, lvs: Sequence[_Live]) -> None:
+        """check_stability on each case's state at the end of the last chunk,
+        as the reference's run loop does every check_every steps; the
+        statistics of every case come from one device call."""
+        stats = self._batch_obj().check_stats([lv.solver for lv in lvs])
+        for lv, st in zip(lvs, stats):
+            err, warns = stability_findings(st, len(lv.solver.Cd_history))
+            lv.warnings += len(warns) > 0
+            if err is not None:
+                lv.error = err
 
     def serve(self, requests: Sequence[tuple], clock: Callable[[], float] = time.perf_counter,
               sleep: Callable[[float], None] = time.sleep) -> List[Result]:
@@ -428,8 +431,8 @@ class Server:
                 live = [lv for lv in live if id(lv) not in gone]
                 # the stability check at this chunk boundary; a case it stops
                 # is retired now, with its history through this chunk
+                self._stability(live)
                 for lv in list(live):
-                    self._stability(lv)
                     if lv.error is not None:
                         done.append(self._result(lv, "diverged", clock() - t0, len(lv.solver.Cd_history)))
                         lv.solver.close()

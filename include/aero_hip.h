@@ -270,6 +270,10 @@ int aero_stationarity_arrays(int32_t n, const double* const* cd, const double* c
 
 /* Observables of the current state (compute_macroscopic, d2q9.py:44-63). */
 int aero_macroscopic(aero_ctx* ctx, double* rho, double* ux, double* uy, double* uz);
+/* check_stability's statistics (diagnostics.py:32-71) of the current state:
+ * NCHECK doubles -- any NaN, any rho <= 0, max |u|, sum rho, cell count;
+ * 3D on the z mid-plane (solver3d.py:608-615).  aero_batch_check (below)
+ * computes them for many contexts at once. */
 int aero_check(aero_ctx* ctx, double* stats);
 /* Per-link-cell FP64 momentum-exchange sums of the last step (nrec x 12) and
  * the cells they belong to; *nrec returns the count (pass NULL arrays to query). */
@@ -304,6 +308,11 @@ int aero_batch_create(aero_batch** out);
 int aero_batch_destroy(aero_batch* batch);
 int aero_batch_step(aero_batch* batch, int32_t n, aero_ctx* const* ctxs, int32_t nsteps, const int64_t* step0s,
                     const double* const* planes, double* const* hists);
+/* aero_check for n contexts at once on the batch's stream (stats: n x
+ * NCHECK), one host synchronisation for all: the serving loop checks every
+ * live case after each chunk, as the reference's run loop does every
+ * check_every steps (solver.py:517-570). */
+int aero_batch_check(aero_batch* batch, int32_t n, aero_ctx* const* ctxs, double* stats);
 
 #ifdef __cplusplus
 }

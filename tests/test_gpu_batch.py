@@ -92,3 +92,42 @@ def test_batch_rejects_les_and_duplicates():
     with pytest.raises(RuntimeError, match="twice"):
         b.advance([plain, plain], 2)
     b.close()
+
+
+def _host_stats(s):
+    """check_stability's statistics from the materialised f64 field (the
+    reference's own check reads self.f; 3D on the z mid-plane)."""
+    f = np.asarray(s.f, dtype=np.float64)
+    if f.ndim == 4:
+        f = f[:, f.shape[1] // 2]
+    from aero_amd.lattice import E, E3
+    e = E if f.shape[0] == 9 else E3
+    rho = f.sum(axis=0)
+    mx = np.tensordot(e[:, 0].astype(np.float64), f, axes=1)
+    my = np.tensordot(e[:, 1].astype(np.float64), f, axes=1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = np.where(rho > 0, 1.0 / rho, 0.0)
+    with np.errstate(invalid="ignore", over="ignore"):
+        um = np.sqrt((inv * mx) ** 2 + (inv * my) ** 2)
+    um = np.where(np.isnan(um), 0.0, um)  # a NaN speed is flagged by the NaN test instead
+    return np.array([float(np.isnan(f).any()), float((rho <= 0).any()), float(um.max()), float(rho.sum()),
+                     float(rho.size)])
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_batched_stability_stats_match_each_case(storage):
+    """aero_batch_check (one call for every live case) gives each case's
+    aero_check statistics bit for bit, and both agree with the statistics of
+    the materialised float64 field."""
+    from aero_amd.batch import Batch
+    cases = _make_all(storage)
+    b = Batch()
+    b.advance(cases, 40, record=False)
+    many = b.check_stats(cases)
+    for k, s in enumerate(cases):
+        one = s._device_stats()
+        np.testing.assert_array_equal(many[k], one, err_msg=f"case {k}")
+        ref = _host_stats(s)
+        assert many[k][0] == ref[0] and many[k][1] == ref[1] and many[k][4] == ref[4], (k, many[k], ref)
+        np.testing.assert_allclose(many[k][2:4], ref[2:4], rtol=1e-12, err_msg=f"case {k}")
+    b.close()

@@ -93,6 +93,9 @@ class _FakeBatch:
     def __init__(self, clock):
         self.clock, self.calls = clock, []
 
+    def check_stats(self, solvers):
+        return np.array([s._device_stats() for s in solvers]).reshape(len(solvers), 5)
+
     def advance(self, solvers, n, record=True):
         self.calls.append(len(solvers))
         self.clock[0] += 0.01 * n / 100
