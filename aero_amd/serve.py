@@ -239,7 +239,7 @@ class Server:
     def __init__(self, chunk: int = 500, max_live: int = 512, mem_reserve: int = 8 << 30,
                  convergence_window: int = 5000, convergence_tol: float = 0.01, strouhal_window: int = 4096,
                  strouhal_tol: float = 0.05, batch=None, free_memory: Optional[Callable[[], int]] = None,
-                 keep_history: bool = False, build_workers: int = 8, device_stats: bool = True,
+                 keep_history: bool = False, build_workers: int = 16, device_stats: bool = True,
                  on_result: Optional[Callable[["Result"], None]] = None):
         """on_result, when given, is called with each Result as it is decided."""
         self.chunk = int(chunk)
