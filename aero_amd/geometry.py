@@ -12,6 +12,7 @@ and q values are bit-identical:
 
 from __future__ import annotations
 
+import threading
 from typing import List, Optional, Tuple
 
 import numpy as np
@@ -82,6 +83,9 @@ class Rectangle(Geometry):
         return np.sqrt(np.maximum(dx, 0.0) ** 2 + np.maximum(dy, 0.0) ** 2) + np.minimum(np.maximum(dx, dy), 0.0)
 
 
+_MPL_IMPORT = threading.Lock()
+
+
 class Polygon(Geometry):
     """Polygon with vertices in fractional coordinates (aero/geometry/polygon.py:16-62).
 
@@ -96,7 +100,8 @@ class Polygon(Geometry):
         self.vertices = list(vertices)
 
     def mark_solid(self, Ny, Nx):
-        from matplotlib.path import Path
+        with _MPL_IMPORT:  # case builders run on several threads; matplotlib's first import is not thread-safe
+            from matplotlib.path import Path
         verts = [(x * Nx, y * Ny) for x, y in self.vertices]
         n = len(verts)
         path = Path(list(verts) + [verts[0]], [Path.MOVETO] + [Path.LINETO] * (n - 1) + [Path.CLOSEPOLY])

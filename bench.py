@@ -295,6 +295,40 @@ def run_trt_les_form(name, steps, warmup):
 
 
 SLAB_FORMS = {"slab_512x256x32_f32": "f32", "slab_512x256x32_f16": "f16"}
+BATCH_FORMS = {"c4_batch_256_f32": "f32", "c4_batch_256_f16": "f16"}
+
+
+def run_batch_form(name, steps, warmup):
+    """The batched 2D workload's stepping alone: the 256 cases of
+    serve.c4_cases(256, seed=0) (nx 128-1024, ny = nx/2; BGK / TRT / MRT),
+    all live, advanced together through Batch.advance -- per step one launch
+    for each collision group.  Reports lattice updates over the wall time of
+    the timed steps (launches, chunk-end reductions and coefficient-row
+    downloads included) and the algorithmic bytes per step: 2 * 9 * s per
+    cell."""
+    from concurrent.futures import ThreadPoolExecutor
+    from aero_amd import serve
+    from aero_amd.batch import Batch
+    st = BATCH_FORMS[name]
+    specs = serve.c4_cases(256, seed=0, storage=st)
+    with ThreadPoolExecutor(16) as ex:
+        solvers = list(ex.map(lambda c: c.build(), specs))
+    cells = {}
+    for c in specs:
+        cells[c.collision] = cells.get(c.collision, 0) + c.cells
+    total = sum(cells.values())
+    b = Batch()
+    b.advance(solvers, warmup, record=False)
+    t0 = time.perf_counter()
+    b.advance(solvers, steps, record=False)
+    wall = time.perf_counter() - t0
+    b.close()
+    for s in solvers:
+        s.close()
+    esz = {"f32": 4, "f16": 2}[st]
+    return {"form": name, "cases": len(specs), "cells": total, "cells_by_collision": cells, "steps": steps,
+            "MLUPS": round(total * steps / wall / 1e6, 1), "ms_per_step": round(wall / steps * 1e3, 4),
+            "algorithmic_bytes_per_step": 2 * 9 * esz * total}
 
 
 def run_slab_form(name, steps, warmup):
@@ -508,7 +542,7 @@ def main():
     ap.add_argument("--serve-full", action="store_true",
                     help="serving leg on the full 256-case workload and the full arrival-rate sweep")
     ap.add_argument("--form", choices=sorted(FORMS) + sorted(LES_FORMS) + sorted(CYL2D_FORMS) + sorted(TRT_LES_FORMS)
-                    + sorted(SLAB_FORMS), default=None,
+                    + sorted(SLAB_FORMS) + sorted(BATCH_FORMS), default=None,
                     help="only time one storage form of the TRT sphere (prints one JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (calibrates the HBM byte counters)")
@@ -532,6 +566,9 @@ def main():
         return
     if a.form in SLAB_FORMS:
         print(json.dumps(run_slab_form(a.form, a.steps, a.warmup)), flush=True)
+        return
+    if a.form in BATCH_FORMS:
+        print(json.dumps(run_batch_form(a.form, a.steps, a.warmup)), flush=True)
         return
     if a.form:
         print(json.dumps(run_form(a.form, a.steps, a.warmup)), flush=True)
