@@ -1098,7 +1098,8 @@ __device__ __forceinline__ void write_coefficients(const double F[NFORCE], doubl
 // Fixed-order FP64 reduction of the per-link-cell momentum-exchange sums by
 // one block of BLOCK threads: thread t sums link cells t, t+256, ...
 // sequentially; the four waves then add their partials into a 64-lane LDS
-// row one wave after another, and a fixed tree reduces that row.  The order
+// row one wave after another, and a fixed tree of wavefront shuffles
+// reduces that row.  The order
 // never depends on timing or on what else runs, so a case reproduces its own
 // bits alone or in a batch, whichever launch performs the reduction.  6 KB of
 // LDS, so the step kernel that hosts it keeps its occupancy.
@@ -1176,18 +1177,19 @@ __device__ __forceinline__ void reduce_forces_block(const ForceReduce& r) {
     }
     __syncthreads();
   }
-  for (int h = 32; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) {
-#pragma unroll
-      for (int k = 0; k < NFORCE; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
+  // the 64-lane row by wavefront shuffles in wave 0: at each halving, lane
+  // i adds lane i + h (the fixed tree; lanes past the live half carry
+  // values nobody reads)
+  if (wave == 0) {
     double F[NFORCE];
 #pragma unroll
-    for (int k = 0; k < NFORCE; ++k) F[k] = red[k][0];
-    write_coefficients(F, r.fd, r.md, r.row);
+    for (int k = 0; k < NFORCE; ++k) F[k] = red[k][lane];
+#pragma unroll
+    for (int h = 32; h > 0; h >>= 1) {
+#pragma unroll
+      for (int k = 0; k < NFORCE; ++k) F[k] += __shfl_down(F[k], h, 64);
+    }
+    if (lane == 0) write_coefficients(F, r.fd, r.md, r.row);
   }
 }
 
