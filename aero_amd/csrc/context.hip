@@ -1086,15 +1086,13 @@ static void upload_flags(aero_ctx* c, const std::vector<uint8_t>& fl) {
      "copy wave flags");
   // grouped kernels: the wave list, row groups of RPW_GROUP rows by 64-column segments;
   // a group item where no cell of the group (nor the outlet's x = nx - 2 source) is flagged
-  // and no row is a wall row
   const aero_desc& d = c->d;
   if (rows_per_wave(d.q, d.collision, d.storage, d.thermal != 0) > 1) {
     const int RG = RPW_GROUP, segs = c->xp / 64;
     std::vector<uint32_t> items;
     items.reserve(nwave);
     for (int r0 = 0; r0 < c->nrow; r0 += RG) {
-      // wall rows (y = 0, ny - 1; 2D, so a row is its y) take the one-row boundary path
-      const bool rows_ok = r0 + RG <= c->nrow && (d.wall == AERO_WALL_NONE || (r0 > 0 && r0 + RG - 1 < d.ny - 1));
+      const bool rows_ok = r0 + RG <= c->nrow;
       for (int sgm = 0; sgm < segs; ++sgm) {
         const int x1 = std::min(sgm * 64 + 63, nx - 1);
         // a convective outlet row keeps per-row state: its waves step one row each

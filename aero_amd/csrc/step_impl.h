@@ -1242,14 +1242,14 @@ struct ScalarPrefetch {
 };
 
 // The rows of a group item (grouped kernels, D2Q9): rows y0 .. y0+RG-1 of 64
-// aligned columns x, none of them flagged or on a wall row (the inlet, a
-// zero-gradient or pressure outlet and sponge columns may be among them).
+// aligned columns x, none of them flagged (the inlet, a zero-gradient or
+// pressure outlet, sponge columns and the wall rows may be among them).
 // Every pull of the RG rows is issued, into the raw storage type, before the
 // first row is converted: the wave keeps RG x Q loads in flight.  Each row
 // then runs what the one-row path runs for such a cell, in the same order and
 // with the same arithmetic: the pull (the outlet column pulls x = nx-2, as
-// post_bc), the inlet at x = 0, the pressure outlet, the sponge, the
-// collision (cvt(raw) == ld).
+// post_bc), the inlet at x = 0, the pressure outlet, the sponge and the
+// walls, the collision (cvt(raw) == ld).
 template <class L, int COLL, int ST, int KMODE, int RG>
 __device__ __forceinline__ void step_rows_group(const StepParams& p, const StepDyn& dyn,
                                                 const typename Storage<ST>::T* __restrict__ B, int x, int y0,
@@ -1287,7 +1287,7 @@ __device__ __forceinline__ void step_rows_group(const StepParams& p, const StepD
       fn[i] = S::cvt(__builtin_bit_cast(T, (unsigned short)raw[r / 2][i][r % 2]), (C)p.fsh[fsh_class<L>(i)]);
     if (xg == 0) apply_inlet<L, C>(p, dyn, y, 0, fn);
     if (x == p.nx - 1 && p.outlet_mode == OUT_PRESSURE) pressure_outlet_2d<C>(p, fn);
-    sponge_and_walls<L, C>(p, x, y, fn);  // no wall row among them
+    sponge_and_walls<L, C>(p, x, y, fn);
     finish_cell<L, COLL, ST, KMODE>(p, y * p.nx + x, (size_t)(unsigned)y * (unsigned)p.qxp + xw, opaque_false(),
                                     fn);
   }

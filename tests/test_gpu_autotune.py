@@ -19,11 +19,27 @@ from tests.test_gpu_parity import make_solver
 pytestmark = pytest.mark.gpu
 
 
+def _walled(wall_bc, collision, seed):
+    """A 2-byte 2D cylinder between no-slip or moving walls, pressure inlet
+    and outlet (the grouped rows' wall and pressure-outlet handling)."""
+    import aero_amd
+    from aero_amd.geometry import Cylinder
+    from aero_amd.lattice import compute_feq
+    ny, nx = 72, 320
+    solid = Cylinder(radius=7.0, cx_frac=0.3).mark_solid(ny, nx)
+    s = aero_amd.Solver(Ny=ny, Nx=nx, solid=solid, omega=1.6, u0=0.05, D=14.0, collision=collision, storage="f16",
+                        wall_bc=wall_bc, wall_velocity_top=0.03, wall_velocity_bottom=-0.01,
+                        inlet_bc="pressure", outlet_bc="pressure", rho_in=1.002, rho_out=1.0)
+    rng = np.random.default_rng(seed)
+    s.f = compute_feq(np.ones((ny, nx)), np.full((ny, nx), 0.05), rng.uniform(-1e-4, 1e-4, (ny, nx)))
+    return s
+
+
 def _mix():
     return [_cylinder(160, 80, 6.0, "bgk", "f32", 1), _cylinder(96, 48, 5.0, "trt", "f16", 2),
             _cylinder(128, 64, 6.0, "mrt", "f32", 3), make_solver(3, G.load(3, "trt_bouzidi_conv_slip"), "f32"),
             _cylinder(640, 96, 8.0, "bgk", "f16", 4), _cylinder(512, 128, 10.0, "trt", "bf16", 5),
-            _cylinder(384, 96, 8.0, "mrt", "f16", 6)]
+            _cylinder(384, 96, 8.0, "mrt", "f16", 6), _walled("moving", "trt", 7), _walled("noslip", "mrt", 8)]
 
 
 
@@ -50,7 +66,7 @@ def test_tuned_schedules_are_bit_identical():
         for a, b in zip(ref_f, fs):
             np.testing.assert_array_equal(a, b)
         # single-case stepping under the same table (a batch of one)
-        for k in (0, 4, 6):
+        for k in (0, 4, 6, 7):
             s, t = _mix()[k], _mix()[k]
             autotune.clear()
             r0 = s._advance(300)
