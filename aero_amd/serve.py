@@ -8,8 +8,9 @@ advances all live cases together, ``chunk`` steps at a time, through
 share a kernel.  Between chunks it
 
 * runs the reference's stability check on every live case
-  (check_stability, aero/diagnostics.py:32-71, on device statistics): a case
-  it would stop with an error is retired as diverged, and its warnings are
+  (check_stability, aero/diagnostics.py:32-71, on device statistics taken
+  for all cases in one call at the start of the next chunk): a case it
+  would stop with an error is retired as diverged, and its warnings are
   counted;
 * retires every case that met the reference's auto-stop criterion
   (``detect_statistical_stationarity``, aero/diagnostics.py:197-228, on the
@@ -231,6 +232,8 @@ class _Live:
     admitted: float
     warnings: int = 0
     error: Optional[str] = None  # set when check_stability would have raised
+    steps: int = 0               # steps taken, the last chunk's included
+    rows: Optional[np.ndarray] = None  # the last chunk's rows, not yet in the histories
 
 
 class Server:
@@ -328,11 +331,24 @@ class Server:
                 enable_ring(solver, w + self.chunk)  # room for the chunk stepped during the check
             live.append(_Live(rid, spec, solver, arr, clock() - t0))
 
-    def _stability(self, lvs: Sequence[_Live]) -> None:
-        """check_stability on each case's state at the end of the last chunk,
-        as the reference's run loop does every check_every steps; the
-        statistics of every case come from one device call."""
-        stats = self._batch_obj().check_stats([lv.solver for lv in lvs])
+    @staticmethod
+    def _chunk(batch, solvers, n):
+        """Device work of one chunk: check_stability's statistics of every
+        case's current state (one call for all), then n steps of every case."""
+        stats = batch.check_stats(solvers)
+        return stats, batch.advance(solvers, n, False)
+
+    @staticmethod
+    def _take_rows(lv: "_Live") -> None:
+        """Record the case's last chunk of coefficient rows in its histories."""
+        if lv.rows is not None:
+            lv.solver._record(lv.rows)
+            lv.rows = None
+
+    def _stability(self, lvs: Sequence["_Live"], stats) -> None:
+        """check_stability on each case's state at a chunk boundary (its
+        history so far), as the reference's run loop does every check_every
+        steps; stats holds one row of device statistics per case."""
         for lv, st in zip(lvs, stats):
             err, warns = stability_findings(st, len(lv.solver.Cd_history))
             lv.warnings += len(warns) > 0
@@ -346,13 +362,16 @@ class Server:
 
         Admitted requests are built (geometry, links, device state) by a pool
         of builder threads while the device steps the live cases; a built
-        case joins the batch at the next chunk.  The host's stop checks of
-        chunk j run while the device advances chunk j+1 (a worker thread
-        holds the device call).  A case found done at chunk j is retired with
-        its history cut at the end of chunk j, exactly what it would have
-        been alone; its speculative chunk is discarded.  A case whose history
+        case joins the batch at the next chunk.  Each chunk's device work (a
+        worker thread holds the call) first takes every case's stability
+        statistics at the chunk boundary, then steps; meanwhile the host
+        records the previous chunk's rows and runs the stop test on them.  A
+        case found done at the end of chunk j (check_stability's error, then
+        the auto-stop test, the reference's order) is retired with its
+        history cut at the end of chunk j, exactly what it would have been
+        alone; its speculative chunk j+1 is discarded.  A case whose history
         has reached max_steps is decided before the next chunk instead (it
-        stops whatever the check finds), so it costs no speculative chunk."""
+        stops whatever the checks find), so it costs no speculative chunk."""
         from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
         if any(r[1].shape == "polygon" for r in requests):
             import matplotlib.path  # noqa: F401  (imported once here, not raced by the builder threads)
@@ -391,52 +410,53 @@ class Server:
                     continue
                 # A case whose history has reached max_steps stops at this check
                 # whatever it finds, so it is decided now rather than stepped for
-                # a speculative chunk: same reason, same history cut, same result.
-                now = clock() - t0
-                at_limit = [lv for lv in live if len(lv.solver.Cd_history) >= lv.spec.max_steps]
-                stops = self._stops(at_limit)
-                for lv in at_limit:
-                    done.append(self._result(lv, self._done(lv, stops.get(id(lv), False)), now,
-                                             len(lv.solver.Cd_history)))
-                    lv.solver.close()
-                    live.remove(lv)
+                # a speculative chunk: its last chunk's rows are recorded, the
+                # stability check of its last state runs, then the stop test.
+                at_limit = [lv for lv in live if lv.steps >= lv.spec.max_steps]
+                if at_limit:
+                    for lv in at_limit:
+                        self._take_rows(lv)
+                    self._stability(at_limit, self._batch_obj().check_stats([lv.solver for lv in at_limit]))
+                    stops = self._stops(at_limit)
+                    now = clock() - t0
+                    for lv in at_limit:
+                        done.append(self._result(lv, self._done(lv, stops.get(id(lv), False)), now,
+                                                 len(lv.solver.Cd_history)))
+                        lv.solver.close()
+                        live.remove(lv)
                 if not live:
                     continue
                 n = self.chunk
                 batch = self._batch_obj()
                 stepping = list(live)
                 tb = clock()
-                fut = pool.submit(batch.advance, [lv.solver for lv in stepping], n, False)
-                # stop checks on the rows recorded so far, while the device steps
-                retire = []
-                stops = self._stops(stepping)
+                # the device checks every case's state at this boundary (the end
+                # of its last chunk), then steps the next chunk
+                fut = pool.submit(self._chunk, batch, [lv.solver for lv in stepping], n)
+                # while it runs: the last chunk's rows join the histories and the
+                # stop test reads them
                 for lv in stepping:
-                    if lv.solver.Cd_history:
-                        reason = self._done(lv, stops.get(id(lv), False))
-                        if reason is not None:
-                            retire.append((lv, reason, len(lv.solver.Cd_history)))
-                rows = fut.result()
+                    self._take_rows(lv)
+                stops = self._stops(stepping)
+                stats, rows = fut.result()
                 self.busy_seconds += clock() - tb
+                # decisions at the boundary: check_stability first, as the
+                # reference's run loop, then the stop test; a case they retire
+                # keeps its history through the boundary and its speculative
+                # chunk is discarded
+                checked = [k for k, lv in enumerate(stepping) if lv.solver.Cd_history]
+                self._stability([stepping[k] for k in checked], stats[checked])
                 now = clock() - t0
-                gone = set()
-                for lv, reason, upto in retire:
-                    done.append(self._result(lv, reason, now, upto))
-                    gone.add(id(lv))
                 for lv, r in zip(stepping, rows):
-                    if id(lv) in gone:
-                        lv.solver.close()
-                    else:
-                        lv.solver._record(r)
-                        self.steps_done += n * lv.spec.cells
-                live = [lv for lv in live if id(lv) not in gone]
-                # the stability check at this chunk boundary; a case it stops
-                # is retired now, with its history through this chunk
-                self._stability(live)
-                for lv in list(live):
-                    if lv.error is not None:
-                        done.append(self._result(lv, "diverged", clock() - t0, len(lv.solver.Cd_history)))
+                    reason = self._done(lv, stops.get(id(lv), False)) if lv.solver.Cd_history else None
+                    if reason is not None:
+                        done.append(self._result(lv, reason, now, len(lv.solver.Cd_history)))
                         lv.solver.close()
                         live.remove(lv)
+                    else:
+                        lv.rows = r
+                        lv.steps += n
+                        self.steps_done += n * lv.spec.cells
         finally:
             builders.shutdown(wait=True)
             pool.shutdown(wait=True)
