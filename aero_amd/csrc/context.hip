@@ -80,26 +80,19 @@ constexpr size_t NT_STORE_BYTES = (size_t)192 << 20;
 
 // MI355X hands the workgroups of a launch to its 8 XCDs round-robin (block b
 // to XCD b mod 8), and each XCD has its own L2.  Launch slots are grouped in
-// rounds of XCDS * run: within a round XCD x takes `run` consecutive tiles of
-// the natural order, so the rows a block pulls from its neighbours (x +- 1,
-// y +- 1) were read by the same XCD a moment earlier.  One round over the
-// whole list gives every XCD one contiguous eighth of the cells (best when
-// the z +- 1 planes of all eight eighths fit the last-level cache together:
-// 2D grids, small 3D ones); short runs keep the whole device sweeping the grid
-// together, so large 3D grids still find the planes at z +- 1 in that cache.
-constexpr int XCDS = 8;
+// rounds of XCDS * run (xcd_block): within a round XCD x takes `run`
+// consecutive tiles of the natural order, so the rows a block pulls from its
+// neighbours (x +- 1, y +- 1) were read by the same XCD a moment earlier.
+// One round over the whole list gives every XCD one contiguous eighth of the
+// cells (best when the z +- 1 planes of all eight eighths fit the last-level
+// cache together: 2D grids, small 3D ones); short runs keep the whole device
+// sweeping the grid together, so large 3D grids still find the planes at
+// z +- 1 in that cache.
 constexpr int XCD_RUN_LARGE = 8;
 constexpr size_t ZREUSE_BYTES = (size_t)64 << 20;
 static void xcd_order(const std::vector<int2>& nat, int run, std::vector<int2>& out) {
   const int n = (int)nat.size();
-  const int round = XCDS * run;
-  for (int r0 = 0; r0 < n; r0 += round) {
-    const int m = std::min(round, n - r0);
-    int off[XCDS + 1];
-    off[0] = 0;
-    for (int x = 0; x < XCDS; ++x) off[x + 1] = off[x] + (m > x ? (m - 1 - x) / XCDS + 1 : 0);
-    for (int b = 0; b < m; ++b) out.push_back(nat[r0 + off[b % XCDS] + b / XCDS]);
-  }
+  for (int pos = 0; pos < n; ++pos) out.push_back(nat[xcd_block(pos, run, n)]);
 }
 
 size_t storage_bytes(int st) {
@@ -1619,14 +1612,7 @@ struct NoallocLayout {
 __global__ __launch_bounds__(BLOCK) void noalloc_tiles_kernel(int2* tiles, int n, int run) {
   const int b = blockIdx.x * BLOCK + threadIdx.x;
   if (b >= n) return;
-  const int round = XCDS * run;
-  const int r0 = b / round * round;
-  const int j = b - r0;
-  const int m = min(round, n - r0);
-  const int x = j % XCDS;
-  int off = 0;
-  for (int xx = 0; xx < x; ++xx) off += m > xx ? (m - 1 - xx) / XCDS + 1 : 0;
-  tiles[b] = make_int2(0, r0 + off + j / XCDS);
+  tiles[b] = make_int2(0, xcd_block(b, run, n));
 }
 
 __global__ __launch_bounds__(BLOCK) void noalloc_zero_kernel(double* p, int n) {

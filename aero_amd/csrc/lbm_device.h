@@ -347,6 +347,23 @@ __host__ __device__ constexpr int padded_row(int nx) { return (nx + ROW_ALIGN - 
 // 48% of 8 TB/s with one row per wave, 71% with four).  Every other kernel
 // runs one row per wave over the whole nrow * xp thread grid.
 constexpr int RPW_GROUP = 4;
+
+// MI355X hands a launch's workgroups to its 8 XCDs round-robin (workgroup i
+// to XCD i mod 8).  Blocks in XCD order with runs of `run`: within each
+// round of 8 run positions, XCD x walks a contiguous run of the round's
+// blocks, so each XCD's L2 sees neighbouring rows.  xcd_block maps a launch
+// position to its block; the host's tile lists use the same order.
+constexpr int XCDS = 8;
+__host__ __device__ inline int xcd_block(int pos, int run, int n) {
+  if (run <= 0) return pos;
+  const int round = XCDS * run;
+  const int r0 = pos / round * round;
+  const int m = n - r0 < round ? n - r0 : round;
+  const int b = pos - r0, x = b % XCDS;
+  int off = 0;
+  for (int i = 0; i < x; ++i) off += m > i ? (m - 1 - i) / XCDS + 1 : 0;
+  return r0 + off + b / XCDS;
+}
 constexpr uint32_t WITEM_GROUP = 0x80000000u;
 __host__ __device__ constexpr int rows_per_wave(int q, int collision, int storage, bool thermal) {
   return (q == 9 && (storage == ST_F16S || storage == ST_BF16S) && !thermal) ? RPW_GROUP
