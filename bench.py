@@ -520,14 +520,19 @@ def serving(full=False):
         out["batched_2d"]["solves_per_hour"] / out["sequential_2d"]["solves_per_hour"], 2)
     out["batched_f16_over_sequential_f32_solves_per_hour"] = round(
         out["batched_2d_f16"]["solves_per_hour"] / out["sequential_2d"]["solves_per_hour"], 2)
-    for rate in ((2.0, 10.0) if not full else (1.0, 5.0, 10.0, 20.0, 50.0)):
+    legs = [(r, "f32") for r in ((2.0, 10.0) if not full else (1.0, 5.0, 10.0, 20.0, 50.0))]
+    if full:  # the FP16-shifted storage form of the same streams
+        legs += [(r, "f16") for r in (5.0, 20.0, 50.0)]
+    for rate, st in legs:
         n = 48 if not full else 200
-        reqs = serve.c5_requests(n, rate=rate, seed=int(rate), max_steps=4000 if not full else 30000)
-        srv = serve.Server(chunk=500, on_result=_Progress(f"poisson_{rate:g}", every=16))
+        reqs = serve.c5_requests(n, rate=rate, seed=int(rate), max_steps=4000 if not full else 30000, storage=st)
+        name = f"poisson_{rate:g}" + ("" if st == "f32" else f"_{st}")
+        srv = serve.Server(chunk=500, on_result=_Progress(name, every=16))
         t0 = time.perf_counter()
         r5 = srv.serve(reqs)
-        out[f"poisson_{rate:g}_per_s"] = dict(serve.summarize(r5, time.perf_counter() - t0, srv.steps_done,
-                                                              srv.busy_seconds), requests_rate_per_s=rate)
+        out[f"{name}_per_s" if st == "f32" else f"poisson_{rate:g}_per_s_{st}"] = dict(
+            serve.summarize(r5, time.perf_counter() - t0, srv.steps_done, srv.busy_seconds),
+            requests_rate_per_s=rate, storage=st)
     return out
 
 
