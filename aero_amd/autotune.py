@@ -173,7 +173,8 @@ def load_default() -> int:
 def _workloads():
     """The standard workloads the shipped table is tuned on: the D3Q19 TRT
     sphere at 512 x 256 x 256 and 128 x 64 x 64 in FP32 and FP16, the
-    64-case batched 2D draw in FP32 and FP16, and one 2D case alone."""
+    64-case batched 2D draw in FP32 and FP16, one case of that draw alone,
+    and the 400 x 200 cylinder alone in FP32 and FP16."""
     from . import serve
     out = []
     for st in ("f32", "f16"):
@@ -186,6 +187,10 @@ def _workloads():
     for st in ("f32", "f16"):
         out.append((f"c4_64_{st}", 200, serve.c4_cases(64, seed=0, storage=st)))
     out.append(("single2d_f32", 500, serve.c4_cases(1, seed=0)))
+    # the 400 x 200 cylinder alone (the reference's CPU-runnable configuration's grid)
+    for st in ("f32", "f16"):
+        out.append((f"cylinder400_{st}", 1000, [serve.CaseSpec(dim=2, nx=400, ny=200, shape="cylinder", size=(20.0,),
+                                                             Re=100.0, collision="bgk", storage=st)]))
     return out
 
 
