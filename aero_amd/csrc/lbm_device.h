@@ -284,6 +284,42 @@ template <> struct Storage<ST_F16S> {
   }
   AERO_PLANE_ACCESS
 };
+// Stochastic rounding to BF16.  With an 8-bit significand, round-to-nearest
+// stalls: a relaxation step smaller than half a unit in the last place leaves
+// a population unchanged, step after step, and the errors pile up in one
+// direction (the 128 x 64 x 64 sphere's drag +1.07% after 5000 steps).  Adding
+// 16 uniform random bits below the BF16 cut and truncating rounds up with
+// probability equal to the discarded fraction, so the error has zero mean
+// (the same drag then within 0.002%).  The bits come from an integer hash of
+// (cell, step number, population), so a case computes the same bits alone, in
+// a batch or chunked differently: a hash of the cell and step seeds a 32-bit
+// linear congruential sequence (Numerical Recipes' constants) whose high 16
+// bits serve the cell's populations in turn.  mix32 is "lowbias32" from
+// C. Wellons' hash prospector.
+__host__ __device__ inline unsigned mix32(unsigned x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+__host__ __device__ inline unsigned sr_seed(int cell, unsigned step) {
+  return mix32((unsigned)cell * 0x9e3779b1u ^ mix32(step));
+}
+// the next state of the sequence; its high 16 bits are the random bits
+__host__ __device__ inline unsigned sr_next(unsigned h) { return h * 1664525u + 1013904223u; }
+// v rounded to BF16 with the random bits in the high half of r: gfx950's
+// v_cvt_sr_bf16_f32, which (checked bit for bit over normal, subnormal,
+// near-overflow, infinite and NaN inputs) returns (bits(v) + (r >> 16)) >> 16,
+// NaN kept NaN.
+__device__ inline unsigned short bf16_round_sr(float v, unsigned r) {
+  typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+  bf16x2_t o = {};
+  o = __builtin_amdgcn_cvt_sr_bf16_f32(o, v, r, false);
+  return __builtin_bit_cast(unsigned short, o[0]);
+}
+
 template <> struct Storage<ST_BF16S> {
   using T = __hip_bfloat16;
   using C = float;

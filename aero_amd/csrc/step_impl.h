@@ -42,9 +42,10 @@ struct StepDyn {
   AERO_GLOBAL double* hist_prev;    // coefficient row of the previous step
   int g_bc_skip;         // skip the scalar wall values (state just set from a raw field)
   int reduce_prev;       // block 0 reduces the previous step's force sums
+  unsigned rstep;        // step number (mod 2^32): seeds the BF16 stochastic rounding
 };
 __device__ __forceinline__ StepDyn dyn_of(const StepParams& p) {
-  return StepDyn{p.phase, p.plane, p.hist_prev, p.g_bc_skip, p.reduce_prev};
+  return StepDyn{p.phase, p.plane, p.hist_prev, p.g_bc_skip, p.reduce_prev, 0u};
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -680,8 +681,16 @@ __device__ __forceinline__ void scalar_apply(const StepParams& p, const StepDyn&
   if (write_g) {
     const C om = (C)p.omega_T;
     AERO_GLOBAL typename S::T* grow = srow(p, (AERO_GLOBAL typename S::T*)p.gout, y, z);
+    // BF16: stochastic rounding as the populations', with bits of its own
+    // (seeded from the complemented step number, hashed per population)
+    const unsigned gseed = ST == ST_BF16S ? sr_seed(cell, ~dyn.rstep) : 0u;
     auto put = [&](int i, C gp) {
-      S::st((typename S::T*)(grow + (unsigned)(i * p.xp)), (unsigned)x, gp, C(L::W[i]) * (C)p.gshift);
+      if constexpr (ST == ST_BF16S) {
+        ((unsigned short*)(grow + (unsigned)(i * p.xp)))[x] =
+            bf16_round_sr(gp - C(L::W[i]) * (C)p.gshift, mix32(gseed + (unsigned)i));
+      } else {
+        S::st((typename S::T*)(grow + (unsigned)(i * p.xp)), (unsigned)x, gp, C(L::W[i]) * (C)p.gshift);
+      }
     };
     // geq_i = w T (1 + 3 e_i.u): a pair (i, opp i) shares w T and +-3 w T e_i.u
     put(0, solid ? C(L::W[0]) * T : g[0] - om * (g[0] - C(L::W[0]) * T));
@@ -1026,9 +1035,10 @@ __device__ __forceinline__ void collide(const double* mrt_s, const double* minv_
 // pass -- write the post-boundary populations and velocity (KMODE 1).
 // `sb` is the storage element of the wave's first cell: its row's base plus
 // the wave's first x (wave-uniform), so a lane's cell sits at sb + lane.
+
 template <class L, int COLL, int ST, int KMODE>
 __device__ __forceinline__ void finish_cell(const StepParams& p, int cell, size_t sb, bool solid,
-                                            const typename Storage<ST>::C fn[L::Q]) {
+                                            const typename Storage<ST>::C fn[L::Q], unsigned rstep = 0) {
   using S = Storage<ST>;
   using C = typename S::C;
   using T = typename S::T;
@@ -1052,6 +1062,19 @@ __device__ __forceinline__ void finish_cell(const StepParams& p, int cell, size_
     // register across the whole step; the run bases are scalar
     const unsigned lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
     AERO_GLOBAL T* wbase = (AERO_GLOBAL T*)p.fout + sb;
+    if constexpr (ST == ST_BF16S) {
+      // BF16 rounds stochastically (bf16_round_sr), bits from (cell, step, population)
+      unsigned h = sr_seed(cell, rstep);
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        h = sr_next(h);
+        const unsigned short b = bf16_round_sr(out[i] - (float)p.fsh[fsh_class<L>(i)], h);
+        unsigned short* a = (unsigned short*)scalar_base(wbase + (unsigned)(i * p.xp)) + lane;
+        if (p.nt_store) __builtin_nontemporal_store(b, a);
+        else *a = b;
+      }
+      return;
+    }
     if (p.nt_store) {
 #pragma unroll
       for (int i = 0; i < Q; ++i)
@@ -1289,7 +1312,7 @@ __device__ __forceinline__ void step_rows_group(const StepParams& p, const StepD
     if (x == p.nx - 1 && p.outlet_mode == OUT_PRESSURE) pressure_outlet_2d<C>(p, fn);
     sponge_and_walls<L, C>(p, x, y, fn);
     finish_cell<L, COLL, ST, KMODE>(p, y * p.nx + x, (size_t)(unsigned)y * (unsigned)p.qxp + xw, opaque_false(),
-                                    fn);
+                                    fn, dyn.rstep);
   }
 }
 
@@ -1376,7 +1399,7 @@ struct RowsPerWave {
         C_ fq_[L::Q];                                                                                      \
         gather<L, ST>(p, B, x, y, z, (C_)p.u0, fq_);                                                       \
         if (L::DIM == 3 && p.body_force_on) body_force<L, C_>(p, fq_);                                     \
-        finish_cell<L, COLL, ST, KMODE>(p, cell, sb_, opaque_false(), fq_);                                \
+        finish_cell<L, COLL, ST, KMODE>(p, cell, sb_, opaque_false(), fq_, dyn.rstep);                     \
         break;                                                                                             \
       }                                                                                                    \
     }                                                                                                      \
@@ -1402,7 +1425,7 @@ struct RowsPerWave {
     if (rec >= 0) link_post_sum<L, C_>(p, rec, fn);                                                        \
     /* the next step's IBM forcing acts on this post-boundary state (solver.py:280-285) */                 \
     if (p.ibm_on && (flag & FLAG_IBM)) ibm_force<L, C_>((C_)p.ibm_coeff, fn);                             \
-    finish_cell<L, COLL, ST, KMODE>(p, cell, sb_, solid, fn);                                              \
+    finish_cell<L, COLL, ST, KMODE>(p, cell, sb_, solid, fn, dyn.rstep);                                   \
   } while (0)
 
 // One case, parameters by value (split LES step, recycling inlet).
@@ -1431,7 +1454,7 @@ void batch_step_kernel(const BatchCase* __restrict__ cases, const int2* __restri
   const StepParams& p = bc.p[k & 1];
   const StepDyn dyn{0.17 * (double)(bc.step0 + k + 1), bc.planes ? bc.planes + (size_t)3 * p.nrow * k : nullptr,
                     bc.hist + (size_t)NCOEF * (k > 0 ? k - 1 : 0), k == 0 ? bc.first_g_skip : 0,
-                    (k > 0 && p.nrec > 0) ? 1 : 0};
+                    (k > 0 && p.nrec > 0) ? 1 : 0, (unsigned)(bc.step0 + k + 1)};
   const int blk = t.y;
   AERO_STEP_BODY(L, COLL, ST, THERMAL, KMODE);
 }

@@ -10,9 +10,10 @@ and in float32 / shifted-half storage from the same seeded state:
   linear Bouzidi, u0 = 0.05, Re = 100; 5000 steps.
 
 Stated tolerances: mean Cd over the last 20% of the run within 0.5% of the
-FP64 run for FP32 and FP16-shifted storage, the forms this build claims.
-The BF16-shifted form (8-bit significand) is kept as an unclaimed option:
-its bound is 2%, and it measures about 1%.  2D Strouhal number within 0.5%
+FP64 run for FP32, FP16-shifted and BF16-shifted storage.  BF16 (8-bit
+significand) rounds its stores stochastically; with round-to-nearest it
+measured +1.07% on the sphere (C2) and +0.70% on the cylinder (C1), with
+stochastic rounding +0.002% and +0.07%.  2D Strouhal number within 0.5%
 when the FP64 run sheds (a peak in the physical band 0.1-0.3).  For the 3D
 sphere the final density and velocity fields are also compared with the
 FP64 run's (max and RMS, relative to the field's largest magnitude).
@@ -24,12 +25,13 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-CD_TOL = {"f32": 5e-3, "f16": 5e-3, "bf16": 2e-2}  # |Cd - Cd_f64| / Cd_f64, mean over the last 20%
+CD_TOL = {"f32": 5e-3, "f16": 5e-3, "bf16": 5e-3}  # |Cd - Cd_f64| / Cd_f64, mean over the last 20%
 # final rho / u fields of the 3D sphere vs FP64: (max, rms) relative to the
 # largest |rho| and |u| of the FP64 field
 # (measured on MI355X, max / rms over rho, ux, uy, uz: FP32 9.7e-5 / 3.9e-5,
-# FP16 2.9e-3 / 7.9e-5, BF16 8.2e-2 / 8.4e-3)
-C2_FIELD_TOL = {"f32": (3e-4, 1.2e-4), "f16": (9e-3, 2.5e-4), "bf16": (2.5e-1, 2.5e-2)}
+# FP16 2.9e-3 / 7.9e-5, BF16 with stochastic rounding ux 9.6e-3 / 3.7e-4;
+# with round-to-nearest it was 8.2e-2 / 8.4e-3)
+C2_FIELD_TOL = {"f32": (3e-4, 1.2e-4), "f16": (9e-3, 2.5e-4), "bf16": (3e-2, 1.5e-3)}
 
 
 def _c1(storage, steps=20000):
@@ -96,9 +98,9 @@ def _c1_cached(storage):
     return _C1_RUNS[storage]
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
 def test_c1_cd_and_strouhal_within_half_percent_of_fp64(storage):
-    """FP32 and FP16-shifted storage."""
+    """FP32, FP16-shifted and BF16-shifted storage."""
     cd64, st64, h64 = _c1_cached("f64")
     cd, st, h = _c1_cached(storage)
     print(f"[gpu] C1 Cd f64 {cd64:.5f} {storage} {cd:.5f}  St f64 {st64} {storage} {st}", flush=True)
