@@ -46,7 +46,9 @@ class Solver(DeviceSolver):
 
         storage     -- population format on the device: "f64" (reference
                        precision), "f32", "f16" or "bf16" (shifted half storage,
-                       FP32 arithmetic).
+                       FP32 arithmetic; "bf16" rounds its stores
+                       stochastically, from bits that depend only on the
+                       cell, the step number and the population).
         trt_variant -- "reference" uses the reference's TRT rates
                        (trt2d.py:8-18), which are unstable for tau below ~0.76;
                        "lambda" uses the standard magic-parameter TRT
