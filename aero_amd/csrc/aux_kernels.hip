@@ -67,6 +67,30 @@ hipError_t launch_materialize(int q, int st, bool thermal, const StepParams& p, 
   return q == 9 ? mat_l<D2Q9>(st, thermal, p, s) : mat_l<D3Q19>(st, thermal, p, s);
 }
 
+template <class L>
+static hipError_t chk_l(int st, bool thermal, const StepParams* ps, const int2* tiles, int ntiles, hipStream_t s) {
+  const dim3 g((unsigned)ntiles);
+#define AERO_CASE(ST)                                                                  \
+  case ST:                                                                             \
+    if (thermal) check_batch_kernel<L, ST, true><<<g, BLOCK, 0, s>>>(ps, tiles);       \
+    else check_batch_kernel<L, ST, false><<<g, BLOCK, 0, s>>>(ps, tiles);              \
+    return hipGetLastError();
+  switch (st) {
+    AERO_CASE(ST_F64)
+    AERO_CASE(ST_F32)
+    AERO_CASE(ST_F16S)
+    AERO_CASE(ST_BF16S)
+  }
+#undef AERO_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_check_batch(int q, int st, bool thermal, const StepParams* ps, const int2* tiles, int ntiles,
+                              hipStream_t s) {
+  if (ntiles <= 0) return hipSuccess;
+  return q == 9 ? chk_l<D2Q9>(st, thermal, ps, tiles, ntiles, s) : chk_l<D3Q19>(st, thermal, ps, tiles, ntiles, s);
+}
+
 // ---------------------------------------------------------------- collide
 template <class L, int COLL>
 static hipError_t col_lc(int st, const CollideParams& p, hipStream_t s) {
@@ -535,8 +559,8 @@ hipError_t launch_reduce_forces(const ForceReduce& r, hipStream_t s) {
   return hipGetLastError();
 }
 
-__global__ __launch_bounds__(BLOCK) void reduce_check_kernel(const double* part, int nblocks, double* out) {
-  __shared__ double red[NCHECK][BLOCK];
+__device__ __forceinline__ void reduce_check_block(const double* part, int nblocks, double* out,
+                                                   double (*red)[BLOCK]) {
   double a[NCHECK] = {0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < nblocks; b += BLOCK) {
     a[0] = fmax(a[0], part[(size_t)b * NCHECK + 0]);
@@ -564,8 +588,26 @@ __global__ __launch_bounds__(BLOCK) void reduce_check_kernel(const double* part,
   }
 }
 
+__global__ __launch_bounds__(BLOCK) void reduce_check_kernel(const double* part, int nblocks, double* out) {
+  __shared__ double red[NCHECK][BLOCK];
+  reduce_check_block(part, nblocks, out, red);
+}
+
+// One block per case: the same fixed-order reduction of the case's partials.
+__global__ __launch_bounds__(BLOCK) void reduce_check_batch_kernel(const StepParams* ps, double* out) {
+  __shared__ double red[NCHECK][BLOCK];
+  const StepParams& p = ps[blockIdx.x];
+  reduce_check_block(p.check_part, p.nblocks, out + (size_t)NCHECK * blockIdx.x, red);
+}
+
 hipError_t launch_reduce_check(const double* part, int nblocks, double* out, hipStream_t s) {
   reduce_check_kernel<<<1, BLOCK, 0, s>>>(part, nblocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_check_batch(const StepParams* ps, int n, double* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  reduce_check_batch_kernel<<<(unsigned)n, BLOCK, 0, s>>>(ps, out);
   return hipGetLastError();
 }
 

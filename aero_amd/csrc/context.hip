@@ -10,6 +10,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "aero_hip.h"
@@ -561,12 +562,19 @@ struct aero_ctx {
   // doubles), queued on stream s: per-block partials as the state is
   // re-derived (3D: the z mid-plane only, solver3d.py:608-615), then one
   // block sums them in block order.
-  void check_stats(hipStream_t s, double* out) {
-    checkpart.alloc(sizeof(double) * NCHECK * nblocks);
+  // Parameters of the stability check's pass over the last completed step
+  // (materialize, out_mode 2): NCHECK block partials per block into part.
+  StepParams check_params(hipStream_t s, double* part) {
     StepParams p = materialize_params(s);
     p.out_mode = 2;
     p.check_z = Q == 19 ? d.nz / 2 : -1;
-    p.check_part = (decltype(p.check_part))(checkpart.as<double>());
+    p.check_part = (decltype(p.check_part))part;
+    return p;
+  }
+
+  void check_stats(hipStream_t s, double* out) {
+    checkpart.alloc(sizeof(double) * NCHECK * nblocks);
+    const StepParams p = check_params(s, checkpart.as<double>());
     ck(launch_materialize(Q, d.storage, d.thermal != 0, p, s), "check launch");
     ck(launch_reduce_check(checkpart.as<double>(), nblocks, out, s), "check reduce");
   }
@@ -586,6 +594,16 @@ struct aero_ctx {
     gcache_valid = false;
   }
 };
+
+// Wait for the work queued on a context's own stream.  An idle stream (the
+// usual case between batched chunks) costs a status query, not a round trip
+// through the device as a synchronisation does.
+static void drain(aero_ctx* c) {
+  const hipError_t q = hipStreamQuery(c->stream);
+  if (q == hipSuccess) return;
+  if (q != hipErrorNotReady) ck(q, "stream query");
+  ck(hipStreamSynchronize(c->stream), "context sync");
+}
 
 bool StepRunner::Group::holds(const aero_ctx* c) const {
   return q == c->d.q && coll == c->d.collision && st == c->d.storage && th == (c->d.thermal != 0) &&
@@ -1003,6 +1021,9 @@ struct aero_batch {
   hipStream_t stream = nullptr;
   StepRunner runner;
   DevBuf checkout;  // aero_batch_check: NCHECK doubles per case
+  DevBuf checkpart;  // aero_batch_check: block partials of every case
+  DevBuf checkpar;   // aero_batch_check: the cases' parameters, then the tile list
+  std::vector<std::pair<const aero_ctx*, int>> check_sig;  // the cases of the tile list held in checkpar
 };
 
 int aero_batch_create(aero_batch** out) {
@@ -1026,6 +1047,8 @@ int aero_batch_destroy(aero_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   b->runner.release();
   b->checkout.release();
+  b->checkpart.release();
+  b->checkpar.release();
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return 0;
@@ -1052,7 +1075,7 @@ int aero_batch_step(aero_batch* b, int32_t n, aero_ctx* const* ctxs, int32_t nst
       s0[i] = step0s ? step0s[i] : 0;
       pl[i] = planes ? planes[i] : nullptr;
       hs[i] = hists ? hists[i] : nullptr;
-      ck(hipStreamSynchronize(c->stream), "context sync");  // work queued on the context's own stream
+      drain(c);  // work queued on the context's own stream
     }
     run_cases(cs, nsteps, s0.data(), pl.data(), hs.data(), b->stream, b->runner);
   });
@@ -1959,17 +1982,82 @@ int aero_batch_check(aero_batch* b, int32_t n, aero_ctx* const* ctxs, double* st
   return guarded([&] {
     if (n <= 0) return;
     ck(hipSetDevice(b->device), "hipSetDevice");
-    b->checkout.alloc(sizeof(double) * NCHECK * (size_t)n);
+    // One pass per (lattice, storage, scalar) group over the tiles (case,
+    // block) of its cases, then one reduction block per case: the blocks and
+    // the reduction order of aero_check, so each row equals that case's own.
+    long long nparts = 0;
+    std::vector<long long> part0((size_t)n);
     for (int i = 0; i < n; ++i) {
       aero_ctx* c = ctxs[i];
       if (!c) throw std::invalid_argument("null context in batch");
       if (c->device != b->device) throw std::invalid_argument("every context of a batch must be on its device");
-      ck(hipStreamSynchronize(c->stream), "context sync");  // work queued on the context's own stream
-      c->check_stats(b->stream, b->checkout.as<double>() + (size_t)NCHECK * i);
+      part0[i] = nparts;
+      nparts += c->nblocks;
     }
+    b->checkout.alloc(sizeof(double) * NCHECK * (size_t)n);
+    b->checkpart.alloc(sizeof(double) * NCHECK * (size_t)nparts);
+    std::vector<StepParams> ps((size_t)n);
+    std::vector<int> order((size_t)n);
+    for (int i = 0; i < n; ++i) {
+      aero_ctx* c = ctxs[i];
+      drain(c);  // work queued on the context's own stream
+      ps[i] = c->check_params(b->stream, b->checkpart.as<double>() + (size_t)NCHECK * part0[i]);
+      order[i] = i;
+    }
+    auto key = [&](int i) {
+      const aero_ctx* c = ctxs[i];
+      return std::make_tuple(c->Q, c->d.storage, c->d.thermal != 0);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return key(a) < key(c); });
+    struct Group {
+      int rep;        // a case of the group
+      size_t t0, t1;  // its tiles
+    };
+    std::vector<Group> groups;
+    for (size_t j = 0; j < order.size(); ++j) {
+      const int i = order[j];
+      if (j == 0 || key(i) != key(order[j - 1])) groups.push_back({i, 0, 0});
+    }
+    // the tile list depends only on the cases (their order and sizes): kept
+    // on the device while the same cases are checked again
+    std::vector<std::pair<const aero_ctx*, int>> sig((size_t)n);
+    for (int i = 0; i < n; ++i) sig[i] = {ctxs[i], ctxs[i]->nblocks};
+    const size_t pbytes = sizeof(StepParams) * (size_t)n;
+    const size_t toff = (pbytes + 255) / 256 * 256;
+    std::vector<int2> tiles;
+    {
+      size_t nt = 0, g = 0;
+      const bool fresh = sig != b->check_sig;
+      if (fresh) tiles.resize((size_t)nparts);
+      for (size_t j = 0; j < order.size(); ++j) {
+        const int i = order[j];
+        if (j > 0 && key(i) != key(order[j - 1])) groups[++g].t0 = nt;
+        if (fresh)
+          for (int blk = 0; blk < ctxs[i]->nblocks; ++blk) tiles[nt + blk] = make_int2(i, blk);
+        nt += ctxs[i]->nblocks;
+        groups[g].t1 = nt;
+      }
+      if (fresh) {
+        b->check_sig.clear();
+        b->checkpar.alloc(toff + sizeof(int2) * tiles.size());
+      }
+    }
+    const StepParams* dps = (const StepParams*)b->checkpar.p;
+    const int2* dtiles = (const int2*)((char*)b->checkpar.p + toff);
+    ck(hipMemcpyAsync(b->checkpar.p, ps.data(), pbytes, hipMemcpyHostToDevice, b->stream), "upload check params");
+    if (!tiles.empty())
+      ck(hipMemcpyAsync((void*)dtiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, b->stream),
+         "upload check tiles");
+    for (const Group& g : groups) {
+      const aero_ctx* c = ctxs[g.rep];
+      ck(launch_check_batch(c->Q, c->d.storage, c->d.thermal != 0, dps, dtiles + g.t0, (int)(g.t1 - g.t0), b->stream),
+         "check launch");
+    }
+    ck(launch_reduce_check_batch(dps, n, b->checkout.as<double>(), b->stream), "check reduce");
     ck(hipMemcpyAsync(stats, b->checkout.p, sizeof(double) * NCHECK * (size_t)n, hipMemcpyDeviceToHost, b->stream),
        "download");
-    ck(hipStreamSynchronize(b->stream), "sync");
+    ck(hipStreamSynchronize(b->stream), "sync");  // the host vectors above stay alive until here
+    if (!tiles.empty()) b->check_sig = std::move(sig);
   });
 }
 

@@ -125,6 +125,12 @@ hipError_t launch_check_f64(int q, int nx, int ny, int nz, int zc, const double*
                             hipStream_t s);
 hipError_t launch_reduce_forces(const ForceReduce& r, hipStream_t s);
 hipError_t launch_reduce_check(const double* part, int nblocks, double* out, hipStream_t s);
+// Batched form over n cases whose parameters (out_mode 2) sit in device memory:
+// the block partials of every tile (case, block) of one (lattice, storage,
+// scalar) group, then one fixed-order reduction per case into out[NCHECK * i].
+hipError_t launch_check_batch(int q, int st, bool thermal, const StepParams* ps, const int2* tiles, int ntiles,
+                              hipStream_t s);
+hipError_t launch_reduce_check_batch(const StepParams* ps, int n, double* out, hipStream_t s);
 hipError_t launch_reduce_recycling(double* recyc, int nrow, double target_u0, hipStream_t s);
 
 }  // namespace aero

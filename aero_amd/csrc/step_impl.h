@@ -1488,13 +1488,11 @@ hipError_t launch_step(const StepParams& p, hipStream_t s) {
 // the last completed step (reads the previous buffer side).  out_mode 0
 // writes the f64 field, 1 writes rho/ux/uy/uz, 2 reduces the stability check.
 template <class L, int ST, bool THERMAL>
-__global__ __launch_bounds__(BLOCK) void materialize_kernel(StepParams p) {
+__device__ __forceinline__ void materialize_block(const StepParams& p, int lb, double (*red)[NCHECK]) {
   using S = Storage<ST>;
   using C = typename S::C;
   using T = typename S::T;
   constexpr int Q = L::Q;
-  __shared__ double red[BLOCK / 64][NCHECK];
-  const int lb = blockIdx.x;
   const long long c64 = (long long)lb * BLOCK + threadIdx.x;
   double v_nan = 0, v_neg = 0, v_umax = 0, v_rho = 0, v_cnt = 0;
   if (c64 < p.ncell) {
@@ -1572,6 +1570,23 @@ __global__ __launch_bounds__(BLOCK) void materialize_kernel(StepParams p) {
       for (int k = 0; k < NCHECK; ++k) p.check_part[(size_t)lb * NCHECK + k] = r[k];
     }
   }
+}
+
+template <class L, int ST, bool THERMAL>
+__global__ __launch_bounds__(BLOCK) void materialize_kernel(StepParams p) {
+  __shared__ double red[BLOCK / 64][NCHECK];
+  materialize_block<L, ST, THERMAL>(p, blockIdx.x, red);
+}
+
+// The stability check's block partials (out_mode 2) of many cases in one
+// launch: tile b is block `y` of case `x`, whose parameters sit in device
+// memory.  Each block computes what materialize_kernel's block does.
+template <class L, int ST, bool THERMAL>
+__global__ __launch_bounds__(BLOCK) void check_batch_kernel(const StepParams* __restrict__ ps,
+                                                            const int2* __restrict__ tiles) {
+  __shared__ double red[BLOCK / 64][NCHECK];
+  const int2 t = tiles[blockIdx.x];
+  materialize_block<L, ST, THERMAL>(ps[t.x], t.y, red);
 }
 
 // Collision from an explicit post-boundary field F (compute type): used after

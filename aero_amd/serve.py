@@ -33,6 +33,7 @@ The reference has no counterpart: it runs one case per process.
 from __future__ import annotations
 
 import math
+import sys
 import time
 from dataclasses import dataclass, field
 from typing import Callable, List, Optional, Sequence
@@ -236,6 +237,10 @@ class _Live:
     rows: Optional[np.ndarray] = None  # the last chunk's rows, not yet in the histories
 
 
+# Thread switch interval (seconds) while Server.serve runs.
+GIL_SWITCH = 2e-4
+
+
 class Server:
     """Continuous batching of wind-tunnel cases on one device."""
 
@@ -319,17 +324,22 @@ class Server:
             self.on_result(res)
         return res
 
-    def _join_built(self, building: list, live: list, clock, t0: float) -> None:
-        """Move finished builds into the live set (with their history ring)."""
+    def _build(self, spec: CaseSpec):
+        """A builder thread's work: the case's solver with its history ring."""
+        solver = spec.build()
+        if self.device_stats and hasattr(solver, "_ctx"):
+            from .stationarity import enable_ring
+            w = max(self.stop_kw["convergence_window"], self.stop_kw["strouhal_window"])
+            enable_ring(solver, w + self.chunk)  # room for the chunk stepped during the check
+        return solver
+
+    @staticmethod
+    def _join_built(building: list, live: list, clock, t0: float) -> None:
+        """Move finished builds into the live set."""
         for b in [b for b in building if b[0].done()]:
             building.remove(b)
             fut, arr, rid, spec, _ = b
-            solver = fut.result()
-            if self.device_stats and hasattr(solver, "_ctx"):
-                from .stationarity import enable_ring
-                w = max(self.stop_kw["convergence_window"], self.stop_kw["strouhal_window"])
-                enable_ring(solver, w + self.chunk)  # room for the chunk stepped during the check
-            live.append(_Live(rid, spec, solver, arr, clock() - t0))
+            live.append(_Live(rid, spec, fut.result(), arr, clock() - t0))
 
     @staticmethod
     def _chunk(batch, solvers, n):
@@ -371,7 +381,15 @@ class Server:
         history cut at the end of chunk j, exactly what it would have been
         alone; its speculative chunk j+1 is discarded.  A case whose history
         has reached max_steps is decided before the next chunk instead (it
-        stops whatever the checks find), so it costs no speculative chunk."""
+        stops whatever the checks find), so it costs no speculative chunk.
+        A retired case's Result is written, and its device memory released,
+        while the next chunk steps (at once when admission waits for memory).
+
+        While it runs, the interpreter's thread switch interval is cut to
+        GIL_SWITCH seconds (restored on return): the thread that drives the
+        device needs the GIL briefly around each call, and waiting out the
+        default 5 ms behind the host's bookkeeping would leave the device
+        idle each time."""
         from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
         if any(r[1].shape == "polygon" for r in requests):
             import matplotlib.path  # noqa: F401  (imported once here, not raced by the builder threads)
@@ -383,6 +401,27 @@ class Server:
         t0 = clock()
         pool = ThreadPoolExecutor(max_workers=1)
         builders = ThreadPoolExecutor(max_workers=self.build_workers)
+        switch = sys.getswitchinterval()
+        sys.setswitchinterval(min(switch, GIL_SWITCH))
+        retired: list = []  # (live case, reason, history length): decided, not yet written up
+
+        # The cases retired at a decision are written up while the next chunk
+        # steps, and their device memory is released last (a free of device
+        # memory waits for the device's work in flight).
+        def report():
+            now = clock() - t0
+            for lv, reason, upto in retired:
+                done.append(self._result(lv, reason, now, upto))
+
+        def release():
+            for lv, _, _ in retired:
+                lv.solver.close()
+            retired.clear()
+
+        def write_up():
+            report()
+            release()
+
         try:
             while waiting or live or building:
                 # finished builds join first, so their memory is counted once
@@ -397,17 +436,15 @@ class Server:
                     if free is None:
                         free = self._free()
                     if (live or building) and free - pending - need < self.mem_reserve:
+                        if retired:  # release the retired cases' memory now
+                            write_up()
+                            free = self._free()
+                            continue
                         break  # wait for a live case to retire
                     waiting.pop(0)
-                    building.append((builders.submit(spec.build), arr, rid, spec, need))
+                    building.append((builders.submit(self._build, spec), arr, rid, spec, need))
                     pending += need
                 self._join_built(building, live, clock, t0)
-                if not live:
-                    if building:
-                        wait([b[0] for b in building], return_when=FIRST_COMPLETED)
-                    elif waiting:
-                        sleep(max(0.0, waiting[0][0] - (clock() - t0)))
-                    continue
                 # A case whose history has reached max_steps stops at this check
                 # whatever it finds, so it is decided now rather than stepped for
                 # a speculative chunk: its last chunk's rows are recorded, the
@@ -418,13 +455,15 @@ class Server:
                         self._take_rows(lv)
                     self._stability(at_limit, self._batch_obj().check_stats([lv.solver for lv in at_limit]))
                     stops = self._stops(at_limit)
-                    now = clock() - t0
                     for lv in at_limit:
-                        done.append(self._result(lv, self._done(lv, stops.get(id(lv), False)), now,
-                                                 len(lv.solver.Cd_history)))
-                        lv.solver.close()
+                        retired.append((lv, self._done(lv, stops.get(id(lv), False)), len(lv.solver.Cd_history)))
                         live.remove(lv)
                 if not live:
+                    write_up()
+                    if building:
+                        wait([b[0] for b in building], return_when=FIRST_COMPLETED)
+                    elif waiting:
+                        sleep(max(0.0, waiting[0][0] - (clock() - t0)))
                     continue
                 n = self.chunk
                 batch = self._batch_obj()
@@ -433,11 +472,14 @@ class Server:
                 # the device checks every case's state at this boundary (the end
                 # of its last chunk), then steps the next chunk
                 fut = pool.submit(self._chunk, batch, [lv.solver for lv in stepping], n)
-                # while it runs: the last chunk's rows join the histories and the
-                # stop test reads them
+                # while it runs: the cases retired at the boundary are written
+                # up, the last chunk's rows join the histories and the stop test
+                # reads them
+                report()
                 for lv in stepping:
                     self._take_rows(lv)
                 stops = self._stops(stepping)
+                release()
                 stats, rows = fut.result()
                 self.busy_seconds += clock() - tb
                 # decisions at the boundary: check_stability first, as the
@@ -446,20 +488,20 @@ class Server:
                 # chunk is discarded
                 checked = [k for k, lv in enumerate(stepping) if lv.solver.Cd_history]
                 self._stability([stepping[k] for k in checked], stats[checked])
-                now = clock() - t0
                 for lv, r in zip(stepping, rows):
                     reason = self._done(lv, stops.get(id(lv), False)) if lv.solver.Cd_history else None
                     if reason is not None:
-                        done.append(self._result(lv, reason, now, len(lv.solver.Cd_history)))
-                        lv.solver.close()
+                        retired.append((lv, reason, len(lv.solver.Cd_history)))
                         live.remove(lv)
                     else:
                         lv.rows = r
                         lv.steps += n
                         self.steps_done += n * lv.spec.cells
+            write_up()
         finally:
             builders.shutdown(wait=True)
             pool.shutdown(wait=True)
+            sys.setswitchinterval(switch)
         return done
 
 
