@@ -170,6 +170,20 @@ struct StepRunner {
     const void* tiles = nullptr;
   };
   std::map<std::pair<int, int>, Captured> graphs;
+  // Streams beside the runner's own for kernel groups stepped concurrently:
+  // groups hold disjoint cases, so each group's steps need only their own
+  // order, and one group's launches fill the device while another's drain.
+  // Forked from and joined back to the runner's stream by events per chunk.
+  static constexpr int NSIDE = 3;
+  hipStream_t side[NSIDE] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[NSIDE] = {};
+  void ensure_side(int m) {
+    if (!fork_ev) ck(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), "hipEventCreate");
+    for (int i = 0; i < m && i < NSIDE; ++i) {
+      if (!side[i]) ck(hipStreamCreateWithFlags(&side[i], hipStreamNonBlocking), "hipStreamCreate");
+      if (!join_ev[i]) ck(hipEventCreateWithFlags(&join_ev[i], hipEventDisableTiming), "hipEventCreate");
+    }
+  }
   void drop_graphs() {
     for (auto& kv : graphs)
       if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
@@ -222,6 +236,17 @@ struct StepRunner {
       ev[b] = nullptr;
     }
     drop_graphs();
+    for (int i = 0; i < NSIDE; ++i) {
+      if (side[i]) {
+        (void)hipStreamSynchronize(side[i]);
+        (void)hipStreamDestroy(side[i]);
+      }
+      if (join_ev[i]) (void)hipEventDestroy(join_ev[i]);
+      side[i] = nullptr;
+      join_ev[i] = nullptr;
+    }
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    fork_ev = nullptr;
     tiles.release();
     ncase_cap = rows_cap = 0;
     tile_key.clear();
@@ -738,6 +763,12 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
   bool use_graph = !timed && !groups.empty();
   for (const StepRunner::Group& g : groups) use_graph = use_graph && g.tune.graph == 1;
   for (aero_ctx* c : cs) use_graph = use_graph && c->d.inlet != AERO_IN_RECYCLING;
+  // several kernel groups step on streams of their own (not under capture,
+  // timing events or the recycling prepass, which run on the one stream)
+  bool fork = !use_graph && !timed && groups.size() > 1;
+  for (aero_ctx* c : cs) fork = fork && c->d.inlet != AERO_IN_RECYCLING;
+  const int nside = fork ? std::min<int>((int)groups.size() - 1, StepRunner::NSIDE) : 0;
+  if (nside) R.ensure_side(nside);
   bool any_hist = false;
   for (int i = 0; i < n; ++i) any_hist |= hists[i] != nullptr;
   for (int s0 = 0, ci = 0; s0 < nsteps; s0 += CH, ++ci) {
@@ -771,14 +802,25 @@ static void run_cases(const std::vector<aero_ctx*>& cs, int nsteps, const long l
        "upload cases");
     const BatchCase* cases = R.cases[b].as<BatchCase>();
     auto launch_steps = [&]() {
+      if (nside) {
+        ck(hipEventRecord(R.fork_ev, stream), "fork event");
+        for (int j = 0; j < nside; ++j) ck(hipStreamWaitEvent(R.side[j], R.fork_ev, 0), "fork wait");
+      }
       for (int k = 0; k < nk; ++k) {
         for (aero_ctx* c : cs)
           if (c->d.inlet == AERO_IN_RECYCLING) c->recycling_prepass((c->side + k) & 1, stream);
-        for (const StepRunner::Group& g : groups)
-          ck(launch_batch_step(g.q, g.coll, g.st, g.th, cases, R.tiles.as<int2>() + g.toff, g.ntiles, k, stream,
-                               timed ? cs[0]->ev_pool[2 * k] : nullptr, timed ? cs[0]->ev_pool[2 * k + 1] : nullptr,
-                               g.les != 0),
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+          const StepRunner::Group& g = groups[gi];
+          const int j = nside ? (int)(gi % (size_t)(nside + 1)) : 0;  // 0: the runner's stream
+          ck(launch_batch_step(g.q, g.coll, g.st, g.th, cases, R.tiles.as<int2>() + g.toff, g.ntiles, k,
+                               j ? R.side[j - 1] : stream, timed ? cs[0]->ev_pool[2 * k] : nullptr,
+                               timed ? cs[0]->ev_pool[2 * k + 1] : nullptr, g.les != 0),
              "step launch");
+        }
+      }
+      for (int j = 0; j < nside; ++j) {
+        ck(hipEventRecord(R.join_ev[j], R.side[j]), "join event");
+        ck(hipStreamWaitEvent(stream, R.join_ev[j], 0), "join wait");
       }
     };
     if (use_graph) {
