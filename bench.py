@@ -474,13 +474,15 @@ class _Progress:
 
 
 def serving(full=False):
-    """Batched 2D solves (the 256-case workload; a bounded sample unless
-    `full`), the same cases run one at a time through the plain single-case
-    API, and Poisson arrival streams of mixed 2D/3D requests; returns
-    solves/hour, p50/p99 time-to-result and aggregate MLUPS for each."""
+    """Batched 2D solves (the 256-case workload at its stated size, FP32 and
+    FP16-shifted), the same cases run one at a time through the plain
+    single-case API, and Poisson arrival streams of mixed 2D/3D requests (two
+    bounded rates unless `full`, then the 1-50 requests/s sweep at full
+    size); returns solves/hour, p50/p99 time-to-result and aggregate MLUPS
+    for each."""
     from aero_amd import serve
     out = {}
-    n_cases, max_steps = (256, 30000) if full else (64, 6000)
+    n_cases, max_steps = 256, 30000
     cases = serve.c4_cases(n_cases, seed=0, max_steps=max_steps)
     srv = serve.Server(chunk=500, on_result=_Progress("batched_2d"))
     t0 = time.perf_counter()
@@ -545,7 +547,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the extra storage forms")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving / batched-solves leg")
     ap.add_argument("--serve-full", action="store_true",
-                    help="serving leg on the full 256-case workload and the full arrival-rate sweep")
+                    help="serving leg with the full arrival-rate sweep of Poisson streams")
     ap.add_argument("--form", choices=sorted(FORMS) + sorted(LES_FORMS) + sorted(CYL2D_FORMS) + sorted(TRT_LES_FORMS)
                     + sorted(SLAB_FORMS) + sorted(BATCH_FORMS), default=None,
                     help="only time one storage form of the TRT sphere (prints one JSON line)")
