@@ -372,11 +372,12 @@ constexpr int BLOCK = 256;
 constexpr int ROW_ALIGN = 64;
 __host__ __device__ constexpr int padded_row(int nx) { return (nx + ROW_ALIGN - 1) / ROW_ALIGN * ROW_ALIGN; }
 
-// Row groups of the 2-byte D2Q9 BGK / TRT kernels (no scalar).  Their step
-// waves are listed per case (StepParams::witems): a plain item is one row's
-// 64 aligned columns; a group item (bit 31) covers the same columns in
-// RPW_GROUP consecutive rows, none of them holding a flagged cell, a domain
-// edge, a sponge column or a wall row.  A group wave issues all RPW_GROUP x
+// Row groups of the 2-byte D2Q9 kernels (no scalar).  Their step waves are
+// listed per case (StepParams::witems): a plain item is one row's 64
+// aligned columns; a group item (bit 31) covers the same columns in
+// RPW_GROUP consecutive rows, none of them holding a flagged cell (the inlet
+// column, a zero-gradient or pressure outlet, sponge columns and the wall
+// rows may be among them).  A group wave issues all RPW_GROUP x
 // 9 two-byte loads before it collides its first row: a one-row wave keeps
 // only 9 x 128 bytes in flight, too few to cover HBM latency at the
 // occupancy the kernel reaches (a pull + BGK probe on 8192 x 2048 ran at
