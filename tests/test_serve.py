@@ -180,3 +180,20 @@ def test_server_holds_requests_until_memory_frees():
     # live together depends on how the builder threads are scheduled: a build
     # in progress counts both in the free memory and as pending)
     assert 1 <= _CountedSpec.peak <= 2 and max(batch.calls) <= 2 and _CountedSpec.live == 0
+
+
+def test_server_restores_the_thread_switch_interval_and_closes_every_case():
+    """serve() lowers the interpreter's switch interval while it runs and
+    puts it back; every case it built is closed by the time it returns,
+    including those retired at its last decision."""
+    import sys
+    clock = [0.0]
+    batch = _FakeBatch(clock)
+    _CountedSpec.live = _CountedSpec.peak = 0
+    before = sys.getswitchinterval()
+    srv = serve.Server(chunk=100, max_live=8, mem_reserve=0, batch=batch, free_memory=lambda: 1 << 40)
+    reqs = [(0.0, _CountedSpec(seed=0, max_steps=100 * (k % 3 + 1))) for k in range(6)]
+    res = srv.serve(reqs, clock=lambda: clock[0], sleep=lambda dt: clock.__setitem__(0, clock[0] + dt))
+    assert sys.getswitchinterval() == before
+    assert len(res) == 6 and _CountedSpec.live == 0
+    assert sorted(r.steps for r in res) == [100, 100, 200, 200, 300, 300]
